@@ -1,0 +1,208 @@
+// atvh_internal.hpp — shared definitions of libatv_hip.so (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "atvh.h"
+
+namespace atvh {
+
+// include/atomiverse/core/units.h:138,165-166,173 (metal units)
+constexpr double kB_ev_per_k = 8.617333262e-5;
+constexpr double force_to_accel = 1.0 / 103.6427;
+constexpr double mass_v2_to_energy = 103.6427;
+constexpr double coulomb = 14.399645;
+
+constexpr int kMaxTypes = 16;
+constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
+constexpr int kMaxGrid = 8192;    // cap for grid-stride kernels that may be predicated off
+constexpr int kPartialStride = 8; // doubles per block partial (pe, vir[6], spare)
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define ATVH_HIP(call)                                                                  \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            throw ::atvh::Error(ATVH_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Device-resident control block of one system. Written only by device code
+// during a run; the host reads it at synchronisation points.
+struct DevCtl {
+    double xi;        // Nose-Hoover thermostat variable
+    double nh_scale;  // NH velocity scale not yet applied to v (lazy)
+    double pe;        // last potential energy
+    double vir[6];    // last pair virial (Voigt xx yy zz xy xz yz)
+    double ke;        // kinetic energy at end of the last step
+    double ke_half;   // kinetic energy after the last kick (NH input)
+    int disp_flag;    // some atom moved more than skin/2 since the last build
+    int rebuild;      // rebuild decided for the current step
+    int ago;          // steps since the last rebuild (LAMMPS `ago`)
+    int nrebuilds;    // total builds including the initial one
+    int overflow;     // neighbor capacity exceeded (count of atoms)
+    int max_nbr;      // max neighbors of the last build
+    int always;       // constant 1: predicate for unconditional launches
+    int pad_;
+    long long nbr_total; // full-list entries of the last build
+};
+
+struct TimerSlot {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    double total_ms = 0.0;
+    int64_t launches = 0;
+};
+
+} // namespace atvh
+
+struct atvh_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool timing = false;
+    std::map<std::string, atvh::TimerSlot> timers;
+    std::vector<hipEvent_t> event_pool;
+};
+
+struct atvh_system {
+    atvh_ctx* ctx = nullptr;
+    int64_t n = 0;
+    int64_t npad = 0; // column stride of the ELL neighbor list
+    double box[3] = {0, 0, 0};
+
+    // per-atom device arrays, cell-sorted order
+    double *x = nullptr, *y = nullptr, *z = nullptr;
+    double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    double *fx = nullptr, *fy = nullptr, *fz = nullptr;
+    double* q = nullptr;
+    int* type = nullptr;
+    int* tag = nullptr; // frame row of the atom now at this slot
+    // permutation scratch (same layout)
+    double *ax = nullptr, *ay = nullptr, *az = nullptr;
+    double *avx = nullptr, *avy = nullptr, *avz = nullptr;
+    double *afx = nullptr, *afy = nullptr, *afz = nullptr;
+    double* aq = nullptr;
+    int* atype = nullptr;
+    int* atag = nullptr;
+    // x_save for the displacement check
+    double *xs = nullptr, *ys = nullptr, *zs = nullptr;
+
+    int ntypes = 1;
+    double mass_t[atvh::kMaxTypes] = {1.0};
+    bool has_charge = false;
+
+    // neighbor list
+    bool nl_configured = false;
+    bool nl_built = false;
+    double cutoff = 0, skin = 0, cutoff_skin = 0;
+    int every = 1, delay = 10, check = 1;
+    int nc[3] = {0, 0, 0};
+    double cs[3] = {0, 0, 0};
+    int reach[3] = {0, 0, 0};
+    int ncells = 0;
+    bool use_half = true;
+    int nstencil = 0;
+    int* d_stencil = nullptr; // int4 {di,dj,dk,kind}
+    int* cell_of = nullptr;   // [n] cell of each slot (after sort: of the sorted slot)
+    int* acell_of = nullptr;
+    int* cell_count = nullptr;
+    int* cell_start = nullptr; // [ncells+1]
+    int* cell_fill = nullptr;
+    int* perm = nullptr;
+    int* scan_sums = nullptr;
+    int nscan_blocks = 0;
+    int* nbr = nullptr;      // ELL, column-major: nbr[k*npad + i]
+    int* numneigh = nullptr; // [n]
+    int cap = 0;
+
+    // pair style
+    bool pair_configured = false;
+    double lj_eps = 1, lj_sigma = 1, lj_cutoff = 0;
+    int lj_ntypes = 0;
+    std::vector<double> lj_eps_tab, lj_sigma_tab;
+    double* d_ljtab = nullptr; // per type pair: {lj1=24eps*s6*2? see pair kernel}, packed
+
+    // integrator
+    bool integ_configured = false;
+    double dt = 0;
+    int thermostat = 0;
+    double T_target = 0, tau_T = 0;
+    int64_t ndof = 0;
+
+    // reductions
+    double* partials = nullptr;
+    int npartials = 0;
+    atvh::DevCtl* ctl = nullptr;
+
+    // thermo record of the last md_run
+    double* d_thermo = nullptr;
+    int thermo_cap = 0;
+    int thermo_n = 0;
+};
+
+namespace atvh {
+
+inline int grid_for(int64_t n, int block = kBlock) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1)
+        g = 1;
+    return static_cast<int>(g);
+}
+inline int grid_capped(int64_t n, int block = kBlock) {
+    int g = grid_for(n, block);
+    return g > kMaxGrid ? kMaxGrid : g;
+}
+
+// Timing helpers (HIP events on the context stream, around one kernel family).
+struct ScopedTimer {
+    atvh_ctx* ctx;
+    const char* name;
+    hipEvent_t start = nullptr;
+    ScopedTimer(atvh_ctx* c, const char* nm);
+    ~ScopedTimer();
+};
+
+template <class T>
+void dev_alloc(T** p, size_t count) {
+    if (count == 0)
+        count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+    if (e != hipSuccess)
+        throw Error(ATVH_E_NOMEM, std::string("hipMalloc(") + std::to_string(count * sizeof(T)) +
+                                      " B): " + hipGetErrorString(e));
+}
+template <class T>
+void dev_free(T*& p) {
+    if (p) {
+        (void)hipFree(p);
+        p = nullptr;
+    }
+}
+
+// neighbor.hip
+void nl_setup_grid(atvh_system* s);
+void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces);
+void nl_enqueue_decide(atvh_system* s);
+void nl_free(atvh_system* s);
+
+// pair_lj.hip
+int pair_grid(const atvh_system* s);
+void pair_enqueue(atvh_system* s, int precision, bool eflag, bool vflag);
+void pair_free(atvh_system* s);
+
+// integrate.hip
+void integ_enqueue_initial(atvh_system* s);
+void integ_enqueue_final(atvh_system* s);
+void energy_enqueue_finalize(atvh_system* s, double* record_pe, double* record_ke);
+
+} // namespace atvh

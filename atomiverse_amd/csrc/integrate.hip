@@ -1,0 +1,447 @@
+// integrate.hip — velocity Verlet + single-chain Nose-Hoover, kinetic energy,
+// the displacement half of the rebuild check, and the fused MD loop.
+//
+// Reference order per step (Runtime::step_verlet_, src/cpu/driver/runtime.cpp:452-502):
+//   initial_integrate: VelocityVerlet (src/cpu/dynamics/velocity_verlet.cpp:41-55)
+//                      v += dtf/m f; x += dt v; wrap_orthorhombic (pbc.h:37-51)
+//                      then NoseHoover (nose_hoover.cpp:104-170):
+//                      KE = 0.5 mvv2e sum m v^2; xi += dt/2 (2KE - Ndof kB T)/Q;
+//                      v *= exp(-dt/2 xi)
+//   decide + rebuild, zero + force,
+//   final_integrate:   v += dtf/m f, then NoseHoover again; end_of_step: KE.
+//
+// MI355X schedule: the NH velocity scale is applied lazily — it is stored in
+// the control block and multiplied into v inside the next kick, which is the
+// same operation order as the reference ((v*s) + dtfm*f), so no extra pass
+// over v is spent on it. Each kick kernel also emits fixed-order per-block
+// kinetic partials; a one-block kernel reduces them, advances xi and (after
+// the initial kick) takes the neighbor-list decision, all on the device.
+// The initial kernel also performs the needs_rebuild displacement test
+// (cell_list.cpp:372-391): 4*|min_image(x - x_save)|^2 > skin^2 per atom.
+
+#include "atvh_internal.hpp"
+
+#include <cmath>
+#include <cstring>
+
+using namespace atvh;
+
+namespace {
+
+struct IntArgs {
+    double *x, *y, *z, *vx, *vy, *vz;
+    const double *fx, *fy, *fz;
+    const int* type;
+    const double *xs, *ys, *zs;
+    double dtfm[kMaxTypes];
+    double mass[kMaxTypes];
+    int ntypes;
+    double dt;
+    double lx, ly, lz;
+    double skin_sq;
+    int thermostat;
+    DevCtl* ctl;
+    double* ke_partials;
+};
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_down(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void block_sum_to(double v, double* out) {
+    __shared__ double sh[kBlock / 64];
+    v = wave_sum_d(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        sh[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w)
+            t += sh[w];
+        *out = t;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_initial(int64_t n, IntArgs a) {
+    const double s = a.thermostat ? a.ctl->nh_scale : 1.0;
+    double mv2 = 0.0;
+    int moved = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int t = a.ntypes > 1 ? a.type[i] : 0;
+        const double dtfm = a.dtfm[t];
+        double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
+        if (a.thermostat) {
+            vx *= s;
+            vy *= s;
+            vz *= s;
+        }
+        vx += dtfm * a.fx[i];
+        vy += dtfm * a.fy[i];
+        vz += dtfm * a.fz[i];
+        double x = a.x[i] + a.dt * vx;
+        double y = a.y[i] + a.dt * vy;
+        double z = a.z[i] + a.dt * vz;
+        x -= a.lx * floor(x / a.lx);
+        y -= a.ly * floor(y / a.ly);
+        z -= a.lz * floor(z / a.lz);
+        a.x[i] = x;
+        a.y[i] = y;
+        a.z[i] = z;
+        a.vx[i] = vx;
+        a.vy[i] = vy;
+        a.vz[i] = vz;
+        mv2 += a.mass[t] * (vx * vx + vy * vy + vz * vz);
+        double dx = x - a.xs[i], dy = y - a.ys[i], dz = z - a.zs[i];
+        dx -= a.lx * nearbyint(dx / a.lx);
+        dy -= a.ly * nearbyint(dy / a.ly);
+        dz -= a.lz * nearbyint(dz / a.lz);
+        const double dsq = dx * dx + dy * dy + dz * dz;
+        if (dsq * 4.0 > a.skin_sq)
+            moved = 1;
+    }
+    if (__any(moved) && (threadIdx.x & 63) == 0)
+        a.ctl->disp_flag = 1;
+    block_sum_to(mv2, &a.ke_partials[blockIdx.x]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_final(int64_t n, IntArgs a) {
+    const double s = a.thermostat ? a.ctl->nh_scale : 1.0;
+    double mv2 = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int t = a.ntypes > 1 ? a.type[i] : 0;
+        const double dtfm = a.dtfm[t];
+        double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
+        if (a.thermostat) {
+            vx *= s;
+            vy *= s;
+            vz *= s;
+        }
+        vx += dtfm * a.fx[i];
+        vy += dtfm * a.fy[i];
+        vz += dtfm * a.fz[i];
+        a.vx[i] = vx;
+        a.vy[i] = vy;
+        a.vz[i] = vz;
+        mv2 += a.mass[t] * (vx * vx + vy * vy + vz * vz);
+    }
+    block_sum_to(mv2, &a.ke_partials[blockIdx.x]);
+}
+
+// Fixed-order sum of nb partials (stride `stride`) over one 256-thread block.
+__device__ double block_fixed_sum(const double* p, int nb, int stride) {
+    __shared__ double sh[kBlock];
+    double t = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock)
+        t += p[(int64_t)b * stride];
+    sh[threadIdx.x] = t;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+struct PostArgs {
+    DevCtl* ctl;
+    const double* ke_partials;
+    int nb;
+    int thermostat;
+    double dt, Q, ndof_kT; // ndof * kB * T
+    // decide (initial phase only)
+    int decide, every, delay, check;
+    double* record_ke; // may be null
+    int step;
+    int final_phase;
+};
+
+// NoseHoover::launch scalar part + NeighborList::decide_reneighbor.
+__global__ __launch_bounds__(kBlock) void k_post(PostArgs a) {
+    const double mv2 = block_fixed_sum(a.ke_partials, a.nb, 1);
+    if (threadIdx.x != 0)
+        return;
+    DevCtl* c = a.ctl;
+    const double ke = 0.5 * mass_v2_to_energy * mv2;
+    double scale = 1.0;
+    if (a.thermostat) {
+        c->xi += (a.dt * 0.5) * (2.0 * ke - a.ndof_kT) / a.Q;
+        scale = exp(-a.dt * 0.5 * c->xi);
+        c->nh_scale = scale;
+    }
+    if (a.final_phase) {
+        c->ke = ke * scale * scale; // KineticEnergy at end_of_step (scaled v)
+        if (a.record_ke)
+            a.record_ke[a.step] = c->ke;
+    } else {
+        c->ke_half = ke;
+    }
+    if (a.decide) {
+        c->ago += 1;
+        const bool delay_ok = c->ago >= a.delay;
+        const bool every_ok = (c->ago % a.every) == 0;
+        const bool rb = delay_ok && every_ok && (!a.check || c->disp_flag != 0);
+        c->disp_flag = 0;
+        c->rebuild = rb ? 1 : 0;
+        if (rb) {
+            c->ago = 0;
+            c->nrebuilds += 1;
+            c->nbr_total = 0;
+            c->max_nbr = 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_energy_finalize(DevCtl* ctl, const double* partials,
+                                                            int nb, double* record_pe, int step) {
+    double v[7];
+    for (int k = 0; k < 7; ++k)
+        v[k] = block_fixed_sum(partials + k, nb, kPartialStride);
+    if (threadIdx.x == 0) {
+        ctl->pe = v[0];
+        for (int k = 0; k < 6; ++k)
+            ctl->vir[k] = v[k + 1];
+        if (record_pe)
+            record_pe[step] = v[0];
+    }
+}
+
+IntArgs make_int_args(atvh_system* s) {
+    IntArgs a;
+    a.x = s->x;
+    a.y = s->y;
+    a.z = s->z;
+    a.vx = s->vx;
+    a.vy = s->vy;
+    a.vz = s->vz;
+    a.fx = s->fx;
+    a.fy = s->fy;
+    a.fz = s->fz;
+    a.type = s->type;
+    a.xs = s->xs;
+    a.ys = s->ys;
+    a.zs = s->zs;
+    // VelocityVerlet: dtf = 0.5*dt*force_to_accel, dtfm = dtf / m
+    const double dtf = 0.5 * s->dt * force_to_accel;
+    for (int t = 0; t < kMaxTypes; ++t) {
+        a.mass[t] = t < s->ntypes ? s->mass_t[t] : 1.0;
+        a.dtfm[t] = dtf / a.mass[t];
+    }
+    a.ntypes = s->ntypes;
+    a.dt = s->dt;
+    a.lx = s->box[0];
+    a.ly = s->box[1];
+    a.lz = s->box[2];
+    a.skin_sq = s->skin * s->skin;
+    a.thermostat = s->thermostat;
+    a.ctl = s->ctl;
+    a.ke_partials = s->partials + size_t(s->npartials) * kPartialStride;
+    return a;
+}
+
+PostArgs make_post_args(atvh_system* s, bool final_phase, bool decide) {
+    PostArgs p;
+    p.ctl = s->ctl;
+    p.ke_partials = s->partials + size_t(s->npartials) * kPartialStride;
+    p.nb = grid_capped(s->n);
+    p.thermostat = s->thermostat;
+    p.dt = s->dt;
+    // NoseHoover::build / launch: Q = Ndof kB T tau^2
+    p.Q = static_cast<double>(s->ndof) * kB_ev_per_k * s->T_target * s->tau_T * s->tau_T;
+    p.ndof_kT = static_cast<double>(s->ndof) * kB_ev_per_k * s->T_target;
+    p.decide = decide ? 1 : 0;
+    p.every = s->every;
+    p.delay = s->delay;
+    p.check = s->check;
+    p.record_ke = nullptr;
+    p.step = 0;
+    p.final_phase = final_phase ? 1 : 0;
+    return p;
+}
+
+void check_integ(atvh_system* s) {
+    if (!s->integ_configured)
+        throw Error(ATVH_E_STATE, "integrator not configured");
+}
+
+} // namespace
+
+namespace atvh {
+
+void integ_enqueue_initial(atvh_system* s) {
+    ScopedTimer tm(s->ctx, "integrate");
+    IntArgs a = make_int_args(s);
+    k_initial<<<grid_capped(s->n), kBlock, 0, s->ctx->stream>>>(s->n, a);
+    ATVH_HIP(hipGetLastError());
+}
+
+void integ_enqueue_final(atvh_system* s) {
+    ScopedTimer tm(s->ctx, "integrate");
+    IntArgs a = make_int_args(s);
+    k_final<<<grid_capped(s->n), kBlock, 0, s->ctx->stream>>>(s->n, a);
+    ATVH_HIP(hipGetLastError());
+}
+
+void energy_enqueue_finalize(atvh_system* s, double* record_pe, double* record_ke) {
+    (void)record_ke;
+    k_energy_finalize<<<1, kBlock, 0, s->ctx->stream>>>(s->ctl, s->partials, pair_grid(s),
+                                                        record_pe, 0);
+    ATVH_HIP(hipGetLastError());
+}
+
+} // namespace atvh
+
+#define API_TRY try {
+#define API_CATCH(C)                                                                    \
+    }                                                                                   \
+    catch (const atvh::Error& e_) {                                                     \
+        if (C)                                                                          \
+            (C)->err = e_.what();                                                       \
+        return e_.code;                                                                 \
+    }                                                                                   \
+    catch (const std::exception& e_) {                                                  \
+        if (C)                                                                          \
+            (C)->err = e_.what();                                                       \
+        return ATVH_E_ARG;                                                              \
+    }
+
+extern "C" {
+
+int atvh_integrator_configure(atvh_system* s, double dt, int thermostat, double T_target,
+                              double tau_T, int64_t ndof) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    if (!(dt > 0.0))
+        throw Error(ATVH_E_ARG, "VelocityVerlet: dt must be positive");
+    if (thermostat != 0 && thermostat != 1)
+        throw Error(ATVH_E_ARG, "unknown thermostat");
+    if (thermostat == 1 && (!(T_target > 0.0) || !(tau_T > 0.0)))
+        throw Error(ATVH_E_ARG, "NoseHoover: T and tau must be positive");
+    s->dt = dt;
+    s->thermostat = thermostat;
+    s->T_target = T_target;
+    s->tau_T = tau_T;
+    // publish_global_dof (src/cpu/driver/dof.cpp:49): Ndof = 3N - 3
+    s->ndof = ndof > 0 ? ndof : 3 * s->n - 3;
+    s->integ_configured = true;
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+int atvh_step_vvnh(atvh_system* s, int phase) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    check_integ(s);
+    ATVH_HIP(hipSetDevice(c->device));
+    if (phase == ATVH_PHASE_INITIAL) {
+        integ_enqueue_initial(s);
+        PostArgs p = make_post_args(s, false, false);
+        k_post<<<1, kBlock, 0, c->stream>>>(p);
+    } else if (phase == ATVH_PHASE_FINAL) {
+        integ_enqueue_final(s);
+        PostArgs p = make_post_args(s, true, false);
+        k_post<<<1, kBlock, 0, c->stream>>>(p);
+    } else {
+        throw Error(ATVH_E_ARG, "atvh_step_vvnh: unknown phase");
+    }
+    ATVH_HIP(hipGetLastError());
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+int atvh_md_run(atvh_system* s, int nsteps, int precision, int energy_every) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    if (nsteps < 0)
+        throw Error(ATVH_E_ARG, "atvh_md_run: nsteps < 0");
+    if (precision != ATVH_PREC_FP64 && precision != ATVH_PREC_MIXED)
+        throw Error(ATVH_E_ARG, "atvh_md_run: unknown precision");
+    check_integ(s);
+    if (!s->nl_configured)
+        throw Error(ATVH_E_STATE, "atvh_md_run: no neighbor list configured");
+    if (!s->pair_configured)
+        throw Error(ATVH_E_STATE, "atvh_md_run: no force evaluator configured");
+    ATVH_HIP(hipSetDevice(c->device));
+    if (!s->nl_built) {
+        const int rc = atvh_nlist_build(s);
+        if (rc != ATVH_OK)
+            return rc;
+    }
+    hipStream_t st = c->stream;
+    if (s->thermo_cap < nsteps) {
+        dev_free(s->d_thermo);
+        s->thermo_cap = nsteps;
+        dev_alloc(&s->d_thermo, size_t(2) * nsteps);
+    }
+    s->thermo_n = nsteps;
+    if (nsteps > 0) {
+        // NaN marks steps without an energy request; layout [pe | ke], stride thermo_cap
+        std::vector<double> nanv(size_t(2) * s->thermo_cap, std::nan(""));
+        ATVH_HIP(hipMemcpyAsync(s->d_thermo, nanv.data(), nanv.size() * sizeof(double),
+                                hipMemcpyHostToDevice, st));
+        ATVH_HIP(hipStreamSynchronize(st));
+    }
+    double* rec_pe = s->d_thermo;
+    double* rec_ke = s->d_thermo + s->thermo_cap;
+    const IntArgs ia = make_int_args(s);
+    PostArgs p_init = make_post_args(s, false, true);
+    PostArgs p_fin = make_post_args(s, true, false);
+    p_fin.record_ke = rec_ke;
+    const int g = grid_capped(s->n);
+    for (int step = 0; step < nsteps; ++step) {
+        {
+            ScopedTimer tm(c, "integrate");
+            k_initial<<<g, kBlock, 0, st>>>(s->n, ia);
+        }
+        k_post<<<1, kBlock, 0, st>>>(p_init);
+        nl_enqueue_rebuild(s, &s->ctl->rebuild, false);
+        const bool ef = (step == nsteps - 1) || (energy_every > 0 && (step % energy_every) == 0);
+        pair_enqueue(s, precision, ef, ef);
+        if (ef)
+            k_energy_finalize<<<1, kBlock, 0, st>>>(s->ctl, s->partials, pair_grid(s), rec_pe,
+                                                    step);
+        {
+            ScopedTimer tm(c, "integrate");
+            k_final<<<g, kBlock, 0, st>>>(s->n, ia);
+        }
+        p_fin.step = step;
+        k_post<<<1, kBlock, 0, st>>>(p_fin);
+        ATVH_HIP(hipGetLastError());
+    }
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+int atvh_md_thermo(atvh_system* s, double* pe, double* ke, int nsteps) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    if (nsteps > s->thermo_n)
+        throw Error(ATVH_E_ARG, "atvh_md_thermo: more steps requested than recorded");
+    ATVH_HIP(hipStreamSynchronize(c->stream));
+    if (nsteps <= 0)
+        return ATVH_OK;
+    if (pe)
+        ATVH_HIP(hipMemcpy(pe, s->d_thermo, nsteps * sizeof(double), hipMemcpyDeviceToHost));
+    if (ke)
+        ATVH_HIP(hipMemcpy(ke, s->d_thermo + s->thermo_cap, nsteps * sizeof(double),
+                           hipMemcpyDeviceToHost));
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+} // extern "C"

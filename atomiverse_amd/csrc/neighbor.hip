@@ -1,0 +1,746 @@
+// neighbor.hip — cell binning by counting sort and the Verlet list build.
+//
+// Reference semantics (single rank, orthorhombic):
+//   grid:     CellList::compute_grid / build_stencil (src/cpu/neighbor/cell_list.cpp:83-232)
+//   binning:  CellList::bin_atoms (:234-314) — wrapped copy for the cell index only,
+//             floor(x/cs) clamped, z-fastest cell index
+//   pairs:    the half-stencil predicate of refresh_half (:539-738): a pair whose
+//             cells differ is tested in the orientation of the FORWARD stencil
+//             offset as  d = x_i - (x_j + shift),  a home-cell pair as d = x_i - x_j,
+//             accepted iff (dx*dx + dy*dy) + dz*dz < (rc+skin)^2. With the small-grid
+//             fallback (fewer than 2*reach+1 cells on an axis) the full stencil with
+//             while-loop min-image is used instead, as the reference does.
+//   cadence:  NeighborList::decide_reneighbor (include/atomiverse/cpu/neighbor/neighbor_list.h:128-141)
+//
+// MI355X layout: atoms are physically re-sorted by (cell, tag) at every
+// build, so cell c owns the contiguous slot range [cell_start[c],
+// cell_start[c+1]) and neighbour gathers stay in L2. The list is a FULL
+// list (both directions of every accepted pair) stored column-major
+// (ELL, nbr[k*npad + i]) so a wave's k-th loads are one coalesced 256 B line;
+// each unordered pair is tested once per side with the canonical orientation
+// above, so both sides reach the same bit-identical decision. The whole
+// rebuild is predicated on a device flag, so the host never waits for the
+// decision. All expressions here are compiled with -ffp-contract=off.
+
+#include "atvh_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+using namespace atvh;
+
+namespace {
+
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kBlock * kScanItems; // elements per scan block
+
+// CellList::bin_atoms: cell of every slot (current order) + histogram.
+__global__ void k_nl_bin(const int* __restrict__ pred, int64_t n, const double* __restrict__ x,
+                         const double* __restrict__ y, const double* __restrict__ z, double lx,
+                         double ly, double lz, double csx, double csy, double csz, int ncx, int ncy,
+                         int ncz, int* __restrict__ cell_of, int* __restrict__ count) {
+    if (!*pred)
+        return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double xi = x[i], yi = y[i], zi = z[i];
+        xi -= lx * floor(xi / lx);
+        yi -= ly * floor(yi / ly);
+        zi -= lz * floor(zi / lz);
+        int cx = static_cast<int>(floor((xi - 0.0) / csx));
+        int cy = static_cast<int>(floor((yi - 0.0) / csy));
+        int cz = static_cast<int>(floor((zi - 0.0) / csz));
+        cx = max(0, min(cx, ncx - 1));
+        cy = max(0, min(cy, ncy - 1));
+        cz = max(0, min(cz, ncz - 1));
+        const int c = cx * ncy * ncz + cy * ncz + cz;
+        cell_of[i] = c;
+        atomicAdd(&count[c], 1);
+    }
+}
+
+// Block-wide exclusive scan of one int per thread (LDS Hillis-Steele).
+__device__ int block_exclusive_scan(int v, int* sh, int& total) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        int add = t >= off ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    const int incl = sh[t];
+    total = sh[kBlock - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+// Scan phase 1: per-tile exclusive scan (tile = kScanTile cells) + tile total.
+__global__ __launch_bounds__(kBlock) void k_nl_scan_tiles(const int* __restrict__ pred, int ncells,
+                                                          const int* __restrict__ count,
+                                                          int* __restrict__ start,
+                                                          int* __restrict__ tile_sums) {
+    if (!*pred)
+        return;
+    __shared__ int sh[kBlock];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int sum = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < ncells) ? count[base + k] : 0;
+        sum += v[k];
+    }
+    int total;
+    int ex = block_exclusive_scan(sum, sh, total);
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < ncells)
+            start[base + k] = ex;
+        ex += v[k];
+    }
+    if (threadIdx.x == 0)
+        tile_sums[blockIdx.x] = total;
+}
+
+// Scan phase 2: one block scans the tile totals (any count, chunked).
+__global__ __launch_bounds__(kBlock) void k_nl_scan_sums(const int* __restrict__ pred, int ntiles,
+                                                         int* __restrict__ tile_sums,
+                                                         int* __restrict__ start, int ncells) {
+    if (!*pred)
+        return;
+    __shared__ int sh[kBlock];
+    int carry = 0;
+    for (int b0 = 0; b0 < ntiles; b0 += kBlock) {
+        const int idx = b0 + threadIdx.x;
+        const int v = idx < ntiles ? tile_sums[idx] : 0;
+        int total;
+        const int ex = block_exclusive_scan(v, sh, total);
+        if (idx < ntiles)
+            tile_sums[idx] = ex + carry;
+        carry += total;
+    }
+    if (threadIdx.x == 0)
+        start[ncells] = carry;
+}
+
+// Scan phase 3: add the tile offsets.
+__global__ __launch_bounds__(kBlock) void k_nl_scan_add(const int* __restrict__ pred, int ncells,
+                                                        const int* __restrict__ tile_sums,
+                                                        int* __restrict__ start) {
+    if (!*pred)
+        return;
+    const int off = tile_sums[blockIdx.x];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < ncells)
+            start[base + k] += off;
+}
+
+// Counting-sort scatter: slot of every atom inside its cell (order fixed
+// afterwards by k_nl_cell_order). Also clears the histogram for next time.
+__global__ void k_nl_scatter(const int* __restrict__ pred, int64_t n,
+                             const int* __restrict__ cell_of, const int* __restrict__ start,
+                             int* __restrict__ fill, int* __restrict__ count,
+                             int* __restrict__ perm) {
+    if (!*pred)
+        return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int c = cell_of[i];
+        const int slot = start[c] + atomicAdd(&fill[c], 1);
+        perm[slot] = static_cast<int>(i);
+        count[c] = 0;
+    }
+}
+
+// Deterministic order inside a cell: ascending tag (insertion sort; cells
+// hold a handful of atoms). Also clears the fill counters.
+__global__ void k_nl_cell_order(const int* __restrict__ pred, int ncells,
+                                const int* __restrict__ start, const int* __restrict__ tag,
+                                int* __restrict__ perm, int* __restrict__ fill) {
+    if (!*pred)
+        return;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x) {
+        const int b = start[c], e = start[c + 1];
+        for (int k = b + 1; k < e; ++k) {
+            const int p = perm[k];
+            const int tp = tag[p];
+            int m = k - 1;
+            while (m >= b && tag[perm[m]] > tp) {
+                perm[m + 1] = perm[m];
+                --m;
+            }
+            perm[m + 1] = p;
+        }
+        fill[c] = 0;
+    }
+}
+
+struct PermArgs {
+    const int* perm;
+    const double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *q;
+    const int *type, *tag, *cell_of;
+    double *ax, *ay, *az, *avx, *avy, *avz, *afx, *afy, *afz, *aq;
+    int *atype, *atag, *acell_of;
+    int with_forces;
+};
+
+__global__ void k_nl_permute(const int* __restrict__ pred, int64_t n, PermArgs a) {
+    if (!*pred)
+        return;
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        const int p = a.perm[s];
+        a.ax[s] = a.x[p];
+        a.ay[s] = a.y[p];
+        a.az[s] = a.z[p];
+        a.avx[s] = a.vx[p];
+        a.avy[s] = a.vy[p];
+        a.avz[s] = a.vz[p];
+        a.aq[s] = a.q[p];
+        a.atype[s] = a.type[p];
+        a.atag[s] = a.tag[p];
+        a.acell_of[s] = a.cell_of[p];
+        if (a.with_forces) {
+            a.afx[s] = a.fx[p];
+            a.afy[s] = a.fy[p];
+            a.afz[s] = a.fz[p];
+        }
+    }
+}
+
+struct CopyArgs {
+    const double *ax, *ay, *az, *avx, *avy, *avz, *afx, *afy, *afz, *aq;
+    const int *atype, *atag, *acell_of;
+    double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *q, *xs, *ys, *zs;
+    int *type, *tag, *cell_of;
+    int with_forces;
+};
+
+// Copy the permuted arrays back and snapshot x_save (CellList::save_positions).
+__global__ void k_nl_copyback(const int* __restrict__ pred, int64_t n, CopyArgs a) {
+    if (!*pred)
+        return;
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        const double xx = a.ax[s], yy = a.ay[s], zz = a.az[s];
+        a.x[s] = xx;
+        a.y[s] = yy;
+        a.z[s] = zz;
+        a.xs[s] = xx;
+        a.ys[s] = yy;
+        a.zs[s] = zz;
+        a.vx[s] = a.avx[s];
+        a.vy[s] = a.avy[s];
+        a.vz[s] = a.avz[s];
+        a.q[s] = a.aq[s];
+        a.type[s] = a.atype[s];
+        a.tag[s] = a.atag[s];
+        a.cell_of[s] = a.acell_of[s];
+        if (a.with_forces) {
+            a.fx[s] = a.afx[s];
+            a.fy[s] = a.afy[s];
+            a.fz[s] = a.afz[s];
+        }
+    }
+}
+
+struct BuildArgs {
+    const double *x, *y, *z;
+    const int* cell_of;
+    const int* start;
+    const int4* stencil; // {di, dj, dk, kind}: kind 0 home, 1 forward, 2 backward
+    int nstencil;
+    int ncx, ncy, ncz;
+    double lx, ly, lz;
+    double rc_sq;
+    int use_half;
+    int* nbr;
+    int* numneigh;
+    int cap;
+    int64_t npad;
+    DevCtl* ctl;
+};
+
+// One thread per slot i: scan the full stencil, test each candidate with the
+// canonical orientation, append accepted j (full list). HALF selects the
+// reference's half-stencil arithmetic (every axis has >= 2*reach+1 cells);
+// otherwise the small-grid min-image fallback.
+template <bool HALF>
+__global__ __launch_bounds__(kBlock) void k_nl_build(const int* __restrict__ pred, int64_t n,
+                                                     BuildArgs a) {
+    if (!*pred)
+        return;
+    __shared__ long long sh_tot[kBlock / 64];
+    __shared__ int sh_max[kBlock / 64];
+    long long my_cnt_sum = 0;
+    int my_max = 0;
+    int my_over = 0;
+    const double half_lx = 0.5 * a.lx, half_ly = 0.5 * a.ly, half_lz = 0.5 * a.lz;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ci = a.cell_of[i];
+        const int cz0 = ci % a.ncz;
+        const int cy0 = (ci / a.ncz) % a.ncy;
+        const int cx0 = ci / (a.ncy * a.ncz);
+        const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
+        int cnt = 0;
+        int visited[HALF ? 1 : 128];
+        int nvis = 0;
+        for (int s = 0; s < a.nstencil; ++s) {
+            const int4 o = a.stencil[s];
+            int cx = cx0 + o.x, cy = cy0 + o.y, cz = cz0 + o.z;
+            double shx = 0.0, shy = 0.0, shz = 0.0;
+            if (cx < 0) { cx += a.ncx; shx = -a.lx; } else if (cx >= a.ncx) { cx -= a.ncx; shx = a.lx; }
+            if (cy < 0) { cy += a.ncy; shy = -a.ly; } else if (cy >= a.ncy) { cy -= a.ncy; shy = a.ly; }
+            if (cz < 0) { cz += a.ncz; shz = -a.lz; } else if (cz >= a.ncz) { cz -= a.ncz; shz = a.lz; }
+            if (!HALF) {
+                // small grids: fold residues, dedupe aliased cells
+                if (cx < 0 || cx >= a.ncx) cx = ((cx % a.ncx) + a.ncx) % a.ncx;
+                if (cy < 0 || cy >= a.ncy) cy = ((cy % a.ncy) + a.ncy) % a.ncy;
+                if (cz < 0 || cz >= a.ncz) cz = ((cz % a.ncz) + a.ncz) % a.ncz;
+            }
+            const int cj = cx * a.ncy * a.ncz + cy * a.ncz + cz;
+            if (!HALF) {
+                bool seen = false;
+                for (int v = 0; v < nvis; ++v)
+                    if (visited[v] == cj) { seen = true; break; }
+                if (seen)
+                    continue;
+                if (nvis < 128)
+                    visited[nvis++] = cj;
+            }
+            const int jb = a.start[cj], je = a.start[cj + 1];
+            for (int j = jb; j < je; ++j) {
+                if (j == i)
+                    continue;
+                const double xj = a.x[j], yj = a.y[j], zj = a.z[j];
+                double dx, dy, dz;
+                if (!HALF) {
+                    dx = xi - xj;
+                    dy = yi - yj;
+                    dz = zi - zj;
+                    while (dx > half_lx) dx -= a.lx;
+                    while (dx < -half_lx) dx += a.lx;
+                    while (dy > half_ly) dy -= a.ly;
+                    while (dy < -half_ly) dy += a.ly;
+                    while (dz > half_lz) dz -= a.lz;
+                    while (dz < -half_lz) dz += a.lz;
+                } else if (o.w == 2) {
+                    // backward offset: the pair belongs to j's forward scan
+                    dx = xj - (xi - shx);
+                    dy = yj - (yi - shy);
+                    dz = zj - (zi - shz);
+                } else {
+                    dx = xi - (xj + shx);
+                    dy = yi - (yj + shy);
+                    dz = zi - (zj + shz);
+                }
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 < a.rc_sq) {
+                    if (cnt < a.cap)
+                        a.nbr[(int64_t)cnt * a.npad + i] = j;
+                    ++cnt;
+                }
+            }
+        }
+        a.numneigh[i] = cnt < a.cap ? cnt : a.cap;
+        my_cnt_sum += cnt;
+        my_max = max(my_max, cnt);
+        if (cnt > a.cap)
+            ++my_over;
+    }
+    // block statistics: total entries, max, overflow
+    for (int off = 32; off > 0; off >>= 1) {
+        my_cnt_sum += __shfl_down(my_cnt_sum, off, 64);
+        my_max = max(my_max, __shfl_down(my_max, off, 64));
+        my_over += __shfl_down(my_over, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __shared__ int sh_over[kBlock / 64];
+    if (lane == 0) {
+        sh_tot[wid] = my_cnt_sum;
+        sh_max[wid] = my_max;
+        sh_over[wid] = my_over;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        int m = 0, ov = 0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            t += sh_tot[w];
+            m = max(m, sh_max[w]);
+            ov += sh_over[w];
+        }
+        atomicAdd(reinterpret_cast<unsigned long long*>(&a.ctl->nbr_total),
+                  static_cast<unsigned long long>(t));
+        atomicMax(&a.ctl->max_nbr, m);
+        if (ov)
+            atomicAdd(&a.ctl->overflow, ov);
+    }
+}
+
+// LAMMPS Neighbor::decide for one step (neighbor_list.h:128-141) — the
+// displacement half (needs_rebuild, cell_list.cpp:372-391) is folded into the
+// initial-integrate kernel, which raises disp_flag.
+__global__ void k_nl_decide(DevCtl* ctl, int every, int delay, int check) {
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    ctl->ago += 1;
+    const bool delay_ok = ctl->ago >= delay;
+    const bool every_ok = (ctl->ago % every) == 0;
+    const bool rb = delay_ok && every_ok && (!check || ctl->disp_flag != 0);
+    ctl->disp_flag = 0;
+    ctl->rebuild = rb ? 1 : 0;
+    if (rb) {
+        ctl->ago = 0;
+        ctl->nrebuilds += 1;
+        ctl->nbr_total = 0;
+        ctl->max_nbr = 0;
+    }
+}
+
+__global__ void k_nl_stats_reset(DevCtl* ctl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->nbr_total = 0;
+        ctl->max_nbr = 0;
+        ctl->overflow = 0;
+    }
+}
+
+__global__ void k_nl_export(int64_t n, const int* __restrict__ tag, const int* __restrict__ numneigh,
+                            const int* __restrict__ nbr, int64_t npad,
+                            const long long* __restrict__ offs, long long* __restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ti = tag[i];
+        long long w = offs[i];
+        const int k_n = numneigh[i];
+        for (int k = 0; k < k_n; ++k) {
+            const int j = nbr[(int64_t)k * npad + i];
+            const int tj = tag[j];
+            if (ti < tj) {
+                out[2 * w] = ti;
+                out[2 * w + 1] = tj;
+                ++w;
+            }
+        }
+    }
+}
+
+__global__ void k_nl_export_count(int64_t n, const int* __restrict__ tag,
+                                  const int* __restrict__ numneigh, const int* __restrict__ nbr,
+                                  int64_t npad, long long* __restrict__ cnt) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ti = tag[i];
+        long long c = 0;
+        const int k_n = numneigh[i];
+        for (int k = 0; k < k_n; ++k)
+            if (ti < tag[nbr[(int64_t)k * npad + i]])
+                ++c;
+        cnt[i] = c;
+    }
+}
+
+} // namespace
+
+namespace atvh {
+
+// CellList::compute_grid + build_stencil on the host (box-change only).
+void nl_setup_grid(atvh_system* s) {
+    const double cutoff_skin = s->cutoff_skin;
+    for (int d = 0; d < 3; ++d) {
+        const double L = s->box[d];
+        const int nc = std::max(int(1), static_cast<int>(std::floor(L / (0.5 * cutoff_skin))));
+        s->nc[d] = nc;
+        s->cs[d] = L / nc;
+    }
+    for (int d = 0; d < 3; ++d)
+        s->reach[d] = std::max(1, static_cast<int>(std::ceil(cutoff_skin / s->cs[d] - 1e-12)));
+    const int64_t ncells64 = int64_t(s->nc[0]) * s->nc[1] * s->nc[2];
+    if (ncells64 > (int64_t(1) << 28))
+        throw Error(ATVH_E_ARG, "neighbor grid too large");
+    s->ncells = static_cast<int>(ncells64);
+    s->use_half = true;
+    for (int d = 0; d < 3; ++d)
+        if (s->nc[d] < 2 * s->reach[d] + 1)
+            s->use_half = false;
+    const double rc_sq = cutoff_skin * cutoff_skin;
+    auto min_dist_sq = [&](int di, int dj, int dk) {
+        const double ddx = di > 0 ? (di - 1) * s->cs[0] : (di < 0 ? (di + 1) * s->cs[0] : 0.0);
+        const double ddy = dj > 0 ? (dj - 1) * s->cs[1] : (dj < 0 ? (dj + 1) * s->cs[1] : 0.0);
+        const double ddz = dk > 0 ? (dk - 1) * s->cs[2] : (dk < 0 ? (dk + 1) * s->cs[2] : 0.0);
+        return ddx * ddx + ddy * ddy + ddz * ddz;
+    };
+    std::vector<int> st;
+    for (int di = -s->reach[0]; di <= s->reach[0]; ++di)
+        for (int dj = -s->reach[1]; dj <= s->reach[1]; ++dj)
+            for (int dk = -s->reach[2]; dk <= s->reach[2]; ++dk) {
+                if (min_dist_sq(di, dj, dk) > rc_sq)
+                    continue;
+                const bool home = (di == 0 && dj == 0 && dk == 0);
+                const bool forward = (di > 0) || (di == 0 && dj > 0) || (di == 0 && dj == 0 && dk > 0);
+                st.push_back(di);
+                st.push_back(dj);
+                st.push_back(dk);
+                st.push_back(home ? 0 : (forward ? 1 : 2));
+            }
+    s->nstencil = static_cast<int>(st.size() / 4);
+    nl_free(s);
+    dev_alloc(&s->d_stencil, st.size());
+    ATVH_HIP(hipMemcpy(s->d_stencil, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
+    dev_alloc(&s->cell_of, s->n);
+    dev_alloc(&s->acell_of, s->n);
+    dev_alloc(&s->perm, s->n);
+    dev_alloc(&s->cell_count, s->ncells);
+    dev_alloc(&s->cell_fill, s->ncells);
+    dev_alloc(&s->cell_start, s->ncells + 1);
+    s->nscan_blocks = (s->ncells + kScanTile - 1) / kScanTile;
+    dev_alloc(&s->scan_sums, s->nscan_blocks);
+    ATVH_HIP(hipMemset(s->cell_count, 0, sizeof(int) * s->ncells));
+    ATVH_HIP(hipMemset(s->cell_fill, 0, sizeof(int) * s->ncells));
+    dev_alloc(&s->numneigh, s->n);
+    // capacity: 1.5x the mean full-list count at this density, plus margin
+    const double vol = s->box[0] * s->box[1] * s->box[2];
+    const double mean = double(s->n) / vol * 4.18879020478639 * cutoff_skin * cutoff_skin * cutoff_skin;
+    int cap = static_cast<int>(std::ceil(1.5 * mean + 32));
+    cap = (cap + 7) / 8 * 8;
+    if (cap > s->n)
+        cap = static_cast<int>(std::max<int64_t>(8, (s->n + 7) / 8 * 8));
+    s->cap = cap;
+    dev_alloc(&s->nbr, size_t(cap) * s->npad);
+}
+
+void nl_free(atvh_system* s) {
+    dev_free(s->d_stencil);
+    dev_free(s->cell_of);
+    dev_free(s->acell_of);
+    dev_free(s->perm);
+    dev_free(s->cell_count);
+    dev_free(s->cell_fill);
+    dev_free(s->cell_start);
+    dev_free(s->scan_sums);
+    dev_free(s->numneigh);
+    dev_free(s->nbr);
+}
+
+static void nl_realloc_capacity(atvh_system* s, int cap) {
+    dev_free(s->nbr);
+    s->cap = (cap + 7) / 8 * 8;
+    dev_alloc(&s->nbr, size_t(s->cap) * s->npad);
+}
+
+void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
+    ScopedTimer tm(s->ctx, "nlist");
+    hipStream_t st = s->ctx->stream;
+    const int64_t n = s->n;
+    const int g = grid_capped(n);
+    k_nl_bin<<<g, kBlock, 0, st>>>(pred, n, s->x, s->y, s->z, s->box[0], s->box[1], s->box[2],
+                                   s->cs[0], s->cs[1], s->cs[2], s->nc[0], s->nc[1], s->nc[2],
+                                   s->cell_of, s->cell_count);
+    k_nl_scan_tiles<<<s->nscan_blocks, kBlock, 0, st>>>(pred, s->ncells, s->cell_count,
+                                                        s->cell_start, s->scan_sums);
+    k_nl_scan_sums<<<1, kBlock, 0, st>>>(pred, s->nscan_blocks, s->scan_sums, s->cell_start,
+                                         s->ncells);
+    k_nl_scan_add<<<s->nscan_blocks, kBlock, 0, st>>>(pred, s->ncells, s->scan_sums,
+                                                      s->cell_start);
+    k_nl_scatter<<<g, kBlock, 0, st>>>(pred, n, s->cell_of, s->cell_start, s->cell_fill,
+                                       s->cell_count, s->perm);
+    k_nl_cell_order<<<grid_capped(s->ncells), kBlock, 0, st>>>(pred, s->ncells, s->cell_start,
+                                                               s->tag, s->perm, s->cell_fill);
+    PermArgs pa{s->perm, s->x,  s->y,  s->z,  s->vx,  s->vy,  s->vz,  s->fx,  s->fy,  s->fz,
+                s->q,    s->type, s->tag, s->cell_of, s->ax, s->ay, s->az, s->avx, s->avy,
+                s->avz,  s->afx, s->afy, s->afz, s->aq, s->atype, s->atag, s->acell_of,
+                permute_forces ? 1 : 0};
+    k_nl_permute<<<g, kBlock, 0, st>>>(pred, n, pa);
+    CopyArgs ca{s->ax, s->ay, s->az, s->avx, s->avy, s->avz, s->afx, s->afy, s->afz, s->aq,
+                s->atype, s->atag, s->acell_of, s->x, s->y, s->z, s->vx, s->vy, s->vz,
+                s->fx, s->fy, s->fz, s->q, s->xs, s->ys, s->zs, s->type, s->tag, s->cell_of,
+                permute_forces ? 1 : 0};
+    k_nl_copyback<<<g, kBlock, 0, st>>>(pred, n, ca);
+    BuildArgs ba{s->x,         s->y,        s->z,        s->cell_of, s->cell_start,
+                 reinterpret_cast<const int4*>(s->d_stencil), s->nstencil, s->nc[0], s->nc[1],
+                 s->nc[2],     s->box[0],   s->box[1],   s->box[2],  s->cutoff_skin * s->cutoff_skin,
+                 s->use_half ? 1 : 0, s->nbr, s->numneigh, s->cap, s->npad, s->ctl};
+    if (s->use_half)
+        k_nl_build<true><<<grid_for(n), kBlock, 0, st>>>(pred, n, ba);
+    else
+        k_nl_build<false><<<grid_for(n), kBlock, 0, st>>>(pred, n, ba);
+    ATVH_HIP(hipGetLastError());
+}
+
+void nl_enqueue_decide(atvh_system* s) {
+    k_nl_decide<<<1, 64, 0, s->ctx->stream>>>(s->ctl, s->every, s->delay, s->check);
+    ATVH_HIP(hipGetLastError());
+}
+
+} // namespace atvh
+
+#define API_TRY try {
+#define API_CATCH(C)                                                                    \
+    }                                                                                   \
+    catch (const atvh::Error& e_) {                                                     \
+        if (C)                                                                          \
+            (C)->err = e_.what();                                                       \
+        return e_.code;                                                                 \
+    }                                                                                   \
+    catch (const std::exception& e_) {                                                  \
+        if (C)                                                                          \
+            (C)->err = e_.what();                                                       \
+        return ATVH_E_ARG;                                                              \
+    }
+
+extern "C" {
+
+int atvh_nlist_configure(atvh_system* s, double cutoff, double skin, int every, int delay,
+                         int check) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    if (!(cutoff > 0.0))
+        throw Error(ATVH_E_ARG, "atvh_nlist_configure: cutoff must be positive");
+    ATVH_HIP(hipSetDevice(c->device));
+    ATVH_HIP(hipStreamSynchronize(c->stream));
+    // NeighborList(cutoff, skin, every, delay, check) (neighbor_list.h:79-83),
+    // CellList(cutoff, skin): skin < 0 -> 0.3*cutoff (cell_list.cpp:24-26)
+    s->cutoff = cutoff;
+    s->skin = skin < 0 ? 0.3 * cutoff : skin;
+    s->cutoff_skin = cutoff + s->skin;
+    s->every = every < 1 ? 1 : every;
+    s->delay = delay < 0 ? 0 : delay;
+    s->check = check ? 1 : 0;
+    nl_setup_grid(s);
+    s->nl_configured = true;
+    s->nl_built = false;
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+int atvh_nlist_build(atvh_system* s) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    if (!s->nl_configured)
+        throw Error(ATVH_E_STATE, "atvh_nlist_build: neighbor list not configured");
+    ATVH_HIP(hipSetDevice(c->device));
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        k_nl_stats_reset<<<1, 64, 0, c->stream>>>(s->ctl);
+        // the initial build also orders the frame's forces (first kick uses them)
+        nl_enqueue_rebuild(s, &s->ctl->always, true);
+        ATVH_HIP(hipStreamSynchronize(c->stream));
+        DevCtl h{};
+        ATVH_HIP(hipMemcpy(&h, s->ctl, sizeof(h), hipMemcpyDeviceToHost));
+        if (h.overflow == 0) {
+            h.ago = 0;
+            h.nrebuilds = 1;
+            h.disp_flag = 0;
+            h.rebuild = 0;
+            ATVH_HIP(hipMemcpy(s->ctl, &h, sizeof(h), hipMemcpyHostToDevice));
+            s->nl_built = true;
+            return ATVH_OK;
+        }
+        // grow and rebuild from the (now sorted) positions
+        nl_realloc_capacity(s, static_cast<int>(h.max_nbr * 1.25) + 16);
+    }
+    throw Error(ATVH_E_CAPACITY, "atvh_nlist_build: neighbor capacity could not be satisfied");
+    API_CATCH(c)
+}
+
+int atvh_nlist_decide_and_rebuild(atvh_system* s) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    if (!s->nl_built)
+        throw Error(ATVH_E_STATE, "atvh_nlist_decide_and_rebuild: call atvh_nlist_build first");
+    nl_enqueue_decide(s);
+    nl_enqueue_rebuild(s, &s->ctl->rebuild, false);
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+int atvh_nlist_stats(atvh_system* s, int64_t* nrebuilds, int64_t* npairs_half, int* max_neighbors,
+                     int* capacity) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    ATVH_HIP(hipStreamSynchronize(c->stream));
+    DevCtl h{};
+    ATVH_HIP(hipMemcpy(&h, s->ctl, sizeof(h), hipMemcpyDeviceToHost));
+    if (nrebuilds)
+        *nrebuilds = h.nrebuilds;
+    if (npairs_half)
+        *npairs_half = h.nbr_total / 2;
+    if (max_neighbors)
+        *max_neighbors = h.max_nbr;
+    if (capacity)
+        *capacity = s->cap;
+    if (h.overflow)
+        throw Error(ATVH_E_CAPACITY, "neighbor capacity exceeded during a rebuild");
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+int atvh_nlist_export_pairs(atvh_system* s, int64_t* pairs, int64_t cap, int64_t* npairs) {
+    if (!s || !npairs)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    if (!s->nl_built)
+        throw Error(ATVH_E_STATE, "atvh_nlist_export_pairs: no neighbor list");
+    ATVH_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const int64_t n = s->n;
+    long long* d_cnt = nullptr;
+    dev_alloc(&d_cnt, n);
+    k_nl_export_count<<<grid_capped(n), kBlock, 0, st>>>(n, s->tag, s->numneigh, s->nbr, s->npad,
+                                                         d_cnt);
+    ATVH_HIP(hipGetLastError());
+    std::vector<long long> h(n);
+    ATVH_HIP(hipMemcpyAsync(h.data(), d_cnt, n * sizeof(long long), hipMemcpyDeviceToHost, st));
+    ATVH_HIP(hipStreamSynchronize(st));
+    long long tot = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const long long v = h[i];
+        h[i] = tot;
+        tot += v;
+    }
+    *npairs = tot;
+    if (!pairs || cap < tot) {
+        dev_free(d_cnt);
+        if (!pairs && cap == 0)
+            return ATVH_OK;
+        throw Error(ATVH_E_CAPACITY, "atvh_nlist_export_pairs: output capacity too small");
+    }
+    long long* d_out = nullptr;
+    dev_alloc(&d_out, size_t(2 * std::max<long long>(tot, 1)));
+    ATVH_HIP(hipMemcpyAsync(d_cnt, h.data(), n * sizeof(long long), hipMemcpyHostToDevice, st));
+    k_nl_export<<<grid_capped(n), kBlock, 0, st>>>(n, s->tag, s->numneigh, s->nbr, s->npad, d_cnt,
+                                                   d_out);
+    ATVH_HIP(hipGetLastError());
+    std::vector<long long> out(2 * tot);
+    if (tot)
+        ATVH_HIP(hipMemcpyAsync(out.data(), d_out, 2 * tot * sizeof(long long),
+                                hipMemcpyDeviceToHost, st));
+    ATVH_HIP(hipStreamSynchronize(st));
+    dev_free(d_out);
+    dev_free(d_cnt);
+    // sort ascending by (a, b)
+    std::vector<int64_t> key(tot);
+    for (long long k = 0; k < tot; ++k)
+        key[k] = out[2 * k] * int64_t(n) + out[2 * k + 1];
+    std::sort(key.begin(), key.end());
+    for (long long k = 0; k < tot; ++k) {
+        pairs[2 * k] = key[k] / n;
+        pairs[2 * k + 1] = key[k] % n;
+    }
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+} // extern "C"

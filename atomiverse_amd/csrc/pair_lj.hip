@@ -1,0 +1,350 @@
+// pair_lj.hip — shifted Lennard-Jones force, energy and virial on the full
+// ELL Verlet list.
+//
+// Physics: PairLJCut (src/cpu/potentials/pair_lj_cut.cpp:31-42 constructor
+// shift, :257-298 per-type table, :300-830 compute):
+//   r6inv = r2inv^3 * sigma^6,  fpair = 24 eps r2inv (2 r6inv^2 - r6inv),
+//   E = 4 eps (r6inv^2 - r6inv) - V(rc)   (energy shifted, force not),
+//   virial Voigt (dx fx, dy fy, dz fz, dx fy, dx fz, dy fz).
+// CUDA counterpart replaced: lj_force_tile (src/cuda/potentials/pair_lj_cut.cu:89-246).
+//
+// MI355X design: one thread per atom i, FULL list, so every force row has a
+// single writer (no atomics, deterministic, batch-invariant); energy and
+// virial are halved. Per-block partials of (E, virial) are summed in a fixed
+// order by the finalize kernel. Precision:
+//   FP64  — everything in double, IEEE division (the plain drop-in path);
+//   MIXED — the displacement and its minimum image are formed in FP64 (an FP32
+//           difference of ~100 Å coordinates would cost ~1e-5 Å), converted
+//           to FP32 for r2, 1/r2 (v_rcp_f32), the LJ polynomial and the
+//           per-atom force sum; energy and virial are accumulated in FP64.
+
+#include "atvh_internal.hpp"
+
+#include <cmath>
+
+using namespace atvh;
+
+namespace {
+
+struct LJArgs {
+    const double *x, *y, *z;
+    const int* type;
+    const int* nbr;
+    const int* numneigh;
+    int64_t npad;
+    double lx, ly, lz, ilx, ily, ilz;
+    double rc_sq;
+    // single type
+    double eps24, eps4, sig6, vshift;
+    // per type: double4 {eps24, eps4, sig6, shift} per (ti*ntypes + tj)
+    const double4* tab;
+    int ntypes;
+    double *fx, *fy, *fz;
+    double* partials;
+};
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Fixed-order block reduction of 7 doubles into partials[block*8 + k].
+__device__ __forceinline__ void block_partials(double (&acc)[7], double* partials) {
+    __shared__ double sh[kBlock / 64][7];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+        acc[k] = wave_sum(acc[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            sh[wid][k] = acc[k];
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w)
+            t += sh[w][threadIdx.x];
+        partials[blockIdx.x * kPartialStride + threadIdx.x] = t;
+    }
+}
+
+template <bool EF, bool VF, bool TYPED>
+__global__ __launch_bounds__(kBlock) void k_lj_fp64(int64_t n, LJArgs a) {
+#pragma clang fp contract(fast)
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
+        const int ti = TYPED ? a.type[i] : 0;
+        const int nn = a.numneigh[i];
+        double fxi = 0.0, fyi = 0.0, fzi = 0.0;
+        const int* __restrict__ col = a.nbr + i;
+        for (int k = 0; k < nn; ++k) {
+            const int j = col[(int64_t)k * a.npad];
+            double dx = xi - a.x[j];
+            double dy = yi - a.y[j];
+            double dz = zi - a.z[j];
+            dx -= a.lx * rint(dx * a.ilx);
+            dy -= a.ly * rint(dy * a.ily);
+            dz -= a.lz * rint(dz * a.ilz);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < a.rc_sq) {
+                double eps24 = a.eps24, eps4 = a.eps4, sig6 = a.sig6, shift = a.vshift;
+                if (TYPED) {
+                    const double4 t = a.tab[ti * a.ntypes + a.type[j]];
+                    eps24 = t.x;
+                    eps4 = t.y;
+                    sig6 = t.z;
+                    shift = t.w;
+                }
+                const double r2inv = 1.0 / r2;
+                const double r6inv = r2inv * r2inv * r2inv * sig6;
+                const double fpair = eps24 * r2inv * (2.0 * r6inv * r6inv - r6inv);
+                const double fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
+                fxi += fxp;
+                fyi += fyp;
+                fzi += fzp;
+                if (EF)
+                    acc[0] += eps4 * (r6inv * r6inv - r6inv) - shift;
+                if (VF) {
+                    acc[1] += dx * fxp;
+                    acc[2] += dy * fyp;
+                    acc[3] += dz * fzp;
+                    acc[4] += dx * fyp;
+                    acc[5] += dx * fzp;
+                    acc[6] += dy * fzp;
+                }
+            }
+        }
+        a.fx[i] = fxi;
+        a.fy[i] = fyi;
+        a.fz[i] = fzi;
+    }
+    if (EF || VF) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            acc[k] *= 0.5; // full list: every pair seen from both ends
+        block_partials(acc, a.partials);
+    }
+}
+
+template <bool EF, bool VF, bool TYPED>
+__global__ __launch_bounds__(kBlock) void k_lj_mixed(int64_t n, LJArgs a) {
+#pragma clang fp contract(fast)
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    const float eps24s = static_cast<float>(a.eps24), eps4s = static_cast<float>(a.eps4);
+    const float sig6s = static_cast<float>(a.sig6), shifts = static_cast<float>(a.vshift);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
+        const int ti = TYPED ? a.type[i] : 0;
+        const int nn = a.numneigh[i];
+        float fxi = 0.f, fyi = 0.f, fzi = 0.f;
+        const int* __restrict__ col = a.nbr + i;
+        for (int k = 0; k < nn; ++k) {
+            const int j = col[(int64_t)k * a.npad];
+            double dxd = xi - a.x[j];
+            double dyd = yi - a.y[j];
+            double dzd = zi - a.z[j];
+            dxd -= a.lx * rint(dxd * a.ilx);
+            dyd -= a.ly * rint(dyd * a.ily);
+            dzd -= a.lz * rint(dzd * a.ilz);
+            // the cutoff test stays in FP64 so in/out decisions match the plain path
+            const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
+            if (r2d < a.rc_sq) {
+                const float dx = static_cast<float>(dxd), dy = static_cast<float>(dyd),
+                            dz = static_cast<float>(dzd);
+                const float r2 = static_cast<float>(r2d);
+                float eps24 = eps24s, eps4 = eps4s, sig6 = sig6s, shift = shifts;
+                if (TYPED) {
+                    const double4 t = a.tab[ti * a.ntypes + a.type[j]];
+                    eps24 = static_cast<float>(t.x);
+                    eps4 = static_cast<float>(t.y);
+                    sig6 = static_cast<float>(t.z);
+                    shift = static_cast<float>(t.w);
+                }
+                const float r2inv = __builtin_amdgcn_rcpf(r2);
+                const float r6inv = r2inv * r2inv * r2inv * sig6;
+                const float fpair = eps24 * r2inv * (2.0f * r6inv * r6inv - r6inv);
+                const float fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
+                fxi += fxp;
+                fyi += fyp;
+                fzi += fzp;
+                if (EF)
+                    acc[0] += static_cast<double>(eps4 * (r6inv * r6inv - r6inv) - shift);
+                if (VF) {
+                    acc[1] += static_cast<double>(dx * fxp);
+                    acc[2] += static_cast<double>(dy * fyp);
+                    acc[3] += static_cast<double>(dz * fzp);
+                    acc[4] += static_cast<double>(dx * fyp);
+                    acc[5] += static_cast<double>(dx * fzp);
+                    acc[6] += static_cast<double>(dy * fzp);
+                }
+            }
+        }
+        a.fx[i] = static_cast<double>(fxi);
+        a.fy[i] = static_cast<double>(fyi);
+        a.fz[i] = static_cast<double>(fzi);
+    }
+    if (EF || VF) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            acc[k] *= 0.5;
+        block_partials(acc, a.partials);
+    }
+}
+
+#define LJ_DISPATCH(KERNEL, EF, VF, TY, GRID, ST, N, ARGS)                                   \
+    do {                                                                                      \
+        if (EF) {                                                                             \
+            if (VF) {                                                                         \
+                if (TY) KERNEL<true, true, true><<<GRID, kBlock, 0, ST>>>(N, ARGS);           \
+                else KERNEL<true, true, false><<<GRID, kBlock, 0, ST>>>(N, ARGS);             \
+            } else {                                                                          \
+                if (TY) KERNEL<true, false, true><<<GRID, kBlock, 0, ST>>>(N, ARGS);          \
+                else KERNEL<true, false, false><<<GRID, kBlock, 0, ST>>>(N, ARGS);            \
+            }                                                                                 \
+        } else {                                                                              \
+            if (VF) {                                                                         \
+                if (TY) KERNEL<false, true, true><<<GRID, kBlock, 0, ST>>>(N, ARGS);          \
+                else KERNEL<false, true, false><<<GRID, kBlock, 0, ST>>>(N, ARGS);            \
+            } else {                                                                          \
+                if (TY) KERNEL<false, false, true><<<GRID, kBlock, 0, ST>>>(N, ARGS);         \
+                else KERNEL<false, false, false><<<GRID, kBlock, 0, ST>>>(N, ARGS);           \
+            }                                                                                 \
+        }                                                                                     \
+    } while (0)
+
+} // namespace
+
+namespace atvh {
+
+int pair_grid(const atvh_system* s) { return grid_capped(s->n); }
+
+void pair_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
+    if (!s->pair_configured)
+        throw Error(ATVH_E_STATE, "pair style not configured");
+    if (!s->nl_built)
+        throw Error(ATVH_E_STATE, "PairLJCut: no neighbor list (add a NeighborList and build it)");
+    if (s->lj_cutoff > s->cutoff + 1e-12)
+        throw Error(ATVH_E_ARG, "PairLJCut: pair cutoff exceeds the neighbor-list cutoff");
+    ScopedTimer tm(s->ctx, "pair");
+    LJArgs a;
+    a.x = s->x;
+    a.y = s->y;
+    a.z = s->z;
+    a.type = s->type;
+    a.nbr = s->nbr;
+    a.numneigh = s->numneigh;
+    a.npad = s->npad;
+    a.lx = s->box[0];
+    a.ly = s->box[1];
+    a.lz = s->box[2];
+    a.ilx = 1.0 / s->box[0];
+    a.ily = 1.0 / s->box[1];
+    a.ilz = 1.0 / s->box[2];
+    const double rc = s->lj_cutoff;
+    a.rc_sq = rc * rc;
+    const double sig6 = s->lj_sigma * s->lj_sigma * s->lj_sigma * s->lj_sigma * s->lj_sigma * s->lj_sigma;
+    const double r2inv = 1.0 / a.rc_sq;
+    const double r6inv = r2inv * r2inv * r2inv * sig6;
+    a.eps24 = 24.0 * s->lj_eps;
+    a.eps4 = 4.0 * s->lj_eps;
+    a.sig6 = sig6;
+    a.vshift = 4.0 * s->lj_eps * (r6inv * r6inv - r6inv);
+    a.tab = reinterpret_cast<const double4*>(s->d_ljtab);
+    a.ntypes = s->lj_ntypes;
+    a.fx = s->fx;
+    a.fy = s->fy;
+    a.fz = s->fz;
+    a.partials = s->partials;
+    const bool typed = s->lj_ntypes > 0;
+    const int g = pair_grid(s);
+    hipStream_t st = s->ctx->stream;
+    if (precision == ATVH_PREC_MIXED)
+        LJ_DISPATCH(k_lj_mixed, eflag, vflag, typed, g, st, s->n, a);
+    else
+        LJ_DISPATCH(k_lj_fp64, eflag, vflag, typed, g, st, s->n, a);
+    ATVH_HIP(hipGetLastError());
+}
+
+void pair_free(atvh_system* s) { dev_free(s->d_ljtab); }
+
+} // namespace atvh
+
+#define API_TRY try {
+#define API_CATCH(C)                                                                    \
+    }                                                                                   \
+    catch (const atvh::Error& e_) {                                                     \
+        if (C)                                                                          \
+            (C)->err = e_.what();                                                       \
+        return e_.code;                                                                 \
+    }                                                                                   \
+    catch (const std::exception& e_) {                                                  \
+        if (C)                                                                          \
+            (C)->err = e_.what();                                                       \
+        return ATVH_E_ARG;                                                              \
+    }
+
+extern "C" {
+
+int atvh_pair_lj_configure(atvh_system* s, double eps, double sigma, double cutoff, int ntypes,
+                           const double* eps_tab, const double* sigma_tab) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    if (!(cutoff > 0.0))
+        throw Error(ATVH_E_ARG, "PairLJCut: cutoff must be positive");
+    if (ntypes < 0 || ntypes > kMaxTypes)
+        throw Error(ATVH_E_ARG, "PairLJCut: ntypes out of range");
+    if (ntypes > 0 && (!eps_tab || !sigma_tab))
+        throw Error(ATVH_E_ARG, "PairLJCut: the per-type form needs eps and sigma tables");
+    ATVH_HIP(hipSetDevice(c->device));
+    s->lj_eps = eps;
+    s->lj_sigma = sigma;
+    s->lj_cutoff = cutoff;
+    s->lj_ntypes = ntypes;
+    dev_free(s->d_ljtab);
+    if (ntypes > 0) {
+        // finalize_type_table_ (pair_lj_cut.cpp:257-298): sigma^6 and the
+        // energy shift per pair; 24eps and 4eps prefactors.
+        std::vector<double> tab(4 * ntypes * ntypes);
+        for (int ab = 0; ab < ntypes * ntypes; ++ab) {
+            const double e = eps_tab[ab], sg = sigma_tab[ab];
+            const double s2 = sg * sg;
+            const double s6 = s2 * s2 * s2;
+            const double r2inv = 1.0 / (cutoff * cutoff);
+            const double r6inv = r2inv * r2inv * r2inv * s6;
+            tab[4 * ab + 0] = 24.0 * e;
+            tab[4 * ab + 1] = 4.0 * e;
+            tab[4 * ab + 2] = s6;
+            tab[4 * ab + 3] = 4.0 * e * (r6inv * r6inv - r6inv);
+        }
+        dev_alloc(&s->d_ljtab, tab.size());
+        ATVH_HIP(hipMemcpy(s->d_ljtab, tab.data(), tab.size() * sizeof(double),
+                           hipMemcpyHostToDevice));
+    }
+    s->pair_configured = true;
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+int atvh_pair_compute(atvh_system* s, int precision, int eflag, int vflag) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    if (precision != ATVH_PREC_FP64 && precision != ATVH_PREC_MIXED)
+        throw Error(ATVH_E_ARG, "atvh_pair_compute: unknown precision");
+    ATVH_HIP(hipSetDevice(c->device));
+    pair_enqueue(s, precision, eflag != 0, vflag != 0);
+    if (eflag || vflag)
+        energy_enqueue_finalize(s, nullptr, nullptr);
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+} // extern "C"

@@ -1,0 +1,119 @@
+"""Synthetic benchmark workloads in the reference's metal units.
+
+Reduced Lennard-Jones quantities map to metal units (eV, Å, fs, amu, K) with
+eps = 1 eV, sigma = 1 Å, m = 1 amu:
+    T   = T* / kB                     (kB = 8.617333262e-5 eV/K)
+    tau = sigma sqrt(m/eps) = sqrt(103.6427) fs = 10.1805 fs
+    dt  = dt* tau,  tau_NH = tau_NH* tau
+    q   = q* / sqrt(14.399645) e      (so q*q*/r* is in units of eps)
+The generators are the library's own (atvh_util_*), so the GPU path and the
+CPU checker consume bit-identical inputs.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from ._abi import dptr, iptr
+
+KB = 8.617333262e-5
+MVV2E = 103.6427
+TAU_FS = math.sqrt(MVV2E)
+QQR2E = 14.399645
+
+
+@dataclass
+class LJWorkload:
+    name: str
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    vx: np.ndarray
+    vy: np.ndarray
+    vz: np.ndarray
+    mass: np.ndarray
+    box: np.ndarray
+    cutoff: float = 2.5
+    skin: float = 0.3
+    dt: float = 0.005 * TAU_FS
+    eps: float = 1.0
+    sigma: float = 1.0
+    T: float = 0.72 / KB
+    tau_T: float = 0.5 * TAU_FS
+    thermostat: int = 0
+    every: int = 1
+    delay: int = 0
+    check: int = 1
+    type_: np.ndarray | None = None
+    charge: np.ndarray | None = None
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self):
+        return self.x.size
+
+
+def fcc_lattice(nx, ny, nz, a, nmax=-1):
+    lib = _abi.load()
+    n = lib.atvh_util_fcc_lattice(nx, ny, nz, a, nmax, None, None, None)
+    x, y, z = (np.empty(n) for _ in range(3))
+    lib.atvh_util_fcc_lattice(nx, ny, nz, a, nmax, dptr(x), dptr(y), dptr(z))
+    return x, y, z
+
+
+def rocksalt_lattice(nx, ny, nz, spacing, q0):
+    lib = _abi.load()
+    n = lib.atvh_util_rocksalt_lattice(nx, ny, nz, spacing, None, None, None, None, None, q0)
+    x, y, z, q = (np.empty(n) for _ in range(4))
+    t = np.empty(n, np.int32)
+    lib.atvh_util_rocksalt_lattice(nx, ny, nz, spacing, dptr(x), dptr(y), dptr(z), iptr(t), dptr(q), q0)
+    return x, y, z, t, q
+
+
+def init_velocity(mass, T, seed, ndof=None):
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    n = mass.size
+    if ndof is None:
+        ndof = 3 * n - 3
+    vx, vy, vz = (np.empty(n) for _ in range(3))
+    rc = _abi.load().atvh_util_init_velocity(n, dptr(mass), T, seed, ndof, dptr(vx), dptr(vy), dptr(vz))
+    if rc != 0:
+        raise _abi.HipError(rc, "atvh_util_init_velocity")
+    return vx, vy, vz
+
+
+def lj_fcc(ncell, density=0.8442, T_star=0.72, seed=87287, thermostat=0, name=None):
+    """FCC LJ fluid: ncell^3 cells, 4 atoms per cell, seeded MB velocities."""
+    a = (4.0 / density) ** (1.0 / 3.0)
+    x, y, z = fcc_lattice(ncell, ncell, ncell, a)
+    n = x.size
+    L = ncell * a
+    mass = np.ones(n)
+    T = T_star / KB
+    vx, vy, vz = init_velocity(mass, T, seed)
+    return LJWorkload(name or f"lj_fcc_{n}", x, y, z, vx, vy, vz, mass, np.array([L, L, L]), T=T,
+                      thermostat=thermostat, meta=dict(ncell=ncell, density=density, T_star=T_star, seed=seed))
+
+
+def lj_c0():
+    """4000-atom FCC LJ fluid, NVE (10^3 cells, rho*=0.8442, T*=0.72, seed 87287)."""
+    return lj_fcc(10, seed=87287, thermostat=0, name="lj_fcc_4000_nve")
+
+
+def lj_c1(thermostat=1):
+    """2,048,000-atom FCC LJ fluid (80^3 cells), Nose-Hoover tau*=0.5, seed 87288."""
+    return lj_fcc(80, seed=87288, thermostat=thermostat, name="lj_fcc_2048000_nvt")
+
+
+def upload(sysobj, w: LJWorkload):
+    sysobj.upload(w.x, w.y, w.z, w.mass, w.vx, w.vy, w.vz, type_=w.type_, charge=w.charge)
+
+
+def configure(sysobj, w: LJWorkload):
+    """NeighborList + PairLJCut + VelocityVerlet(+NoseHoover), as the configs state."""
+    sysobj.nlist_configure(w.cutoff, w.skin, w.every, w.delay, bool(w.check))
+    sysobj.pair_lj(w.eps, w.sigma, w.cutoff)
+    sysobj.integrator(w.dt, w.thermostat, w.T, w.tau_T, 0)

@@ -1,0 +1,164 @@
+#!/usr/bin/env python3
+"""Benchmark: atom-steps/s of the 2,048,000-atom LJ fluid (80^3 FCC cells,
+rho*=0.8442, T*=0.72, rc=2.5, skin 0.3, Nose-Hoover tau*=0.5) on one MI355X,
+mixed precision (headline `value`) beside the plain FP64 path, with the force
+RMS error of mixed against FP64, the pair kernel's roofline fraction and the
+CPU oracle timed on the host cores.
+
+A step is one full MD step of the fused loop: initial kick/drift/wrap + NH,
+device-side neighbor decide (+ rebuild when due), LJ forces, final kick + NH.
+Inputs are resident in HBM before the timed region starts.
+
+Usage: python bench.py [--gpus 1] [--steps 500] [--warmup 20]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_gpu(av, W, ctx, precision, steps, warmup):
+    w = W.lj_c1(thermostat=1)
+    s = av.System(ctx, w.n, w.box)
+    W.upload(s, w)
+    W.configure(s, w)
+    s.nlist_build()
+    s.run(warmup, precision, energy_every=0)
+    ctx.synchronize()
+    st0 = s.nlist_stats()
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    s.run(steps, precision, energy_every=0)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ctx.enable_timing(False)
+    st1 = s.nlist_stats()
+    kt = {k: ctx.kernel_time(k) for k in ("pair", "integrate", "nlist")}
+    return w, s, t1 - t0, kt, st0, st1
+
+
+def cpu_baseline(W, nsteps):
+    """The CPU oracle (FP64 restatement of the reference CPU path) on the host
+    cores: a bounded sample of the same workload."""
+    from oracle import oracle as O
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    w = W.lj_c1(thermostat=1)
+    n = w.n
+    st = dict(x=w.x.copy(), y=w.y.copy(), z=w.z.copy(), vx=w.vx.copy(), vy=w.vy.copy(), vz=w.vz.copy(),
+              fx=np.zeros(n), fy=np.zeros(n), fz=np.zeros(n), mass=w.mass.copy())
+    p = O.MDParams(dt=w.dt, cutoff=w.cutoff, skin=w.skin, every=w.every, delay=w.delay, check=w.check, eps=w.eps,
+                   sigma=w.sigma, nh=1, T_target=w.T, tau_T=w.tau_T, ndof=0, nthreads=cores, stress=0)
+    O.md_run(st, w.box, p, 1)  # warm (threads, pages)
+    t0 = time.perf_counter()
+    O.md_run(st, w.box, p, nsteps)
+    dt = time.perf_counter() - t0
+    return dict(value=n * nsteps / dt, unit="atom-steps/s", cores=cores, kind="port",
+                sample=f"{nsteps} NVT steps of the 2,048,000-atom LJ config (FP64, one list build "
+                       f"included), {dt:.1f} s on {cores} host threads")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one MI355X; multi-GPU runs are not part of it "
+              "(the single-GPU throughput is the reported value).", file=sys.stderr)
+        sys.exit(2)
+
+    import atomiverse_amd as av
+    from atomiverse_amd import workloads as W
+
+    ctx = av.Context(0)
+    res = {}
+    for name, prec in (("fp64", av.PREC_FP64), ("mixed", av.PREC_MIXED)):
+        log(f"bench: {name} run, {args.warmup} warmup + {args.steps} timed steps")
+        w, s, dt, kt, st0, st1 = run_gpu(av, W, ctx, prec, args.steps, args.warmup)
+        res[name] = dict(w=w, s=s, dt=dt, kt=kt, st0=st0, st1=st1)
+        log(f"  {name}: {dt * 1e3 / args.steps:.3f} ms/step, {w.n * args.steps / dt:.4e} atom-steps/s, "
+            f"rebuilds {st1['nrebuilds'] - st0['nrebuilds']}, kernels "
+            + ", ".join(f"{k} {v[0] / max(v[1], 1) * 1e3:.1f}us x{v[1]}" for k, v in kt.items()))
+
+    # force RMS error, mixed vs FP64, on the mixed run's final positions
+    s = res["mixed"]["s"]
+    s.compute(av.PREC_FP64, False, False)
+    f64 = s.download()
+    s.compute(av.PREC_MIXED, False, False)
+    f32 = s.download()
+    F = np.stack([f64["fx"], f64["fy"], f64["fz"]])
+    M = np.stack([f32["fx"], f32["fy"], f32["fz"]])
+    rms = float(np.sqrt(np.sum((M - F) ** 2) / np.sum(F ** 2)))
+
+    w = res["mixed"]["w"]
+    n = w.n
+    steps = args.steps
+    value = n * steps / res["mixed"]["dt"]
+    fp64_value = n * steps / res["fp64"]["dt"]
+    # roofline of the dominant kernel (pair): algorithmic bytes per launch =
+    # 24N positions + 8N list offsets/counts + 4 B per half-list entry + 24N forces
+    lh = res["mixed"]["st1"]["npairs_half"]
+    bytes_alg = 56 * n + 4 * lh
+    ms, launches = res["mixed"]["kt"]["pair"]
+    avg_s = ms / 1e3 / max(launches, 1)
+    achieved = bytes_alg / avg_s / 1e9
+    out = {
+        "metric": "atom-steps/s, 2.048M-atom LJ, mixed vs FP64 at stated err; p50/p99 latency",
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["mixed"]["dt"] * 1e3 / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 pair forces, fp64 integration/accumulation",
+        "data": "synthetic (seeded FCC lattice + Maxwell-Boltzmann velocities)",
+        "config": {"workload": "lj_fcc_2048000_nvt", "atoms": n, "precision": "mixed", "cutoff": 2.5,
+                   "skin": 0.3, "thermostat": "nose-hoover tau*=0.5", "dt_fs": w.dt,
+                   "rebuilds_in_timed_steps": res["mixed"]["st1"]["nrebuilds"] - res["mixed"]["st0"]["nrebuilds"]},
+        "fp64_value": fp64_value,
+        "fp64_ms_per_step": res["fp64"]["dt"] * 1e3 / steps,
+        "mixed_over_fp64": value / fp64_value,
+        "force_rms_rel_err_mixed_vs_fp64": rms,
+        "roofline": {"bound": "hbm", "kernel": "k_lj_mixed", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_alg_per_launch": bytes_alg, "avg_launch_us": avg_s * 1e6},
+        "kernel_avg_us": {name: {k: v[0] * 1e3 / max(v[1], 1) for k, v in r["kt"].items()}
+                          for name, r in res.items()},
+    }
+    for r in res.values():
+        r["s"].close()
+    if not args.no_cpu:
+        log(f"bench: cpu baseline ({args.cpu_steps} oracle steps)")
+        out["cpu_baseline"] = cpu_baseline(W, args.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

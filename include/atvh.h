@@ -1,0 +1,169 @@
+/* atvh.h — C ABI of libatv_hip.so, the MI355X (gfx950) classical-MD backend.
+ *
+ * The hip:: C++ components (include/atomiverse/hip/) sit on top of this ABI,
+ * exactly where the reference's cuda:: components call their kernels
+ * directly. Conventions:
+ *   - every function returns ATVH_OK (0) or a negative ATVH_E_* code and
+ *     leaves a message for atvh_last_error(); no C++ exception crosses it;
+ *   - host buffers are caller-owned and read/written only during the call;
+ *     device buffers are library-owned;
+ *   - all work of a context is ordered on the context's one HIP stream; a
+ *     context is driven by one host thread at a time;
+ *   - per-atom host arrays are in the caller's row order ("frame order");
+ *     the library keeps atoms cell-sorted on the device and maps back by tag.
+ * Units are the reference's metal units (Å, eV, fs, amu, K, e).
+ */
+#ifndef ATVH_H
+#define ATVH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ATVH_OK 0
+#define ATVH_E_ARG (-1)      /* invalid argument */
+#define ATVH_E_HIP (-2)      /* HIP runtime error */
+#define ATVH_E_STATE (-3)    /* call out of order (e.g. no neighbor list) */
+#define ATVH_E_CAPACITY (-4) /* a device capacity was exceeded */
+#define ATVH_E_NOMEM (-5)    /* device allocation failed */
+
+#define ATVH_PREC_FP64 0  /* plain: FP64 throughout */
+#define ATVH_PREC_MIXED 1 /* FP32 pair/PME math; FP64 integration + accumulation */
+
+#define ATVH_PHASE_INITIAL 0 /* md::Phase::initial_integrate */
+#define ATVH_PHASE_FINAL 1   /* md::Phase::final_integrate */
+
+typedef struct atvh_ctx atvh_ctx;
+typedef struct atvh_system atvh_system;
+
+/* ---- context --------------------------------------------------------------
+ * Replaces cuda::Backend::init's stream set-up (src/cuda/driver/driver.cu:25-32)
+ * and the ATV_CUDA_CHECK error path (include/atomiverse/cuda/device/error.cuh).
+ */
+int atvh_ctx_create(int device, atvh_ctx** out);
+int atvh_ctx_destroy(atvh_ctx* ctx);
+const char* atvh_last_error(const atvh_ctx* ctx);
+/* Use an external stream (hipStream_t passed as void*); NULL restores the
+ * context's own stream. */
+int atvh_ctx_set_stream(atvh_ctx* ctx, void* stream);
+void* atvh_ctx_stream(atvh_ctx* ctx);
+int atvh_ctx_synchronize(atvh_ctx* ctx);
+/* Per-kernel timing on the context stream with HIP events (off by default).
+ * name = kernel family ("pair", "integrate", "nlist", "pme", ...). */
+int atvh_ctx_enable_timing(atvh_ctx* ctx, int enable);
+int atvh_ctx_kernel_time(atvh_ctx* ctx, const char* name, double* total_ms,
+                         int64_t* launches);
+int atvh_ctx_reset_timing(atvh_ctx* ctx);
+
+/* ---- frame (device state) ---------------------------------------------------
+ * Replaces cuda::Frame + cpu::to_cuda / cuda::to_cpu
+ * (include/atomiverse/cuda/frame/frame.cuh:71-200;
+ *  src/cuda/frame/transfer.cu:145-475). Orthorhombic box [0,L).
+ * Positions and velocities are FP64 in both precision modes.
+ */
+int atvh_system_create(atvh_ctx* ctx, int64_t n, const double box[3],
+                       atvh_system** out);
+int atvh_system_destroy(atvh_system* sys);
+int64_t atvh_system_natoms(const atvh_system* sys);
+/* Upload frame-order arrays. vx..vz, fx..fz, type, charge may be NULL
+ * (zero / type 0 / neutral). mass[n] is required; atoms of one type must
+ * share one mass. */
+int atvh_frame_upload(atvh_system* sys, const double* x, const double* y,
+                      const double* z, const double* vx, const double* vy,
+                      const double* vz, const double* fx, const double* fy,
+                      const double* fz, const double* mass, const int* type,
+                      const double* charge);
+/* Download into frame order; any pointer may be NULL. Positions come back
+ * wrapped into [0, L) exactly as wrap_orthorhombic leaves them. */
+int atvh_frame_download(atvh_system* sys, double* x, double* y, double* z,
+                        double* vx, double* vy, double* vz, double* fx,
+                        double* fy, double* fz);
+int atvh_system_box(const atvh_system* sys, double box[3]);
+
+/* ---- neighbor list ------------------------------------------------------------
+ * Replaces cuda::NeighborList (include/atomiverse/cuda/neighbor/neighbor_list.cuh:93-229)
+ * with the reference CPU semantics: CellList::CellList/compute_grid/
+ * build_stencil (src/cpu/neighbor/cell_list.cpp:24-232), counting-sort
+ * binning (:234-314), the half-list pair predicate of refresh_half
+ * (:539-738) and the LAMMPS decide cadence (neighbor_list.h:79-141).
+ * skin < 0 means 0.3*cutoff.
+ */
+int atvh_nlist_configure(atvh_system* sys, double cutoff, double skin,
+                         int every, int delay, int check);
+/* Unconditional build (NeighborList::build at component build time). */
+int atvh_nlist_build(atvh_system* sys);
+/* LAMMPS Neighbor::decide on the device (++ago, displacement check) followed
+ * by the rebuild it decides; no host synchronisation. */
+int atvh_nlist_decide_and_rebuild(atvh_system* sys);
+/* Host read of the rebuild count (synchronises the stream). */
+int atvh_nlist_stats(atvh_system* sys, int64_t* nrebuilds, int64_t* npairs_half,
+                     int* max_neighbors, int* capacity);
+/* Export the current pair set as unordered tag pairs (a < b, a and b are
+ * frame rows), sorted ascending by (a, b). Synchronises. If cap is too small
+ * returns ATVH_E_CAPACITY with *npairs = the needed count. */
+int atvh_nlist_export_pairs(atvh_system* sys, int64_t* pairs, int64_t cap,
+                            int64_t* npairs);
+
+/* ---- pair styles ----------------------------------------------------------------
+ * PairLJCut (src/cpu/potentials/pair_lj_cut.cpp:31-42, 257-298, 300-830;
+ * CUDA counterpart src/cuda/potentials/pair_lj_cut.cu:89-512).
+ * Single-type: ntypes = 0 and (eps, sigma). Per-type: ntypes > 0 with
+ * ntypes*ntypes eps/sigma tables (Lorentz-Berthelot already filled).
+ */
+int atvh_pair_lj_configure(atvh_system* sys, double eps, double sigma,
+                           double cutoff, int ntypes, const double* eps_tab,
+                           const double* sigma_tab);
+/* One force evaluation (ZeroForces folded in: forces are overwritten).
+ * eflag/vflag request the energy and 6-component virial. */
+int atvh_pair_compute(atvh_system* sys, int precision, int eflag, int vflag);
+/* Last computed potential energy (eV) and Voigt virial (eV); synchronises. */
+int atvh_energy(atvh_system* sys, double* pe, double* virial6, double* ke);
+
+/* ---- integration ------------------------------------------------------------------
+ * VelocityVerlet (src/cpu/dynamics/velocity_verlet.cpp:24-68) and single-chain
+ * NoseHoover (src/cpu/dynamics/nose_hoover.cpp:31-170); CUDA counterparts
+ * src/cuda/dynamics/velocity_verlet.cu:22-90, nose_hoover.cu:50-150.
+ * thermostat: 0 = NVE, 1 = Nose-Hoover (T in K, tau in fs). ndof <= 0 -> 3N-3.
+ */
+int atvh_integrator_configure(atvh_system* sys, double dt, int thermostat,
+                              double T_target, double tau_T, int64_t ndof);
+/* One integration phase (ATVH_PHASE_INITIAL: kick + drift + wrap + NH;
+ * ATVH_PHASE_FINAL: kick + NH). */
+int atvh_step_vvnh(atvh_system* sys, int phase);
+/* Nose-Hoover state (xi) read/write; synchronises. */
+int atvh_thermostat_state(atvh_system* sys, double* xi_out, const double* xi_in);
+
+/* ---- fused MD loop -----------------------------------------------------------------
+ * nsteps of Runtime::step_verlet_ (src/cpu/driver/runtime.cpp:452-502) queued
+ * on the stream with no host synchronisation: initial integrate, device-side
+ * decide + rebuild, force, final integrate, kinetic energy.
+ * energy_every > 0 requests PE/virial on every energy_every-th step and the
+ * last step (LAMMPS eflag cadence); 0 = only the last step.
+ */
+int atvh_md_run(atvh_system* sys, int nsteps, int precision, int energy_every);
+/* Device-recorded per-step potential/kinetic energy of the last atvh_md_run
+ * (entries for steps without an energy request hold NaN). Synchronises. */
+int atvh_md_thermo(atvh_system* sys, double* pe, double* ke, int nsteps);
+
+/* ---- synthetic inputs (host only) ---------------------------------------------------
+ * Lattice generators for the benchmark workloads and InitVelocity::build
+ * (src/cpu/dynamics/init.cpp:107-215): mt19937 + normal(0,1)/sqrt(m), COM
+ * removal, rescale to T with ndof. Arrays may be NULL to query the count.
+ */
+/* FCC: 4 sites per cell, cells x-fastest, basis (0,0,0)(.5,.5,0)(.5,0,.5)(0,.5,.5);
+ * stops after nmax sites when nmax >= 0 (trimmed lattice). Returns the count. */
+int64_t atvh_util_fcc_lattice(int nx, int ny, int nz, double a, int64_t nmax, double* x,
+                              double* y, double* z);
+/* Rock salt on a simple-cubic grid: type = (ix+iy+iz)&1, charge +q0 / -q0. */
+int64_t atvh_util_rocksalt_lattice(int nx, int ny, int nz, double spacing, double* x,
+                                   double* y, double* z, int* type, double* charge, double q0);
+int atvh_util_init_velocity(int64_t n, const double* mass, double T, uint32_t seed,
+                            int64_t ndof, double* vx, double* vy, double* vz);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ATVH_H */

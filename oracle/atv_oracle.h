@@ -1,0 +1,124 @@
+/* atv_oracle.h — CPU restatement of the reference classical-MD path (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP backend:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * it, and never as the thing measured or shipped. The product path
+ * (atomiverse_amd/csrc, libatv_hip.so) neither links nor calls it.
+ *
+ * Every function restates one reference function operation for operation; the
+ * citation is on each definition in atv_oracle.cpp. Plain C ABI so ctypes can
+ * bind it; no exceptions cross it (errors are negative return codes).
+ */
+#ifndef ATV_ORACLE_H
+#define ATV_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- geometry / inputs ---------------------------------------------------- */
+
+/* FCC lattice, nx*ny*nz conventional cells of edge a, 4 sites per cell,
+ * sites emitted cell by cell (x fastest over cells), basis order
+ * (0,0,0) (½,½,0) (½,0,½) (0,½,½). Returns the atom count. */
+int64_t orc_fcc_lattice(int nx, int ny, int nz, double a, double* x, double* y,
+                        double* z);
+
+/* InitVelocity::build: mt19937(seed) + normal_distribution(0,1), vx,vy,vz per
+ * atom scaled by 1/sqrt(m); COM removal; rescale to T [K] with ndof. */
+void orc_init_velocity(int64_t n, const double* mass, double T, uint32_t seed,
+                       int64_t ndof, double* vx, double* vy, double* vz);
+
+/* ---- cell list / Verlet list (CellList) ----------------------------------- */
+
+/* compute_grid + build_stencil for a single-rank orthorhombic box.
+ * out_nc[3], out_cs[3], out_reach[3]; returns number of half-stencil offsets,
+ * *out_nfull gets the full-stencil count. */
+int orc_cell_grid(const double box[3], double cutoff_skin, int* out_nc,
+                  double* out_cs, int* out_reach, int* out_nfull);
+
+/* bin_atoms: atom_cell[n], cell_count[ncells], cell_offset[ncells],
+ * binned[n] (counting sort, ascending i inside a cell). */
+int orc_bin_atoms(int64_t n, const double* x, const double* y, const double* z,
+                  const double box[3], double cutoff_skin, int* atom_cell,
+                  int* cell_count, int* cell_offset, int* binned);
+
+/* refresh_half (serial shift-form semantics; nthreads>1 computes the same CSR
+ * with rows split across threads). Returns total pairs, or -(needed) if
+ * `cap` is too small (no partial output then). */
+int64_t orc_build_half(int64_t n, const double* x, const double* y,
+                       const double* z, const double box[3], double cutoff_skin,
+                       int nthreads, int* numneigh, int64_t* firstneigh,
+                       int* neighbors, int64_t cap);
+
+/* needs_rebuild: max min-image |x - x_save|^2 * 4 > skin^2. Returns 0/1 and
+ * writes the max squared displacement. */
+int orc_needs_rebuild(int64_t n, const double* x, const double* y,
+                      const double* z, const double* xs, const double* ys,
+                      const double* zs, const double box[3], double skin,
+                      double* max_disp_sq);
+
+/* ---- pair styles ----------------------------------------------------------- */
+
+/* PairLJCut::compute on a half CSR. Forces are ACCUMULATED into f*, energy
+ * and the 6-component Voigt virial (xx yy zz xy xz yz) are accumulated into
+ * *energy and virial[6]. ntypes == 0 -> single (eps, sigma); ntypes > 0 ->
+ * per-type eps_tab/sigma_tab (ntypes*ntypes, Lorentz-Berthelot already
+ * applied by the caller or by orc_lj_mix). */
+void orc_pair_lj(int64_t n, const double* x, const double* y, const double* z,
+                 const double box[3], const int* numneigh,
+                 const int64_t* firstneigh, const int* neighbors, double eps,
+                 double sigma, double cutoff, int ntypes, const int* type,
+                 const double* eps_tab, const double* sigma_tab, int nthreads,
+                 double* fx, double* fy, double* fz, double* energy,
+                 double* virial);
+
+/* Lorentz-Berthelot fill of the off-diagonal of an ntypes x ntypes table
+ * whose diagonal is set (PairLJCut::finalize_type_table_). */
+void orc_lj_mix(int ntypes, double* eps_tab, double* sigma_tab);
+
+/* ---- integrators ----------------------------------------------------------- */
+
+typedef struct orc_md_params {
+    double dt;       /* fs */
+    double cutoff;   /* Å (LJ and list) */
+    double skin;     /* Å */
+    int every, delay, check;
+    double eps, sigma;      /* single-type LJ */
+    int nh;                 /* 1 = single-chain Nose-Hoover NVT */
+    double T_target, tau_T; /* K, fs */
+    int64_t ndof;           /* <= 0 -> 3N - 3 */
+    int nthreads;
+    int stress;             /* compute the pair virial */
+} orc_md_params;
+
+typedef struct orc_md_state {
+    double xi;       /* NH thermostat variable */
+    int64_t step;    /* cumulative step index */
+    int64_t nrebuilds;
+    int64_t ago;
+} orc_md_state;
+
+/* cpu::md::MD::run with VelocityVerlet (+ NoseHoover) + NeighborList +
+ * PairLJCut, phase order of Runtime::step_verlet_. Forces f* are the frame's
+ * (carried across calls; zero for a fresh frame). `state` is in/out: a
+ * zero-initialised state means "build components" (initial list build).
+ * Per-step potential and kinetic energy are written to pe[nsteps], ke[nsteps]
+ * when non-null. virial[6] gets the last step's pair virial if non-null. */
+int orc_md_run(int64_t n, double* x, double* y, double* z, double* vx,
+               double* vy, double* vz, double* fx, double* fy, double* fz,
+               const double* mass, const double box[3],
+               const orc_md_params* p, orc_md_state* state, int nsteps,
+               double* pe, double* ke, double* virial);
+
+/* KineticEnergy::launch: 0.5 * mvv2e * sum m v^2 */
+double orc_kinetic_energy(int64_t n, const double* mass, const double* vx,
+                          const double* vy, const double* vz);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
