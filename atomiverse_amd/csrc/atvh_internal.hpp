@@ -52,7 +52,7 @@ struct DevCtl {
     int overflow;     // neighbor capacity exceeded (count of atoms)
     int max_nbr;      // max neighbors of the last build
     int always;       // constant 1: predicate for unconditional launches
-    int pad_;
+    int halo_overflow; // bricks whose halo exceeded the LDS capacity
     long long nbr_total; // full-list entries of the last build
 };
 
@@ -121,9 +121,20 @@ struct atvh_system {
     int* perm = nullptr;
     int* scan_sums = nullptr;
     int nscan_blocks = 0;
-    int* nbr = nullptr;      // ELL, column-major: nbr[k*npad + i]
+    int* nbr = nullptr;      // small-grid mode: ELL, column-major: nbr[k*npad + i]
     int* numneigh = nullptr; // [n]
     int cap = 0;
+    // brick mode (every axis has >= 2*reach+1 cells)
+    bool brick_mode = false;
+    int brick_pick = 0;     // index of the chosen brick candidate
+    int brick_pick_min = 0; // smallest candidate index allowed (raised on halo overflow)
+    int bdim[3] = {0, 0, 0};
+    int nb[3] = {0, 0, 0};
+    int nbricks = 0;
+    int* cell_rank = nullptr;       // reference cell index -> layout rank
+    int* brick_cell_base = nullptr; // [nbricks+1]
+    uint16_t* nbr16 = nullptr;      // per brick, column-major: [a0*cap + k*nat + li]
+    uint16_t* self16 = nullptr;     // [n] own row in the brick's halo
 
     // pair style
     bool pair_configured = false;
@@ -194,6 +205,8 @@ void nl_setup_grid(atvh_system* s);
 void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces);
 void nl_enqueue_decide(atvh_system* s);
 void nl_free(atvh_system* s);
+void nl_realloc_capacity(atvh_system* s, int cap);
+void nl_check_ctl(const DevCtl& h);
 
 // pair_lj.hip
 int pair_grid(const atvh_system* s);

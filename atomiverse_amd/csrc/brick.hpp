@@ -1,0 +1,134 @@
+// brick.hpp — brick decomposition of the cell grid and the per-workgroup halo
+// staged in LDS (shared by the list build and the pair kernels).
+//
+// The reference's cell grid (CellList::compute_grid, cell_list.cpp:83-154) is
+// kept exactly; only the ORDER in which cells (and their atoms) are laid out
+// in memory changes: cells are grouped into bricks of bdim cells per axis,
+// bricks are numbered z-fastest, and cells inside a brick z-fastest. A brick's
+// atoms are therefore one contiguous slot range, and its halo — the brick
+// widened by the stencil reach R on every side — is a small cube of cells
+// whose atoms one workgroup copies into LDS. Neighbour indices are 16-bit
+// positions inside that halo, and halo positions are stored as the periodic
+// images the pair actually uses, so the force loop needs no minimum image.
+#pragma once
+
+#include "atvh_internal.hpp"
+
+namespace atvh {
+
+constexpr int kHaloCap = 2048;      // atoms staged per halo (LDS rows)
+constexpr int kHaloCellCap = 768;   // halo cells per brick ((4+4)*(4+4)*(5+4)=576)
+
+struct BrickGeom {
+    int nc[3];    // cells per axis (reference grid)
+    int R[3];     // stencil reach per axis
+    int bdim[3];  // nominal brick size in cells
+    int nb[3];    // bricks per axis
+    double L[3];
+    double cs[3];
+    const int* cell_rank;       // reference cell index -> layout rank
+    const int* brick_cell_base; // [nbricks+1] first rank of each brick
+};
+
+struct HaloTable {
+    int dims[3];   // halo cells per axis
+    int o[3];      // brick origin (cells)
+    int d[3];      // brick size (cells)
+    int ncell;     // halo cells
+    int natoms;    // halo atoms
+};
+
+__device__ __forceinline__ void brick_coords(const BrickGeom& g, int b, int& bx, int& by, int& bz) {
+    bz = b % g.nb[2];
+    by = (b / g.nb[2]) % g.nb[1];
+    bx = b / (g.nb[1] * g.nb[2]);
+}
+
+// Fill the halo table of brick b into LDS: per halo cell its first global
+// slot (h_start), its LDS offset (h_off, exclusive scan of the counts) and
+// the periodic shift code of its image (h_shift: 3 x 2 bits, 0 = -L, 1 = 0,
+// 2 = +L). All threads of the block participate; ends with a barrier.
+__device__ __forceinline__ void build_halo_table(const BrickGeom& g, const int* __restrict__ cell_start,
+                                                 int b, HaloTable& ht, int* h_start, int* h_off,
+                                                 int* h_shift, int* scan_tmp) {
+    int bx, by, bz;
+    brick_coords(g, b, bx, by, bz);
+    ht.o[0] = bx * g.bdim[0];
+    ht.o[1] = by * g.bdim[1];
+    ht.o[2] = bz * g.bdim[2];
+    for (int a = 0; a < 3; ++a) {
+        ht.d[a] = min(g.bdim[a], g.nc[a] - ht.o[a]);
+        ht.dims[a] = ht.d[a] + 2 * g.R[a];
+    }
+    ht.ncell = ht.dims[0] * ht.dims[1] * ht.dims[2];
+    const int nthr = blockDim.x;
+    // per-thread chunk of consecutive halo cells, then a block scan of chunk sums
+    const int per = (ht.ncell + nthr - 1) / nthr;
+    const int h0 = threadIdx.x * per;
+    const int h1 = min(ht.ncell, h0 + per);
+    int local = 0;
+    for (int h = h0; h < h1; ++h) {
+        const int hz = h % ht.dims[2];
+        const int hy = (h / ht.dims[2]) % ht.dims[1];
+        const int hx = h / (ht.dims[1] * ht.dims[2]);
+        int u[3] = {ht.o[0] + hx - g.R[0], ht.o[1] + hy - g.R[1], ht.o[2] + hz - g.R[2]};
+        int code = 0;
+        for (int a = 0; a < 3; ++a) {
+            int s = 1;
+            if (u[a] < 0) {
+                u[a] += g.nc[a];
+                s = 0;
+            } else if (u[a] >= g.nc[a]) {
+                u[a] -= g.nc[a];
+                s = 2;
+            }
+            code |= s << (2 * a);
+        }
+        const int c = u[0] * g.nc[1] * g.nc[2] + u[1] * g.nc[2] + u[2];
+        const int r = g.cell_rank[c];
+        const int s0 = cell_start[r], s1 = cell_start[r + 1];
+        h_start[h] = s0;
+        h_shift[h] = code;
+        h_off[h] = s1 - s0; // count, scanned below
+        local += s1 - s0;
+    }
+    // block exclusive scan of `local`
+    scan_tmp[threadIdx.x] = local;
+    __syncthreads();
+    for (int off = 1; off < nthr; off <<= 1) {
+        const int add = threadIdx.x >= off ? scan_tmp[threadIdx.x - off] : 0;
+        __syncthreads();
+        scan_tmp[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int run = scan_tmp[threadIdx.x] - local;
+    ht.natoms = scan_tmp[nthr - 1];
+    for (int h = h0; h < h1; ++h) {
+        const int cnt = h_off[h];
+        h_off[h] = run;
+        run += cnt;
+    }
+    if (threadIdx.x == 0)
+        h_off[ht.ncell] = ht.natoms;
+    __syncthreads();
+}
+
+// Halo cell of LDS row q (binary search over h_off).
+__device__ __forceinline__ int halo_cell_of(const int* h_off, int ncell, int q) {
+    int lo = 0, hi = ncell; // h_off[lo] <= q < h_off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (h_off[mid] <= q)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ double shift_of(int code, int axis, double L) {
+    const int s = (code >> (2 * axis)) & 3;
+    return s == 0 ? -L : (s == 2 ? L : 0.0);
+}
+
+} // namespace atvh

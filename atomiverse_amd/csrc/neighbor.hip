@@ -12,17 +12,21 @@
 //             while-loop min-image is used instead, as the reference does.
 //   cadence:  NeighborList::decide_reneighbor (include/atomiverse/cpu/neighbor/neighbor_list.h:128-141)
 //
-// MI355X layout: atoms are physically re-sorted by (cell, tag) at every
-// build, so cell c owns the contiguous slot range [cell_start[c],
-// cell_start[c+1]) and neighbour gathers stay in L2. The list is a FULL
-// list (both directions of every accepted pair) stored column-major
-// (ELL, nbr[k*npad + i]) so a wave's k-th loads are one coalesced 256 B line;
-// each unordered pair is tested once per side with the canonical orientation
-// above, so both sides reach the same bit-identical decision. The whole
-// rebuild is predicated on a device flag, so the host never waits for the
-// decision. All expressions here are compiled with -ffp-contract=off.
+// MI355X layout: atoms are physically re-sorted at every build by (cell
+// rank, tag), where the rank orders cells brick by brick (brick.hpp), so a
+// cell owns the contiguous slot range [cell_start[r], cell_start[r+1]) and a
+// brick owns a contiguous range of cells. The list is a FULL list (both
+// directions of every accepted pair) so each force row has a single writer;
+// every unordered pair is tested once per side in the canonical orientation
+// above, so both sides reach the same bit-identical decision. In brick mode
+// one workgroup per brick stages its halo in LDS and stores 16-bit halo rows,
+// column-major inside the brick (a wave's k-th loads are one 128 B line). The
+// small-grid fallback keeps a 32-bit column-major list over global slots.
+// The whole rebuild is predicated on a device flag, so the host never waits
+// for the decision. All expressions here are compiled with -ffp-contract=off.
 
 #include "atvh_internal.hpp"
+#include "brick.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -39,7 +43,8 @@ constexpr int kScanTile = kBlock * kScanItems; // elements per scan block
 __global__ void k_nl_bin(const int* __restrict__ pred, int64_t n, const double* __restrict__ x,
                          const double* __restrict__ y, const double* __restrict__ z, double lx,
                          double ly, double lz, double csx, double csy, double csz, int ncx, int ncy,
-                         int ncz, int* __restrict__ cell_of, int* __restrict__ count) {
+                         int ncz, const int* __restrict__ cell_rank, int* __restrict__ cell_of,
+                         int* __restrict__ count) {
     if (!*pred)
         return;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -56,7 +61,7 @@ __global__ void k_nl_bin(const int* __restrict__ pred, int64_t n, const double* 
         cz = max(0, min(cz, ncz - 1));
         const int c = cx * ncy * ncz + cy * ncz + cz;
         cell_of[i] = c;
-        atomicAdd(&count[c], 1);
+        atomicAdd(&count[cell_rank[c]], 1);
     }
 }
 
@@ -140,14 +145,14 @@ __global__ __launch_bounds__(kBlock) void k_nl_scan_add(const int* __restrict__ 
 // Counting-sort scatter: slot of every atom inside its cell (order fixed
 // afterwards by k_nl_cell_order). Also clears the histogram for next time.
 __global__ void k_nl_scatter(const int* __restrict__ pred, int64_t n,
-                             const int* __restrict__ cell_of, const int* __restrict__ start,
-                             int* __restrict__ fill, int* __restrict__ count,
-                             int* __restrict__ perm) {
+                             const int* __restrict__ cell_of, const int* __restrict__ cell_rank,
+                             const int* __restrict__ start, int* __restrict__ fill,
+                             int* __restrict__ count, int* __restrict__ perm) {
     if (!*pred)
         return;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const int c = cell_of[i];
+        const int c = cell_rank[cell_of[i]];
         const int slot = start[c] + atomicAdd(&fill[c], 1);
         perm[slot] = static_cast<int>(i);
         count[c] = 0;
@@ -249,6 +254,7 @@ __global__ void k_nl_copyback(const int* __restrict__ pred, int64_t n, CopyArgs 
 struct BuildArgs {
     const double *x, *y, *z;
     const int* cell_of;
+    const int* cell_rank;
     const int* start;
     const int4* stencil; // {di, dj, dk, kind}: kind 0 home, 1 forward, 2 backward
     int nstencil;
@@ -311,7 +317,8 @@ __global__ __launch_bounds__(kBlock) void k_nl_build(const int* __restrict__ pre
                 if (nvis < 128)
                     visited[nvis++] = cj;
             }
-            const int jb = a.start[cj], je = a.start[cj + 1];
+            const int rj = a.cell_rank[cj];
+            const int jb = a.start[rj], je = a.start[rj + 1];
             for (int j = jb; j < je; ++j) {
                 if (j == i)
                     continue;
@@ -381,6 +388,182 @@ __global__ __launch_bounds__(kBlock) void k_nl_build(const int* __restrict__ pre
     }
 }
 
+struct BrickBuildArgs {
+    BrickGeom g;
+    const double *x, *y, *z;
+    const int* cell_of;
+    const int* start;
+    const int4* stencil;
+    int nstencil;
+    double rc_sq;
+    uint16_t* nbr16;
+    uint16_t* self16;
+    int* numneigh;
+    int cap;
+    DevCtl* ctl;
+};
+
+// Brick-mode list build: one workgroup per brick. The halo's RAW positions are
+// staged in LDS (the predicate uses the reference's shift form on raw
+// coordinates, not images); each brick atom scans its stencil cells inside the
+// halo and stores accepted neighbours as 16-bit halo rows.
+__global__ __launch_bounds__(kBlock) void k_nl_build_brick(const int* __restrict__ pred,
+                                                           BrickBuildArgs a) {
+    if (!*pred)
+        return;
+    __shared__ int h_start[kHaloCellCap + 1];
+    __shared__ int h_off[kHaloCellCap + 1];
+    __shared__ int h_shift[kHaloCellCap];
+    __shared__ int scan_tmp[kBlock];
+    __shared__ double sx[kHaloCap], sy[kHaloCap], sz[kHaloCap];
+    __shared__ long long sh_tot[kBlock / 64];
+    __shared__ int sh_max[kBlock / 64], sh_over[kBlock / 64];
+    const int b = blockIdx.x;
+    HaloTable ht;
+    build_halo_table(a.g, a.start, b, ht, h_start, h_off, h_shift, scan_tmp);
+    if (ht.natoms > kHaloCap) {
+        if (threadIdx.x == 0)
+            atomicAdd(&a.ctl->halo_overflow, 1);
+        return;
+    }
+    for (int q = threadIdx.x; q < ht.natoms; q += blockDim.x) {
+        const int h = halo_cell_of(h_off, ht.ncell, q);
+        const int gs = h_start[h] + (q - h_off[h]);
+        sx[q] = a.x[gs];
+        sy[q] = a.y[gs];
+        sz[q] = a.z[gs];
+    }
+    __syncthreads();
+    const int a0 = a.start[a.g.brick_cell_base[b]];
+    const int a1 = a.start[a.g.brick_cell_base[b + 1]];
+    const int nat = a1 - a0;
+    const int nz = a.g.nc[2], ny = a.g.nc[1];
+    const int hy = ht.dims[1], hz = ht.dims[2];
+    long long my_cnt_sum = 0;
+    int my_max = 0, my_over = 0;
+    for (int li = threadIdx.x; li < nat; li += blockDim.x) {
+        const int i = a0 + li;
+        const int ci = a.cell_of[i];
+        const int lz = ci % nz - ht.o[2];
+        const int ly = (ci / nz) % ny - ht.o[1];
+        const int lx = ci / (ny * nz) - ht.o[0];
+        const int hs = ((lx + a.g.R[0]) * hy + (ly + a.g.R[1])) * hz + (lz + a.g.R[2]);
+        const int qs = h_off[hs] + (i - h_start[hs]);
+        const double xi = sx[qs], yi = sy[qs], zi = sz[qs];
+        uint16_t* __restrict__ out = a.nbr16 + (int64_t)a0 * a.cap + li;
+        int cnt = 0;
+        for (int s = 0; s < a.nstencil; ++s) {
+            const int4 o = a.stencil[s];
+            const int h = ((lx + a.g.R[0] + o.x) * hy + (ly + a.g.R[1] + o.y)) * hz + (lz + a.g.R[2] + o.z);
+            const int code = h_shift[h];
+            const double shx = shift_of(code, 0, a.g.L[0]);
+            const double shy = shift_of(code, 1, a.g.L[1]);
+            const double shz = shift_of(code, 2, a.g.L[2]);
+            const int qb = h_off[h], qe = h_off[h + 1];
+            for (int q = qb; q < qe; ++q) {
+                if (q == qs)
+                    continue;
+                double dx, dy, dz;
+                if (o.w == 2) {
+                    dx = sx[q] - (xi - shx);
+                    dy = sy[q] - (yi - shy);
+                    dz = sz[q] - (zi - shz);
+                } else {
+                    dx = xi - (sx[q] + shx);
+                    dy = yi - (sy[q] + shy);
+                    dz = zi - (sz[q] + shz);
+                }
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 < a.rc_sq) {
+                    if (cnt < a.cap)
+                        out[(int64_t)cnt * nat] = static_cast<uint16_t>(q);
+                    ++cnt;
+                }
+            }
+        }
+        a.numneigh[i] = cnt < a.cap ? cnt : a.cap;
+        a.self16[i] = static_cast<uint16_t>(qs);
+        my_cnt_sum += cnt;
+        my_max = max(my_max, cnt);
+        if (cnt > a.cap)
+            ++my_over;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        my_cnt_sum += __shfl_down(my_cnt_sum, off, 64);
+        my_max = max(my_max, __shfl_down(my_max, off, 64));
+        my_over += __shfl_down(my_over, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh_tot[wid] = my_cnt_sum;
+        sh_max[wid] = my_max;
+        sh_over[wid] = my_over;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        int m = 0, ov = 0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            t += sh_tot[w];
+            m = max(m, sh_max[w]);
+            ov += sh_over[w];
+        }
+        atomicAdd(reinterpret_cast<unsigned long long*>(&a.ctl->nbr_total), static_cast<unsigned long long>(t));
+        atomicMax(&a.ctl->max_nbr, m);
+        if (ov)
+            atomicAdd(&a.ctl->overflow, ov);
+    }
+}
+
+// Brick-mode export: decode halo rows back to global slots, emit tag pairs a<b.
+struct BrickExportArgs {
+    BrickGeom g;
+    const int* start;
+    const int* tag;
+    const uint16_t* nbr16;
+    const int* numneigh;
+    int cap;
+    const long long* offs; // per slot output offset (null: count only)
+    long long* cnt;        // per slot count output (count pass)
+    long long* out;
+};
+
+__global__ __launch_bounds__(kBlock) void k_nl_export_brick(BrickExportArgs a) {
+    __shared__ int h_start[kHaloCellCap + 1];
+    __shared__ int h_off[kHaloCellCap + 1];
+    __shared__ int h_shift[kHaloCellCap];
+    __shared__ int scan_tmp[kBlock];
+    const int b = blockIdx.x;
+    HaloTable ht;
+    build_halo_table(a.g, a.start, b, ht, h_start, h_off, h_shift, scan_tmp);
+    const int a0 = a.start[a.g.brick_cell_base[b]];
+    const int a1 = a.start[a.g.brick_cell_base[b + 1]];
+    const int nat = a1 - a0;
+    for (int li = threadIdx.x; li < nat; li += blockDim.x) {
+        const int i = a0 + li;
+        const int ti = a.tag[i];
+        const int nn = a.numneigh[i];
+        long long w = a.offs ? a.offs[i] : 0;
+        long long c = 0;
+        for (int k = 0; k < nn; ++k) {
+            const int q = a.nbr16[(int64_t)a0 * a.cap + (int64_t)k * nat + li];
+            const int h = halo_cell_of(h_off, ht.ncell, q);
+            const int j = h_start[h] + (q - h_off[h]);
+            const int tj = a.tag[j];
+            if (ti < tj) {
+                if (a.offs) {
+                    a.out[2 * w] = ti;
+                    a.out[2 * w + 1] = tj;
+                    ++w;
+                }
+                ++c;
+            }
+        }
+        if (!a.offs)
+            a.cnt[i] = c;
+    }
+}
+
 // LAMMPS Neighbor::decide for one step (neighbor_list.h:128-141) — the
 // displacement half (needs_rebuild, cell_list.cpp:372-391) is folded into the
 // initial-integrate kernel, which raises disp_flag.
@@ -406,6 +589,7 @@ __global__ void k_nl_stats_reset(DevCtl* ctl) {
         ctl->nbr_total = 0;
         ctl->max_nbr = 0;
         ctl->overflow = 0;
+        ctl->halo_overflow = 0;
     }
 }
 
@@ -448,6 +632,83 @@ __global__ void k_nl_export_count(int64_t n, const int* __restrict__ tag,
 
 namespace atvh {
 
+// Brick size: the largest candidate whose expected halo fills at most 70% of
+// the LDS rows (uniform density), never larger than the grid.
+static void choose_bricks(atvh_system* s, int first) {
+    const double vol = s->box[0] * s->box[1] * s->box[2];
+    const double per_cell = double(s->n) / vol * s->cs[0] * s->cs[1] * s->cs[2];
+    static const int cand[][3] = {{4, 4, 5}, {4, 4, 4}, {3, 4, 4}, {3, 3, 4}, {3, 3, 3},
+                                  {2, 3, 3}, {2, 2, 3}, {2, 2, 2}, {1, 2, 2}, {1, 1, 2}, {1, 1, 1}};
+    int pick = 10;
+    for (int k = first < 10 ? first : 10; k < 11; ++k) {
+        int d[3], hc = 1, bc = 1;
+        for (int a = 0; a < 3; ++a) {
+            d[a] = std::min(cand[k][a], s->nc[a]);
+            hc *= d[a] + 2 * s->reach[a];
+            bc *= d[a];
+        }
+        if (hc <= kHaloCellCap && per_cell * hc <= 0.7 * kHaloCap && per_cell * bc <= 1.25 * kBlock) {
+            pick = k;
+            break;
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        s->bdim[a] = std::min(cand[pick][a], s->nc[a]);
+        s->nb[a] = (s->nc[a] + s->bdim[a] - 1) / s->bdim[a];
+    }
+    s->nbricks = s->nb[0] * s->nb[1] * s->nb[2];
+    s->brick_pick = pick;
+}
+
+// Rank of every reference cell in the brick-major layout.
+static void build_rank_tables(atvh_system* s, std::vector<int>& rank, std::vector<int>& base) {
+    rank.assign(s->ncells, 0);
+    base.assign(s->nbricks + 1, 0);
+    int next = 0;
+    for (int bx = 0; bx < s->nb[0]; ++bx)
+        for (int by = 0; by < s->nb[1]; ++by)
+            for (int bz = 0; bz < s->nb[2]; ++bz) {
+                const int b = (bx * s->nb[1] + by) * s->nb[2] + bz;
+                base[b] = next;
+                const int o[3] = {bx * s->bdim[0], by * s->bdim[1], bz * s->bdim[2]};
+                int d[3];
+                for (int a = 0; a < 3; ++a)
+                    d[a] = std::min(s->bdim[a], s->nc[a] - o[a]);
+                for (int lx = 0; lx < d[0]; ++lx)
+                    for (int ly = 0; ly < d[1]; ++ly)
+                        for (int lz = 0; lz < d[2]; ++lz) {
+                            const int c = ((o[0] + lx) * s->nc[1] + (o[1] + ly)) * s->nc[2] + (o[2] + lz);
+                            rank[c] = next++;
+                        }
+            }
+    base[s->nbricks] = next;
+}
+
+BrickGeom brick_geom(const atvh_system* s) {
+    BrickGeom g;
+    for (int a = 0; a < 3; ++a) {
+        g.nc[a] = s->nc[a];
+        g.R[a] = s->reach[a];
+        g.bdim[a] = s->bdim[a];
+        g.nb[a] = s->nb[a];
+        g.L[a] = s->box[a];
+        g.cs[a] = s->cs[a];
+    }
+    g.cell_rank = s->cell_rank;
+    g.brick_cell_base = s->brick_cell_base;
+    return g;
+}
+
+void nl_realloc_capacity(atvh_system* s, int cap) {
+    dev_free(s->nbr);
+    dev_free(s->nbr16);
+    s->cap = (cap + 7) / 8 * 8;
+    if (s->brick_mode)
+        dev_alloc(&s->nbr16, size_t(s->cap) * s->n);
+    else
+        dev_alloc(&s->nbr, size_t(s->cap) * s->npad);
+}
+
 // CellList::compute_grid + build_stencil on the host (box-change only).
 void nl_setup_grid(atvh_system* s) {
     const double cutoff_skin = s->cutoff_skin;
@@ -489,8 +750,33 @@ void nl_setup_grid(atvh_system* s) {
             }
     s->nstencil = static_cast<int>(st.size() / 4);
     nl_free(s);
+    // brick layout when the half-stencil arithmetic applies, else one brick
+    // per cell and the global-slot fallback list
+    s->brick_mode = s->use_half;
+    if (s->brick_mode) {
+        choose_bricks(s, s->brick_pick_min);
+    } else {
+        for (int a = 0; a < 3; ++a) {
+            s->bdim[a] = 1;
+            s->nb[a] = s->nc[a];
+        }
+        s->nbricks = s->ncells;
+    }
+    std::vector<int> rank, base;
+    build_rank_tables(s, rank, base);
+    // per-block reduction partials: one row per pair workgroup (bricks) or grid block
+    const int need = std::max(kMaxGrid, s->nbricks);
+    if (need > s->npartials) {
+        dev_free(s->partials);
+        s->npartials = need;
+        dev_alloc(&s->partials, size_t(2 * s->npartials) * kPartialStride); // pair | kinetic
+    }
     dev_alloc(&s->d_stencil, st.size());
     ATVH_HIP(hipMemcpy(s->d_stencil, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
+    dev_alloc(&s->cell_rank, s->ncells);
+    ATVH_HIP(hipMemcpy(s->cell_rank, rank.data(), rank.size() * sizeof(int), hipMemcpyHostToDevice));
+    dev_alloc(&s->brick_cell_base, s->nbricks + 1);
+    ATVH_HIP(hipMemcpy(s->brick_cell_base, base.data(), base.size() * sizeof(int), hipMemcpyHostToDevice));
     dev_alloc(&s->cell_of, s->n);
     dev_alloc(&s->acell_of, s->n);
     dev_alloc(&s->perm, s->n);
@@ -502,6 +788,7 @@ void nl_setup_grid(atvh_system* s) {
     ATVH_HIP(hipMemset(s->cell_count, 0, sizeof(int) * s->ncells));
     ATVH_HIP(hipMemset(s->cell_fill, 0, sizeof(int) * s->ncells));
     dev_alloc(&s->numneigh, s->n);
+    dev_alloc(&s->self16, s->n);
     // capacity: 1.5x the mean full-list count at this density, plus margin
     const double vol = s->box[0] * s->box[1] * s->box[2];
     const double mean = double(s->n) / vol * 4.18879020478639 * cutoff_skin * cutoff_skin * cutoff_skin;
@@ -509,12 +796,13 @@ void nl_setup_grid(atvh_system* s) {
     cap = (cap + 7) / 8 * 8;
     if (cap > s->n)
         cap = static_cast<int>(std::max<int64_t>(8, (s->n + 7) / 8 * 8));
-    s->cap = cap;
-    dev_alloc(&s->nbr, size_t(cap) * s->npad);
+    nl_realloc_capacity(s, cap);
 }
 
 void nl_free(atvh_system* s) {
     dev_free(s->d_stencil);
+    dev_free(s->cell_rank);
+    dev_free(s->brick_cell_base);
     dev_free(s->cell_of);
     dev_free(s->acell_of);
     dev_free(s->perm);
@@ -523,13 +811,9 @@ void nl_free(atvh_system* s) {
     dev_free(s->cell_start);
     dev_free(s->scan_sums);
     dev_free(s->numneigh);
+    dev_free(s->self16);
     dev_free(s->nbr);
-}
-
-static void nl_realloc_capacity(atvh_system* s, int cap) {
-    dev_free(s->nbr);
-    s->cap = (cap + 7) / 8 * 8;
-    dev_alloc(&s->nbr, size_t(s->cap) * s->npad);
+    dev_free(s->nbr16);
 }
 
 void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
@@ -539,15 +823,15 @@ void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
     const int g = grid_capped(n);
     k_nl_bin<<<g, kBlock, 0, st>>>(pred, n, s->x, s->y, s->z, s->box[0], s->box[1], s->box[2],
                                    s->cs[0], s->cs[1], s->cs[2], s->nc[0], s->nc[1], s->nc[2],
-                                   s->cell_of, s->cell_count);
+                                   s->cell_rank, s->cell_of, s->cell_count);
     k_nl_scan_tiles<<<s->nscan_blocks, kBlock, 0, st>>>(pred, s->ncells, s->cell_count,
                                                         s->cell_start, s->scan_sums);
     k_nl_scan_sums<<<1, kBlock, 0, st>>>(pred, s->nscan_blocks, s->scan_sums, s->cell_start,
                                          s->ncells);
     k_nl_scan_add<<<s->nscan_blocks, kBlock, 0, st>>>(pred, s->ncells, s->scan_sums,
                                                       s->cell_start);
-    k_nl_scatter<<<g, kBlock, 0, st>>>(pred, n, s->cell_of, s->cell_start, s->cell_fill,
-                                       s->cell_count, s->perm);
+    k_nl_scatter<<<g, kBlock, 0, st>>>(pred, n, s->cell_of, s->cell_rank, s->cell_start,
+                                       s->cell_fill, s->cell_count, s->perm);
     k_nl_cell_order<<<grid_capped(s->ncells), kBlock, 0, st>>>(pred, s->ncells, s->cell_start,
                                                                s->tag, s->perm, s->cell_fill);
     PermArgs pa{s->perm, s->x,  s->y,  s->z,  s->vx,  s->vy,  s->vz,  s->fx,  s->fy,  s->fz,
@@ -560,20 +844,43 @@ void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
                 s->fx, s->fy, s->fz, s->q, s->xs, s->ys, s->zs, s->type, s->tag, s->cell_of,
                 permute_forces ? 1 : 0};
     k_nl_copyback<<<g, kBlock, 0, st>>>(pred, n, ca);
-    BuildArgs ba{s->x,         s->y,        s->z,        s->cell_of, s->cell_start,
-                 reinterpret_cast<const int4*>(s->d_stencil), s->nstencil, s->nc[0], s->nc[1],
-                 s->nc[2],     s->box[0],   s->box[1],   s->box[2],  s->cutoff_skin * s->cutoff_skin,
-                 s->use_half ? 1 : 0, s->nbr, s->numneigh, s->cap, s->npad, s->ctl};
-    if (s->use_half)
-        k_nl_build<true><<<grid_for(n), kBlock, 0, st>>>(pred, n, ba);
-    else
+    if (s->brick_mode) {
+        BrickBuildArgs ba;
+        ba.g = brick_geom(s);
+        ba.x = s->x;
+        ba.y = s->y;
+        ba.z = s->z;
+        ba.cell_of = s->cell_of;
+        ba.start = s->cell_start;
+        ba.stencil = reinterpret_cast<const int4*>(s->d_stencil);
+        ba.nstencil = s->nstencil;
+        ba.rc_sq = s->cutoff_skin * s->cutoff_skin;
+        ba.nbr16 = s->nbr16;
+        ba.self16 = s->self16;
+        ba.numneigh = s->numneigh;
+        ba.cap = s->cap;
+        ba.ctl = s->ctl;
+        k_nl_build_brick<<<s->nbricks, kBlock, 0, st>>>(pred, ba);
+    } else {
+        BuildArgs ba{s->x, s->y, s->z, s->cell_of, s->cell_rank, s->cell_start,
+                     reinterpret_cast<const int4*>(s->d_stencil), s->nstencil, s->nc[0], s->nc[1],
+                     s->nc[2], s->box[0], s->box[1], s->box[2], s->cutoff_skin * s->cutoff_skin,
+                     0, s->nbr, s->numneigh, s->cap, s->npad, s->ctl};
         k_nl_build<false><<<grid_for(n), kBlock, 0, st>>>(pred, n, ba);
+    }
     ATVH_HIP(hipGetLastError());
 }
 
 void nl_enqueue_decide(atvh_system* s) {
     k_nl_decide<<<1, 64, 0, s->ctx->stream>>>(s->ctl, s->every, s->delay, s->check);
     ATVH_HIP(hipGetLastError());
+}
+
+void nl_check_ctl(const DevCtl& h) {
+    if (h.halo_overflow)
+        throw Error(ATVH_E_CAPACITY, "a brick halo exceeded the LDS capacity (density far above uniform)");
+    if (h.overflow)
+        throw Error(ATVH_E_CAPACITY, "neighbor capacity exceeded during a rebuild");
 }
 
 } // namespace atvh
@@ -627,14 +934,14 @@ int atvh_nlist_build(atvh_system* s) {
     if (!s->nl_configured)
         throw Error(ATVH_E_STATE, "atvh_nlist_build: neighbor list not configured");
     ATVH_HIP(hipSetDevice(c->device));
-    for (int attempt = 0; attempt < 3; ++attempt) {
+    for (int attempt = 0; attempt < 12; ++attempt) {
         k_nl_stats_reset<<<1, 64, 0, c->stream>>>(s->ctl);
         // the initial build also orders the frame's forces (first kick uses them)
         nl_enqueue_rebuild(s, &s->ctl->always, true);
         ATVH_HIP(hipStreamSynchronize(c->stream));
         DevCtl h{};
         ATVH_HIP(hipMemcpy(&h, s->ctl, sizeof(h), hipMemcpyDeviceToHost));
-        if (h.overflow == 0) {
+        if (h.overflow == 0 && h.halo_overflow == 0) {
             h.ago = 0;
             h.nrebuilds = 1;
             h.disp_flag = 0;
@@ -642,6 +949,14 @@ int atvh_nlist_build(atvh_system* s) {
             ATVH_HIP(hipMemcpy(s->ctl, &h, sizeof(h), hipMemcpyHostToDevice));
             s->nl_built = true;
             return ATVH_OK;
+        }
+        if (h.halo_overflow) {
+            // a denser-than-uniform region: fall back to the next smaller brick
+            if (s->brick_pick >= 10)
+                break;
+            s->brick_pick_min = s->brick_pick + 1;
+            nl_setup_grid(s);
+            continue;
         }
         // grow and rebuild from the (now sorted) positions
         nl_realloc_capacity(s, static_cast<int>(h.max_nbr * 1.25) + 16);
@@ -680,8 +995,7 @@ int atvh_nlist_stats(atvh_system* s, int64_t* nrebuilds, int64_t* npairs_half, i
         *max_neighbors = h.max_nbr;
     if (capacity)
         *capacity = s->cap;
-    if (h.overflow)
-        throw Error(ATVH_E_CAPACITY, "neighbor capacity exceeded during a rebuild");
+    nl_check_ctl(h);
     return ATVH_OK;
     API_CATCH(c)
 }
@@ -696,10 +1010,30 @@ int atvh_nlist_export_pairs(atvh_system* s, int64_t* pairs, int64_t cap, int64_t
     ATVH_HIP(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     const int64_t n = s->n;
+    {
+        DevCtl hc{};
+        ATVH_HIP(hipStreamSynchronize(st));
+        ATVH_HIP(hipMemcpy(&hc, s->ctl, sizeof(hc), hipMemcpyDeviceToHost));
+        nl_check_ctl(hc);
+    }
     long long* d_cnt = nullptr;
     dev_alloc(&d_cnt, n);
-    k_nl_export_count<<<grid_capped(n), kBlock, 0, st>>>(n, s->tag, s->numneigh, s->nbr, s->npad,
-                                                         d_cnt);
+    BrickExportArgs ea;
+    if (s->brick_mode) {
+        ea.g = brick_geom(s);
+        ea.start = s->cell_start;
+        ea.tag = s->tag;
+        ea.nbr16 = s->nbr16;
+        ea.numneigh = s->numneigh;
+        ea.cap = s->cap;
+        ea.offs = nullptr;
+        ea.cnt = d_cnt;
+        ea.out = nullptr;
+        k_nl_export_brick<<<s->nbricks, kBlock, 0, st>>>(ea);
+    } else {
+        k_nl_export_count<<<grid_capped(n), kBlock, 0, st>>>(n, s->tag, s->numneigh, s->nbr, s->npad,
+                                                             d_cnt);
+    }
     ATVH_HIP(hipGetLastError());
     std::vector<long long> h(n);
     ATVH_HIP(hipMemcpyAsync(h.data(), d_cnt, n * sizeof(long long), hipMemcpyDeviceToHost, st));
@@ -720,8 +1054,15 @@ int atvh_nlist_export_pairs(atvh_system* s, int64_t* pairs, int64_t cap, int64_t
     long long* d_out = nullptr;
     dev_alloc(&d_out, size_t(2 * std::max<long long>(tot, 1)));
     ATVH_HIP(hipMemcpyAsync(d_cnt, h.data(), n * sizeof(long long), hipMemcpyHostToDevice, st));
-    k_nl_export<<<grid_capped(n), kBlock, 0, st>>>(n, s->tag, s->numneigh, s->nbr, s->npad, d_cnt,
-                                                   d_out);
+    if (s->brick_mode) {
+        ea.offs = d_cnt;
+        ea.cnt = nullptr;
+        ea.out = d_out;
+        k_nl_export_brick<<<s->nbricks, kBlock, 0, st>>>(ea);
+    } else {
+        k_nl_export<<<grid_capped(n), kBlock, 0, st>>>(n, s->tag, s->numneigh, s->nbr, s->npad, d_cnt,
+                                                       d_out);
+    }
     ATVH_HIP(hipGetLastError());
     std::vector<long long> out(2 * tot);
     if (tot)

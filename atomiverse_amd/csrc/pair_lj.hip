@@ -8,17 +8,22 @@
 //   virial Voigt (dx fx, dy fy, dz fz, dx fy, dx fz, dy fz).
 // CUDA counterpart replaced: lj_force_tile (src/cuda/potentials/pair_lj_cut.cu:89-246).
 //
-// MI355X design: one thread per atom i, FULL list, so every force row has a
-// single writer (no atomics, deterministic, batch-invariant); energy and
-// virial are halved. Per-block partials of (E, virial) are summed in a fixed
-// order by the finalize kernel. Precision:
+// MI355X design: one workgroup per brick (brick.hpp) stages the brick's halo
+// in LDS as periodic images, then one thread per brick atom walks its FULL
+// neighbour list of 16-bit halo rows: every force row has a single writer (no
+// atomics, deterministic, batch-invariant), no per-pair minimum image and no
+// global gathers; energy and virial are halved. Per-block partials of
+// (E, virial) are summed in a fixed order by the finalize kernel. Precision:
 //   FP64  — everything in double, IEEE division (the plain drop-in path);
-//   MIXED — the displacement and its minimum image are formed in FP64 (an FP32
-//           difference of ~100 Å coordinates would cost ~1e-5 Å), converted
-//           to FP32 for r2, 1/r2 (v_rcp_f32), the LJ polynomial and the
-//           per-atom force sum; energy and virial are accumulated in FP64.
+//   MIXED — LDS holds FP32 coordinates relative to the brick centre (formed in
+//           FP64, |x| < ~7 Å so the rounding is <= 2.4e-7 Å); r2, 1/r2
+//           (v_rcp_f32), the LJ polynomial and the per-atom force sum are FP32,
+//           energy and virial are accumulated in FP64.
+// Grids with fewer than 2*reach+1 cells on an axis use the global-slot list
+// with per-pair minimum image (k_lj_fp64 / k_lj_mixed below).
 
 #include "atvh_internal.hpp"
+#include "brick.hpp"
 
 #include <cmath>
 
@@ -196,6 +201,248 @@ __global__ __launch_bounds__(kBlock) void k_lj_mixed(int64_t n, LJArgs a) {
     }
 }
 
+struct BrickLJArgs {
+    BrickGeom g;
+    const double *x, *y, *z;
+    const int* type;
+    const int* start;
+    const uint16_t* nbr16;
+    const uint16_t* self16;
+    const int* numneigh;
+    int cap;
+    double rc_sq;
+    double eps24, eps4, sig6, vshift;
+    const double4* tab;
+    int ntypes;
+    double *fx, *fy, *fz;
+    double* partials;
+    DevCtl* ctl;
+};
+
+// Image position of LDS row q of the halo: raw position + the halo cell's
+// periodic shift, re-centred on that cell in case the atom wrapped since the
+// list was built (it is within skin/2 of where it was binned).
+__device__ __forceinline__ void halo_image(const BrickLJArgs& a, const HaloTable& ht, const int* h_start,
+                                           const int* h_off, const int* h_shift, int q, int& gs,
+                                           double& xx, double& yy, double& zz) {
+    const int h = halo_cell_of(h_off, ht.ncell, q);
+    gs = h_start[h] + (q - h_off[h]);
+    const int code = h_shift[h];
+    const int hz = h % ht.dims[2];
+    const int hy = (h / ht.dims[2]) % ht.dims[1];
+    const int hx = h / (ht.dims[1] * ht.dims[2]);
+    const double c0 = (ht.o[0] + hx - a.g.R[0] + 0.5) * a.g.cs[0];
+    const double c1 = (ht.o[1] + hy - a.g.R[1] + 0.5) * a.g.cs[1];
+    const double c2 = (ht.o[2] + hz - a.g.R[2] + 0.5) * a.g.cs[2];
+    xx = a.x[gs] + shift_of(code, 0, a.g.L[0]);
+    yy = a.y[gs] + shift_of(code, 1, a.g.L[1]);
+    zz = a.z[gs] + shift_of(code, 2, a.g.L[2]);
+    xx -= a.g.L[0] * rint((xx - c0) / a.g.L[0]);
+    yy -= a.g.L[1] * rint((yy - c1) / a.g.L[1]);
+    zz -= a.g.L[2] * rint((zz - c2) / a.g.L[2]);
+}
+
+template <bool EF, bool VF, bool TYPED>
+__global__ __launch_bounds__(kBlock) void k_lj_brick_fp64(BrickLJArgs a) {
+    __shared__ int h_start[kHaloCellCap + 1];
+    __shared__ int h_off[kHaloCellCap + 1];
+    __shared__ int h_shift[kHaloCellCap];
+    __shared__ int scan_tmp[kBlock];
+    __shared__ double sx[kHaloCap], sy[kHaloCap], sz[kHaloCap];
+    __shared__ int stype[TYPED ? kHaloCap : 1];
+    const int b = blockIdx.x;
+    HaloTable ht;
+    build_halo_table(a.g, a.start, b, ht, h_start, h_off, h_shift, scan_tmp);
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (ht.natoms > kHaloCap) {
+        if (threadIdx.x == 0)
+            atomicAdd(&a.ctl->halo_overflow, 1);
+        if (EF || VF)
+            block_partials(acc, a.partials);
+        return;
+    }
+    for (int q = threadIdx.x; q < ht.natoms; q += blockDim.x) {
+        int gs;
+        double xx, yy, zz;
+        halo_image(a, ht, h_start, h_off, h_shift, q, gs, xx, yy, zz);
+        sx[q] = xx;
+        sy[q] = yy;
+        sz[q] = zz;
+        if (TYPED)
+            stype[q] = a.type[gs];
+    }
+    __syncthreads();
+    const int a0 = a.start[a.g.brick_cell_base[b]];
+    const int nat = a.start[a.g.brick_cell_base[b + 1]] - a0;
+    const double rc_sq = a.rc_sq;
+    for (int li = threadIdx.x; li < nat; li += blockDim.x) {
+        const int i = a0 + li;
+        const int qs = a.self16[i];
+        const double xi = sx[qs], yi = sy[qs], zi = sz[qs];
+        const int ti = TYPED ? stype[qs] : 0;
+        const int nn = a.numneigh[i];
+        const uint16_t* __restrict__ col = a.nbr16 + (int64_t)a0 * a.cap + li;
+        double fxi = 0.0, fyi = 0.0, fzi = 0.0;
+        for (int k = 0; k < nn; ++k) {
+            const int q = col[k * nat];
+            const double dx = xi - sx[q];
+            const double dy = yi - sy[q];
+            const double dz = zi - sz[q];
+            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+            if (r2 < rc_sq) {
+                double eps24 = a.eps24, eps4 = a.eps4, sig6 = a.sig6, shift = a.vshift;
+                if (TYPED) {
+                    const double4 t = a.tab[ti * a.ntypes + stype[q]];
+                    eps24 = t.x;
+                    eps4 = t.y;
+                    sig6 = t.z;
+                    shift = t.w;
+                }
+                const double r2inv = 1.0 / r2;
+                const double r6inv = r2inv * r2inv * r2inv * sig6;
+                const double fpair = eps24 * r2inv * fma(2.0 * r6inv, r6inv, -r6inv);
+                fxi = fma(fpair, dx, fxi);
+                fyi = fma(fpair, dy, fyi);
+                fzi = fma(fpair, dz, fzi);
+                if (EF)
+                    acc[0] += fma(eps4, r6inv * r6inv - r6inv, -shift);
+                if (VF) {
+                    const double fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
+                    acc[1] += dx * fxp;
+                    acc[2] += dy * fyp;
+                    acc[3] += dz * fzp;
+                    acc[4] += dx * fyp;
+                    acc[5] += dx * fzp;
+                    acc[6] += dy * fzp;
+                }
+            }
+        }
+        a.fx[i] = fxi;
+        a.fy[i] = fyi;
+        a.fz[i] = fzi;
+    }
+    if (EF || VF) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            acc[k] *= 0.5;
+        block_partials(acc, a.partials);
+    }
+}
+
+template <bool EF, bool VF, bool TYPED>
+__global__ __launch_bounds__(kBlock) void k_lj_brick_mixed(BrickLJArgs a) {
+    __shared__ int h_start[kHaloCellCap + 1];
+    __shared__ int h_off[kHaloCellCap + 1];
+    __shared__ int h_shift[kHaloCellCap];
+    __shared__ int scan_tmp[kBlock];
+    __shared__ float4 sp[kHaloCap]; // x, y, z relative to the brick centre; w = type bits
+    const int b = blockIdx.x;
+    HaloTable ht;
+    build_halo_table(a.g, a.start, b, ht, h_start, h_off, h_shift, scan_tmp);
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (ht.natoms > kHaloCap) {
+        if (threadIdx.x == 0)
+            atomicAdd(&a.ctl->halo_overflow, 1);
+        if (EF || VF)
+            block_partials(acc, a.partials);
+        return;
+    }
+    const double bc0 = (ht.o[0] + 0.5 * ht.d[0]) * a.g.cs[0];
+    const double bc1 = (ht.o[1] + 0.5 * ht.d[1]) * a.g.cs[1];
+    const double bc2 = (ht.o[2] + 0.5 * ht.d[2]) * a.g.cs[2];
+    for (int q = threadIdx.x; q < ht.natoms; q += blockDim.x) {
+        int gs;
+        double xx, yy, zz;
+        halo_image(a, ht, h_start, h_off, h_shift, q, gs, xx, yy, zz);
+        float4 v;
+        v.x = static_cast<float>(xx - bc0);
+        v.y = static_cast<float>(yy - bc1);
+        v.z = static_cast<float>(zz - bc2);
+        v.w = TYPED ? __int_as_float(a.type[gs]) : 0.0f;
+        sp[q] = v;
+    }
+    __syncthreads();
+    const int a0 = a.start[a.g.brick_cell_base[b]];
+    const int nat = a.start[a.g.brick_cell_base[b + 1]] - a0;
+    const float rc_sq = static_cast<float>(a.rc_sq);
+    const float eps24s = static_cast<float>(a.eps24), eps4s = static_cast<float>(a.eps4);
+    const float sig6s = static_cast<float>(a.sig6), shifts = static_cast<float>(a.vshift);
+    for (int li = threadIdx.x; li < nat; li += blockDim.x) {
+        const int i = a0 + li;
+        const int qs = a.self16[i];
+        const float4 pi = sp[qs];
+        const int ti = TYPED ? __float_as_int(pi.w) : 0;
+        const int nn = a.numneigh[i];
+        const uint16_t* __restrict__ col = a.nbr16 + (int64_t)a0 * a.cap + li;
+        float fxi = 0.f, fyi = 0.f, fzi = 0.f;
+        for (int k = 0; k < nn; ++k) {
+            const int q = col[k * nat];
+            const float4 pj = sp[q];
+            const float dx = pi.x - pj.x;
+            const float dy = pi.y - pj.y;
+            const float dz = pi.z - pj.z;
+            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            if (r2 < rc_sq) {
+                float eps24 = eps24s, eps4 = eps4s, sig6 = sig6s, shift = shifts;
+                if (TYPED) {
+                    const double4 t = a.tab[ti * a.ntypes + __float_as_int(pj.w)];
+                    eps24 = static_cast<float>(t.x);
+                    eps4 = static_cast<float>(t.y);
+                    sig6 = static_cast<float>(t.z);
+                    shift = static_cast<float>(t.w);
+                }
+                const float r2inv = __builtin_amdgcn_rcpf(r2);
+                const float r6inv = r2inv * r2inv * r2inv * sig6;
+                const float fpair = eps24 * r2inv * __builtin_fmaf(2.0f * r6inv, r6inv, -r6inv);
+                fxi = __builtin_fmaf(fpair, dx, fxi);
+                fyi = __builtin_fmaf(fpair, dy, fyi);
+                fzi = __builtin_fmaf(fpair, dz, fzi);
+                if (EF)
+                    acc[0] += static_cast<double>(__builtin_fmaf(eps4, r6inv * r6inv - r6inv, -shift));
+                if (VF) {
+                    const float fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
+                    acc[1] += static_cast<double>(dx * fxp);
+                    acc[2] += static_cast<double>(dy * fyp);
+                    acc[3] += static_cast<double>(dz * fzp);
+                    acc[4] += static_cast<double>(dx * fyp);
+                    acc[5] += static_cast<double>(dx * fzp);
+                    acc[6] += static_cast<double>(dy * fzp);
+                }
+            }
+        }
+        a.fx[i] = static_cast<double>(fxi);
+        a.fy[i] = static_cast<double>(fyi);
+        a.fz[i] = static_cast<double>(fzi);
+    }
+    if (EF || VF) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            acc[k] *= 0.5;
+        block_partials(acc, a.partials);
+    }
+}
+
+#define LJ_BRICK_DISPATCH(KERNEL, EF, VF, TY, GRID, ST, ARGS)                                  \
+    do {                                                                                      \
+        if (EF) {                                                                             \
+            if (VF) {                                                                         \
+                if (TY) KERNEL<true, true, true><<<GRID, kBlock, 0, ST>>>(ARGS);              \
+                else KERNEL<true, true, false><<<GRID, kBlock, 0, ST>>>(ARGS);                \
+            } else {                                                                          \
+                if (TY) KERNEL<true, false, true><<<GRID, kBlock, 0, ST>>>(ARGS);             \
+                else KERNEL<true, false, false><<<GRID, kBlock, 0, ST>>>(ARGS);               \
+            }                                                                                 \
+        } else {                                                                              \
+            if (VF) {                                                                         \
+                if (TY) KERNEL<false, true, true><<<GRID, kBlock, 0, ST>>>(ARGS);             \
+                else KERNEL<false, true, false><<<GRID, kBlock, 0, ST>>>(ARGS);               \
+            } else {                                                                          \
+                if (TY) KERNEL<false, false, true><<<GRID, kBlock, 0, ST>>>(ARGS);            \
+                else KERNEL<false, false, false><<<GRID, kBlock, 0, ST>>>(ARGS);              \
+            }                                                                                 \
+        }                                                                                     \
+    } while (0)
+
 #define LJ_DISPATCH(KERNEL, EF, VF, TY, GRID, ST, N, ARGS)                                   \
     do {                                                                                      \
         if (EF) {                                                                             \
@@ -221,7 +468,9 @@ __global__ __launch_bounds__(kBlock) void k_lj_mixed(int64_t n, LJArgs a) {
 
 namespace atvh {
 
-int pair_grid(const atvh_system* s) { return grid_capped(s->n); }
+BrickGeom brick_geom(const atvh_system* s);
+
+int pair_grid(const atvh_system* s) { return s->brick_mode ? s->nbricks : grid_capped(s->n); }
 
 void pair_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
     if (!s->pair_configured)
@@ -263,6 +512,37 @@ void pair_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
     const bool typed = s->lj_ntypes > 0;
     const int g = pair_grid(s);
     hipStream_t st = s->ctx->stream;
+    if (s->brick_mode) {
+        BrickLJArgs b;
+        b.g = brick_geom(s);
+        b.x = s->x;
+        b.y = s->y;
+        b.z = s->z;
+        b.type = s->type;
+        b.start = s->cell_start;
+        b.nbr16 = s->nbr16;
+        b.self16 = s->self16;
+        b.numneigh = s->numneigh;
+        b.cap = s->cap;
+        b.rc_sq = a.rc_sq;
+        b.eps24 = a.eps24;
+        b.eps4 = a.eps4;
+        b.sig6 = a.sig6;
+        b.vshift = a.vshift;
+        b.tab = a.tab;
+        b.ntypes = a.ntypes;
+        b.fx = s->fx;
+        b.fy = s->fy;
+        b.fz = s->fz;
+        b.partials = s->partials;
+        b.ctl = s->ctl;
+        if (precision == ATVH_PREC_MIXED)
+            LJ_BRICK_DISPATCH(k_lj_brick_mixed, eflag, vflag, typed, g, st, b);
+        else
+            LJ_BRICK_DISPATCH(k_lj_brick_fp64, eflag, vflag, typed, g, st, b);
+        ATVH_HIP(hipGetLastError());
+        return;
+    }
     if (precision == ATVH_PREC_MIXED)
         LJ_DISPATCH(k_lj_mixed, eflag, vflag, typed, g, st, s->n, a);
     else

@@ -104,7 +104,8 @@ def test_pair_set_and_forces_c0_snapshot(hip_ctx, orc):
     assert rms <= 1e-5, rms
     pe_m, vir_m, _ = s.energy()
     assert pe_m == pytest.approx(e, rel=1e-6)
-    np.testing.assert_allclose(vir_m, ovir, rtol=1e-5)
+    # virial tensor: 1e-5 relative in Frobenius norm (off-diagonals are ~1e-3 of the trace)
+    assert np.linalg.norm(vir_m - ovir) <= 1e-5 * np.linalg.norm(ovir)
 
 
 def test_two_atoms_at_sigma_give_minus_vshift(hip_ctx):
