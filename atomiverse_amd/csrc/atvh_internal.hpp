@@ -66,6 +66,7 @@ struct TimerSlot {
 
 struct atvh_ctx {
     int device = 0;
+    int live_systems = 0; // systems created on this context and not yet destroyed
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     std::string err;
@@ -133,8 +134,11 @@ struct atvh_system {
     int nbricks = 0;
     int* cell_rank = nullptr;       // reference cell index -> layout rank
     int* brick_cell_base = nullptr; // [nbricks+1]
-    uint16_t* nbr16 = nullptr;      // per brick, column-major: [a0*cap + k*nat + li]
+    uint16_t* nbr16 = nullptr;      // per brick, 8-wide groups: [a0*cap + (g*nat + li)*8 + u]
     uint16_t* self16 = nullptr;     // [n] own row in the brick's halo
+    int* halo_rows = nullptr;       // [nbricks*kHaloCap] global slot | image code << 26
+    int* halo_n = nullptr;          // [nbricks] rows in each halo
+    int2* brick_range = nullptr;    // [nbricks] (first slot, atom count)
 
     // pair style
     bool pair_configured = false;

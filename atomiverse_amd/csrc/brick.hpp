@@ -18,6 +18,9 @@ namespace atvh {
 
 constexpr int kHaloCap = 2048;      // atoms staged per halo (LDS rows)
 constexpr int kHaloCellCap = 768;   // halo cells per brick ((4+4)*(4+4)*(5+4)=576)
+constexpr int kRowSlotBits = 26;    // halo row = global slot | image code << 26
+constexpr int kRowSlotMask = (1 << kRowSlotBits) - 1;
+constexpr int kGroup = 8;           // neighbour indices per 16-byte load
 
 struct BrickGeom {
     int nc[3];    // cells per axis (reference grid)

@@ -72,23 +72,30 @@ static void resolve_timers(atvh_ctx* c) {
 
 namespace {
 
+// Frame-order gather for downloads; positions are wrapped into [0, L) here
+// (wrap_orthorhombic, pbc.h:37-51), velocities get the pending NH scale.
 __global__ void k_gather_out(int64_t n, const int* __restrict__ tag, const double* __restrict__ a,
                              const double* __restrict__ b, const double* __restrict__ c,
                              double* __restrict__ oa, double* __restrict__ ob,
-                             double* __restrict__ oc, const DevCtl* ctl, int apply_scale) {
-    const double s = apply_scale ? ctl->nh_scale : 1.0;
+                             double* __restrict__ oc, const DevCtl* ctl, int mode, double la,
+                             double lb, double lc) {
+    const double s = mode == 1 ? ctl->nh_scale : 1.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int t = tag[i];
-        if (apply_scale) {
-            oa[t] = a[i] * s;
-            ob[t] = b[i] * s;
-            oc[t] = c[i] * s;
-        } else {
-            oa[t] = a[i];
-            ob[t] = b[i];
-            oc[t] = c[i];
+        double va = a[i], vb = b[i], vc = c[i];
+        if (mode == 1) {
+            va *= s;
+            vb *= s;
+            vc *= s;
+        } else if (mode == 2) {
+            va -= la * floor(va / la);
+            vb -= lb * floor(vb / lb);
+            vc -= lc * floor(vc / lc);
         }
+        oa[t] = va;
+        ob[t] = vb;
+        oc[t] = vc;
     }
 }
 
@@ -139,6 +146,10 @@ int atvh_ctx_create(int device, atvh_ctx** out) {
 int atvh_ctx_destroy(atvh_ctx* c) {
     if (!c)
         return ATVH_OK;
+    if (c->live_systems > 0) {
+        c->err = "atvh_ctx_destroy: systems created on this context are still alive";
+        return ATVH_E_STATE;
+    }
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (auto& kv : c->timers)
@@ -246,6 +257,7 @@ int atvh_system_create(atvh_ctx* c, int64_t n, const double box[3], atvh_system*
     k_iota<<<grid_capped(n), kBlock, 0, c->stream>>>(n, s->tag);
     ATVH_HIP(hipGetLastError());
     ATVH_HIP(hipStreamSynchronize(c->stream));
+    c->live_systems += 1;
     *out = s;
     return ATVH_OK;
     }
@@ -271,6 +283,8 @@ int atvh_system_destroy(atvh_system* s) {
         (void)hipStreamSynchronize(s->ctx->stream);
     }
     sys_free_all(s);
+    if (s->ctx)
+        s->ctx->live_systems -= 1;
     delete s;
     return ATVH_OK;
 }
@@ -363,11 +377,11 @@ int atvh_frame_download(atvh_system* s, double* x, double* y, double* z, double*
     const int64_t n = s->n;
     hipStream_t st = c->stream;
     auto fetch3 = [&](const double* a, const double* b, const double* cc, double* ha, double* hb,
-                      double* hc, int scale) {
+                      double* hc, int mode) {
         if (!ha && !hb && !hc)
             return;
         k_gather_out<<<grid_capped(n), kBlock, 0, st>>>(n, s->tag, a, b, cc, s->ax, s->ay, s->az,
-                                                        s->ctl, scale);
+                                                        s->ctl, mode, s->box[0], s->box[1], s->box[2]);
         ATVH_HIP(hipGetLastError());
         if (ha)
             ATVH_HIP(hipMemcpyAsync(ha, s->ax, n * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -377,7 +391,7 @@ int atvh_frame_download(atvh_system* s, double* x, double* y, double* z, double*
             ATVH_HIP(hipMemcpyAsync(hc, s->az, n * sizeof(double), hipMemcpyDeviceToHost, st));
         ATVH_HIP(hipStreamSynchronize(st));
     };
-    fetch3(s->x, s->y, s->z, x, y, z, 0);
+    fetch3(s->x, s->y, s->z, x, y, z, 2);
     fetch3(s->vx, s->vy, s->vz, vx, vy, vz, 1);
     fetch3(s->fx, s->fy, s->fz, fx, fy, fz, 0);
     return ATVH_OK;

@@ -4,6 +4,7 @@
 // Reference order per step (Runtime::step_verlet_, src/cpu/driver/runtime.cpp:452-502):
 //   initial_integrate: VelocityVerlet (src/cpu/dynamics/velocity_verlet.cpp:41-55)
 //                      v += dtf/m f; x += dt v; wrap_orthorhombic (pbc.h:37-51)
+//                      (here the wrap happens at the next rebuild, see k_initial)
 //                      then NoseHoover (nose_hoover.cpp:104-170):
 //                      KE = 0.5 mvv2e sum m v^2; xi += dt/2 (2KE - Ndof kB T)/Q;
 //                      v *= exp(-dt/2 xi)
@@ -21,12 +22,15 @@
 
 #include "atvh_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
 using namespace atvh;
 
 namespace {
+
+constexpr int kIntGrid = 1024; // integrator grid cap: few kinetic partials to reduce
 
 struct IntArgs {
     double *x, *y, *z, *vx, *vy, *vz;
@@ -82,12 +86,12 @@ __global__ __launch_bounds__(kBlock) void k_initial(int64_t n, IntArgs a) {
         vx += dtfm * a.fx[i];
         vy += dtfm * a.fy[i];
         vz += dtfm * a.fz[i];
-        double x = a.x[i] + a.dt * vx;
-        double y = a.y[i] + a.dt * vy;
-        double z = a.z[i] + a.dt * vz;
-        x -= a.lx * floor(x / a.lx);
-        y -= a.ly * floor(y / a.ly);
-        z -= a.lz * floor(z / a.lz);
+        // drift; the periodic wrap of wrap_orthorhombic is deferred to the next
+        // neighbor rebuild (and to downloads), which keeps the image of every
+        // listed pair fixed between rebuilds
+        const double x = a.x[i] + a.dt * vx;
+        const double y = a.y[i] + a.dt * vy;
+        const double z = a.z[i] + a.dt * vz;
         a.x[i] = x;
         a.y[i] = y;
         a.z[i] = z;
@@ -250,7 +254,7 @@ PostArgs make_post_args(atvh_system* s, bool final_phase, bool decide) {
     PostArgs p;
     p.ctl = s->ctl;
     p.ke_partials = s->partials + size_t(s->npartials) * kPartialStride;
-    p.nb = grid_capped(s->n);
+    p.nb = std::min(grid_for(s->n), kIntGrid);
     p.thermostat = s->thermostat;
     p.dt = s->dt;
     // NoseHoover::build / launch: Q = Ndof kB T tau^2
@@ -275,17 +279,19 @@ void check_integ(atvh_system* s) {
 
 namespace atvh {
 
+int integ_grid(const atvh_system* s) { return std::min(grid_for(s->n), kIntGrid); }
+
 void integ_enqueue_initial(atvh_system* s) {
     ScopedTimer tm(s->ctx, "integrate");
     IntArgs a = make_int_args(s);
-    k_initial<<<grid_capped(s->n), kBlock, 0, s->ctx->stream>>>(s->n, a);
+    k_initial<<<integ_grid(s), kBlock, 0, s->ctx->stream>>>(s->n, a);
     ATVH_HIP(hipGetLastError());
 }
 
 void integ_enqueue_final(atvh_system* s) {
     ScopedTimer tm(s->ctx, "integrate");
     IntArgs a = make_int_args(s);
-    k_final<<<grid_capped(s->n), kBlock, 0, s->ctx->stream>>>(s->n, a);
+    k_final<<<integ_grid(s), kBlock, 0, s->ctx->stream>>>(s->n, a);
     ATVH_HIP(hipGetLastError());
 }
 
@@ -400,7 +406,7 @@ int atvh_md_run(atvh_system* s, int nsteps, int precision, int energy_every) {
     PostArgs p_init = make_post_args(s, false, true);
     PostArgs p_fin = make_post_args(s, true, false);
     p_fin.record_ke = rec_ke;
-    const int g = grid_capped(s->n);
+    const int g = std::min(grid_for(s->n), kIntGrid);
     for (int step = 0; step < nsteps; ++step) {
         {
             ScopedTimer tm(c, "integrate");

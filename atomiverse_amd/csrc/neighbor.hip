@@ -40,8 +40,8 @@ constexpr int kScanItems = 4;
 constexpr int kScanTile = kBlock * kScanItems; // elements per scan block
 
 // CellList::bin_atoms: cell of every slot (current order) + histogram.
-__global__ void k_nl_bin(const int* __restrict__ pred, int64_t n, const double* __restrict__ x,
-                         const double* __restrict__ y, const double* __restrict__ z, double lx,
+__global__ void k_nl_bin(const int* __restrict__ pred, int64_t n, double* __restrict__ x,
+                         double* __restrict__ y, double* __restrict__ z, double lx,
                          double ly, double lz, double csx, double csy, double csz, int ncx, int ncy,
                          int ncz, const int* __restrict__ cell_rank, int* __restrict__ cell_of,
                          int* __restrict__ count) {
@@ -49,7 +49,17 @@ __global__ void k_nl_bin(const int* __restrict__ pred, int64_t n, const double* 
         return;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
+        // wrap_orthorhombic (pbc.h:37-51) is applied here, at the rebuild,
+        // rather than every step: between rebuilds positions stay unwrapped so
+        // the periodic image of every listed pair is the one fixed at the build.
         double xi = x[i], yi = y[i], zi = z[i];
+        xi -= lx * floor(xi / lx);
+        yi -= ly * floor(yi / ly);
+        zi -= lz * floor(zi / lz);
+        x[i] = xi;
+        y[i] = yi;
+        z[i] = zi;
+        // bin_atoms wraps its own copy once more (a no-op except at x == L)
         xi -= lx * floor(xi / lx);
         yi -= ly * floor(yi / ly);
         zi -= lz * floor(zi / lz);
@@ -390,6 +400,7 @@ __global__ __launch_bounds__(kBlock) void k_nl_build(const int* __restrict__ pre
 
 struct BrickBuildArgs {
     BrickGeom g;
+    int nbricks;
     const double *x, *y, *z;
     const int* cell_of;
     const int* start;
@@ -399,14 +410,19 @@ struct BrickBuildArgs {
     uint16_t* nbr16;
     uint16_t* self16;
     int* numneigh;
+    int* halo_rows;
+    int* halo_n;
+    int2* brick_range;
     int cap;
     DevCtl* ctl;
 };
 
-// Brick-mode list build: one workgroup per brick. The halo's RAW positions are
-// staged in LDS (the predicate uses the reference's shift form on raw
-// coordinates, not images); each brick atom scans its stencil cells inside the
-// halo and stores accepted neighbours as 16-bit halo rows.
+// Brick-mode list build: one workgroup per brick (grid-stride over bricks so a
+// skipped rebuild costs one short launch). It records the brick's halo rows
+// (global slot + image code) for the pair kernels, stages the halo's RAW
+// positions in LDS (the predicate uses the reference's shift form on raw
+// coordinates, not images), and each brick atom scans its stencil cells inside
+// the halo, storing accepted neighbours as 16-bit halo rows in 8-wide groups.
 __global__ __launch_bounds__(kBlock) void k_nl_build_brick(const int* __restrict__ pred,
                                                            BrickBuildArgs a) {
     if (!*pred)
@@ -418,75 +434,83 @@ __global__ __launch_bounds__(kBlock) void k_nl_build_brick(const int* __restrict
     __shared__ double sx[kHaloCap], sy[kHaloCap], sz[kHaloCap];
     __shared__ long long sh_tot[kBlock / 64];
     __shared__ int sh_max[kBlock / 64], sh_over[kBlock / 64];
-    const int b = blockIdx.x;
-    HaloTable ht;
-    build_halo_table(a.g, a.start, b, ht, h_start, h_off, h_shift, scan_tmp);
-    if (ht.natoms > kHaloCap) {
-        if (threadIdx.x == 0)
-            atomicAdd(&a.ctl->halo_overflow, 1);
-        return;
-    }
-    for (int q = threadIdx.x; q < ht.natoms; q += blockDim.x) {
-        const int h = halo_cell_of(h_off, ht.ncell, q);
-        const int gs = h_start[h] + (q - h_off[h]);
-        sx[q] = a.x[gs];
-        sy[q] = a.y[gs];
-        sz[q] = a.z[gs];
-    }
-    __syncthreads();
-    const int a0 = a.start[a.g.brick_cell_base[b]];
-    const int a1 = a.start[a.g.brick_cell_base[b + 1]];
-    const int nat = a1 - a0;
-    const int nz = a.g.nc[2], ny = a.g.nc[1];
-    const int hy = ht.dims[1], hz = ht.dims[2];
     long long my_cnt_sum = 0;
     int my_max = 0, my_over = 0;
-    for (int li = threadIdx.x; li < nat; li += blockDim.x) {
-        const int i = a0 + li;
-        const int ci = a.cell_of[i];
-        const int lz = ci % nz - ht.o[2];
-        const int ly = (ci / nz) % ny - ht.o[1];
-        const int lx = ci / (ny * nz) - ht.o[0];
-        const int hs = ((lx + a.g.R[0]) * hy + (ly + a.g.R[1])) * hz + (lz + a.g.R[2]);
-        const int qs = h_off[hs] + (i - h_start[hs]);
-        const double xi = sx[qs], yi = sy[qs], zi = sz[qs];
-        uint16_t* __restrict__ out = a.nbr16 + (int64_t)a0 * a.cap + li;
-        int cnt = 0;
-        for (int s = 0; s < a.nstencil; ++s) {
-            const int4 o = a.stencil[s];
-            const int h = ((lx + a.g.R[0] + o.x) * hy + (ly + a.g.R[1] + o.y)) * hz + (lz + a.g.R[2] + o.z);
-            const int code = h_shift[h];
-            const double shx = shift_of(code, 0, a.g.L[0]);
-            const double shy = shift_of(code, 1, a.g.L[1]);
-            const double shz = shift_of(code, 2, a.g.L[2]);
-            const int qb = h_off[h], qe = h_off[h + 1];
-            for (int q = qb; q < qe; ++q) {
-                if (q == qs)
-                    continue;
-                double dx, dy, dz;
-                if (o.w == 2) {
-                    dx = sx[q] - (xi - shx);
-                    dy = sy[q] - (yi - shy);
-                    dz = sz[q] - (zi - shz);
-                } else {
-                    dx = xi - (sx[q] + shx);
-                    dy = yi - (sy[q] + shy);
-                    dz = zi - (sz[q] + shz);
-                }
-                const double r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 < a.rc_sq) {
-                    if (cnt < a.cap)
-                        out[(int64_t)cnt * nat] = static_cast<uint16_t>(q);
-                    ++cnt;
+    const int nz = a.g.nc[2], ny = a.g.nc[1];
+    for (int b = blockIdx.x; b < a.nbricks; b += gridDim.x) {
+        HaloTable ht;
+        build_halo_table(a.g, a.start, b, ht, h_start, h_off, h_shift, scan_tmp);
+        const int a0 = a.start[a.g.brick_cell_base[b]];
+        const int nat = a.start[a.g.brick_cell_base[b + 1]] - a0;
+        if (threadIdx.x == 0) {
+            a.halo_n[b] = ht.natoms;
+            a.brick_range[b] = make_int2(a0, nat);
+            if (ht.natoms > kHaloCap)
+                atomicAdd(&a.ctl->halo_overflow, 1);
+        }
+        if (ht.natoms > kHaloCap) {
+            __syncthreads();
+            continue;
+        }
+        int* __restrict__ rows = a.halo_rows + (int64_t)b * kHaloCap;
+        for (int q = threadIdx.x; q < ht.natoms; q += blockDim.x) {
+            const int h = halo_cell_of(h_off, ht.ncell, q);
+            const int gs = h_start[h] + (q - h_off[h]);
+            rows[q] = gs | (h_shift[h] << kRowSlotBits);
+            sx[q] = a.x[gs];
+            sy[q] = a.y[gs];
+            sz[q] = a.z[gs];
+        }
+        __syncthreads();
+        const int hy = ht.dims[1], hz = ht.dims[2];
+        uint16_t* __restrict__ lst = a.nbr16 + (int64_t)a0 * a.cap;
+        for (int li = threadIdx.x; li < nat; li += blockDim.x) {
+            const int i = a0 + li;
+            const int ci = a.cell_of[i];
+            const int lz = ci % nz - ht.o[2];
+            const int ly = (ci / nz) % ny - ht.o[1];
+            const int lx = ci / (ny * nz) - ht.o[0];
+            const int hs = ((lx + a.g.R[0]) * hy + (ly + a.g.R[1])) * hz + (lz + a.g.R[2]);
+            const int qs = h_off[hs] + (i - h_start[hs]);
+            const double xi = sx[qs], yi = sy[qs], zi = sz[qs];
+            int cnt = 0;
+            for (int st = 0; st < a.nstencil; ++st) {
+                const int4 o = a.stencil[st];
+                const int h = ((lx + a.g.R[0] + o.x) * hy + (ly + a.g.R[1] + o.y)) * hz + (lz + a.g.R[2] + o.z);
+                const int code = h_shift[h];
+                const double shx = shift_of(code, 0, a.g.L[0]);
+                const double shy = shift_of(code, 1, a.g.L[1]);
+                const double shz = shift_of(code, 2, a.g.L[2]);
+                const int qb = h_off[h], qe = h_off[h + 1];
+                for (int q = qb; q < qe; ++q) {
+                    if (q == qs)
+                        continue;
+                    double dx, dy, dz;
+                    if (o.w == 2) {
+                        dx = sx[q] - (xi - shx);
+                        dy = sy[q] - (yi - shy);
+                        dz = sz[q] - (zi - shz);
+                    } else {
+                        dx = xi - (sx[q] + shx);
+                        dy = yi - (sy[q] + shy);
+                        dz = zi - (sz[q] + shz);
+                    }
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    if (r2 < a.rc_sq) {
+                        if (cnt < a.cap)
+                            lst[((int64_t)(cnt >> 3) * nat + li) * kGroup + (cnt & 7)] = static_cast<uint16_t>(q);
+                        ++cnt;
+                    }
                 }
             }
+            a.numneigh[i] = cnt < a.cap ? cnt : a.cap;
+            a.self16[i] = static_cast<uint16_t>(qs);
+            my_cnt_sum += cnt;
+            my_max = max(my_max, cnt);
+            if (cnt > a.cap)
+                ++my_over;
         }
-        a.numneigh[i] = cnt < a.cap ? cnt : a.cap;
-        a.self16[i] = static_cast<uint16_t>(qs);
-        my_cnt_sum += cnt;
-        my_max = max(my_max, cnt);
-        if (cnt > a.cap)
-            ++my_over;
+        __syncthreads(); // LDS reused by the next brick
     }
     for (int off = 32; off > 0; off >>= 1) {
         my_cnt_sum += __shfl_down(my_cnt_sum, off, 64);
@@ -517,8 +541,9 @@ __global__ __launch_bounds__(kBlock) void k_nl_build_brick(const int* __restrict
 
 // Brick-mode export: decode halo rows back to global slots, emit tag pairs a<b.
 struct BrickExportArgs {
-    BrickGeom g;
-    const int* start;
+    int nbricks;
+    const int* halo_rows;
+    const int2* brick_range;
     const int* tag;
     const uint16_t* nbr16;
     const int* numneigh;
@@ -529,16 +554,11 @@ struct BrickExportArgs {
 };
 
 __global__ __launch_bounds__(kBlock) void k_nl_export_brick(BrickExportArgs a) {
-    __shared__ int h_start[kHaloCellCap + 1];
-    __shared__ int h_off[kHaloCellCap + 1];
-    __shared__ int h_shift[kHaloCellCap];
-    __shared__ int scan_tmp[kBlock];
     const int b = blockIdx.x;
-    HaloTable ht;
-    build_halo_table(a.g, a.start, b, ht, h_start, h_off, h_shift, scan_tmp);
-    const int a0 = a.start[a.g.brick_cell_base[b]];
-    const int a1 = a.start[a.g.brick_cell_base[b + 1]];
-    const int nat = a1 - a0;
+    const int2 br = a.brick_range[b];
+    const int a0 = br.x, nat = br.y;
+    const int* rows = a.halo_rows + (int64_t)b * kHaloCap;
+    const uint16_t* lst = a.nbr16 + (int64_t)a0 * a.cap;
     for (int li = threadIdx.x; li < nat; li += blockDim.x) {
         const int i = a0 + li;
         const int ti = a.tag[i];
@@ -546,9 +566,8 @@ __global__ __launch_bounds__(kBlock) void k_nl_export_brick(BrickExportArgs a) {
         long long w = a.offs ? a.offs[i] : 0;
         long long c = 0;
         for (int k = 0; k < nn; ++k) {
-            const int q = a.nbr16[(int64_t)a0 * a.cap + (int64_t)k * nat + li];
-            const int h = halo_cell_of(h_off, ht.ncell, q);
-            const int j = h_start[h] + (q - h_off[h]);
+            const int q = lst[((int64_t)(k >> 3) * nat + li) * kGroup + (k & 7)];
+            const int j = rows[q] & kRowSlotMask;
             const int tj = a.tag[j];
             if (ti < tj) {
                 if (a.offs) {
@@ -789,6 +808,13 @@ void nl_setup_grid(atvh_system* s) {
     ATVH_HIP(hipMemset(s->cell_fill, 0, sizeof(int) * s->ncells));
     dev_alloc(&s->numneigh, s->n);
     dev_alloc(&s->self16, s->n);
+    if (s->brick_mode) {
+        if (s->n > kRowSlotMask)
+            throw Error(ATVH_E_ARG, "brick layout supports at most 2^26 atoms per system");
+        dev_alloc(&s->halo_rows, size_t(s->nbricks) * kHaloCap);
+        dev_alloc(&s->halo_n, s->nbricks);
+        dev_alloc(&s->brick_range, s->nbricks);
+    }
     // capacity: 1.5x the mean full-list count at this density, plus margin
     const double vol = s->box[0] * s->box[1] * s->box[2];
     const double mean = double(s->n) / vol * 4.18879020478639 * cutoff_skin * cutoff_skin * cutoff_skin;
@@ -814,6 +840,9 @@ void nl_free(atvh_system* s) {
     dev_free(s->self16);
     dev_free(s->nbr);
     dev_free(s->nbr16);
+    dev_free(s->halo_rows);
+    dev_free(s->halo_n);
+    dev_free(s->brick_range);
 }
 
 void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
@@ -847,6 +876,10 @@ void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
     if (s->brick_mode) {
         BrickBuildArgs ba;
         ba.g = brick_geom(s);
+        ba.nbricks = s->nbricks;
+        ba.halo_rows = s->halo_rows;
+        ba.halo_n = s->halo_n;
+        ba.brick_range = s->brick_range;
         ba.x = s->x;
         ba.y = s->y;
         ba.z = s->z;
@@ -860,7 +893,7 @@ void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
         ba.numneigh = s->numneigh;
         ba.cap = s->cap;
         ba.ctl = s->ctl;
-        k_nl_build_brick<<<s->nbricks, kBlock, 0, st>>>(pred, ba);
+        k_nl_build_brick<<<std::min(s->nbricks, 2048), kBlock, 0, st>>>(pred, ba);
     } else {
         BuildArgs ba{s->x, s->y, s->z, s->cell_of, s->cell_rank, s->cell_start,
                      reinterpret_cast<const int4*>(s->d_stencil), s->nstencil, s->nc[0], s->nc[1],
@@ -1020,8 +1053,9 @@ int atvh_nlist_export_pairs(atvh_system* s, int64_t* pairs, int64_t cap, int64_t
     dev_alloc(&d_cnt, n);
     BrickExportArgs ea;
     if (s->brick_mode) {
-        ea.g = brick_geom(s);
-        ea.start = s->cell_start;
+        ea.nbricks = s->nbricks;
+        ea.halo_rows = s->halo_rows;
+        ea.brick_range = s->brick_range;
         ea.tag = s->tag;
         ea.nbr16 = s->nbr16;
         ea.numneigh = s->numneigh;

@@ -202,69 +202,55 @@ __global__ __launch_bounds__(kBlock) void k_lj_mixed(int64_t n, LJArgs a) {
 }
 
 struct BrickLJArgs {
-    BrickGeom g;
     const double *x, *y, *z;
     const int* type;
-    const int* start;
+    const int* halo_rows;
+    const int* halo_n;
+    const int2* brick_range;
     const uint16_t* nbr16;
     const uint16_t* self16;
     const int* numneigh;
     int cap;
+    double L[3];
+    double cs[3];
+    int nc[3], nb[3], bdim[3];
     double rc_sq;
     double eps24, eps4, sig6, vshift;
     const double4* tab;
     int ntypes;
     double *fx, *fy, *fz;
     double* partials;
-    DevCtl* ctl;
 };
 
-// Image position of LDS row q of the halo: raw position + the halo cell's
-// periodic shift, re-centred on that cell in case the atom wrapped since the
-// list was built (it is within skin/2 of where it was binned).
-__device__ __forceinline__ void halo_image(const BrickLJArgs& a, const HaloTable& ht, const int* h_start,
-                                           const int* h_off, const int* h_shift, int q, int& gs,
-                                           double& xx, double& yy, double& zz) {
-    const int h = halo_cell_of(h_off, ht.ncell, q);
-    gs = h_start[h] + (q - h_off[h]);
-    const int code = h_shift[h];
-    const int hz = h % ht.dims[2];
-    const int hy = (h / ht.dims[2]) % ht.dims[1];
-    const int hx = h / (ht.dims[1] * ht.dims[2]);
-    const double c0 = (ht.o[0] + hx - a.g.R[0] + 0.5) * a.g.cs[0];
-    const double c1 = (ht.o[1] + hy - a.g.R[1] + 0.5) * a.g.cs[1];
-    const double c2 = (ht.o[2] + hz - a.g.R[2] + 0.5) * a.g.cs[2];
-    xx = a.x[gs] + shift_of(code, 0, a.g.L[0]);
-    yy = a.y[gs] + shift_of(code, 1, a.g.L[1]);
-    zz = a.z[gs] + shift_of(code, 2, a.g.L[2]);
-    xx -= a.g.L[0] * rint((xx - c0) / a.g.L[0]);
-    yy -= a.g.L[1] * rint((yy - c1) / a.g.L[1]);
-    zz -= a.g.L[2] * rint((zz - c2) / a.g.L[2]);
+// Image position of a halo row: the global slot's position plus the periodic
+// shift recorded at the build (positions are not re-wrapped between builds).
+__device__ __forceinline__ void row_image(const double* __restrict__ x, const double* __restrict__ y,
+                                          const double* __restrict__ z, double lx, double ly, double lz,
+                                          int row, int& gs, double& xx, double& yy, double& zz) {
+    gs = row & kRowSlotMask;
+    const int code = static_cast<unsigned>(row) >> kRowSlotBits;
+    xx = x[gs] + shift_of(code, 0, lx);
+    yy = y[gs] + shift_of(code, 1, ly);
+    zz = z[gs] + shift_of(code, 2, lz);
 }
 
 template <bool EF, bool VF, bool TYPED>
 __global__ __launch_bounds__(kBlock) void k_lj_brick_fp64(BrickLJArgs a) {
-    __shared__ int h_start[kHaloCellCap + 1];
-    __shared__ int h_off[kHaloCellCap + 1];
-    __shared__ int h_shift[kHaloCellCap];
-    __shared__ int scan_tmp[kBlock];
     __shared__ double sx[kHaloCap], sy[kHaloCap], sz[kHaloCap];
     __shared__ int stype[TYPED ? kHaloCap : 1];
     const int b = blockIdx.x;
-    HaloTable ht;
-    build_halo_table(a.g, a.start, b, ht, h_start, h_off, h_shift, scan_tmp);
+    const int nh = a.halo_n[b];
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (ht.natoms > kHaloCap) {
-        if (threadIdx.x == 0)
-            atomicAdd(&a.ctl->halo_overflow, 1);
+    if (nh > kHaloCap) { // reported by the build; nothing valid to compute
         if (EF || VF)
             block_partials(acc, a.partials);
         return;
     }
-    for (int q = threadIdx.x; q < ht.natoms; q += blockDim.x) {
+    const int* __restrict__ rows = a.halo_rows + (int64_t)b * kHaloCap;
+    for (int q = threadIdx.x; q < nh; q += blockDim.x) {
         int gs;
         double xx, yy, zz;
-        halo_image(a, ht, h_start, h_off, h_shift, q, gs, xx, yy, zz);
+        row_image(a.x, a.y, a.z, a.L[0], a.L[1], a.L[2], rows[q], gs, xx, yy, zz);
         sx[q] = xx;
         sy[q] = yy;
         sz[q] = zz;
@@ -272,50 +258,74 @@ __global__ __launch_bounds__(kBlock) void k_lj_brick_fp64(BrickLJArgs a) {
             stype[q] = a.type[gs];
     }
     __syncthreads();
-    const int a0 = a.start[a.g.brick_cell_base[b]];
-    const int nat = a.start[a.g.brick_cell_base[b + 1]] - a0;
+    const int2 br = a.brick_range[b];
+    const int a0 = br.x, nat = br.y;
+    // kernel arguments copied to registers: the per-pair lambda below must not
+    // take the address of the argument block
     const double rc_sq = a.rc_sq;
+    const double p_eps24 = a.eps24, p_eps4 = a.eps4, p_sig6 = a.sig6, p_shift = a.vshift;
+    const double4* __restrict__ tab = a.tab;
+    const int ntypes = a.ntypes;
+    const uint4* __restrict__ base = reinterpret_cast<const uint4*>(a.nbr16 + (int64_t)a0 * a.cap);
     for (int li = threadIdx.x; li < nat; li += blockDim.x) {
         const int i = a0 + li;
         const int qs = a.self16[i];
         const double xi = sx[qs], yi = sy[qs], zi = sz[qs];
         const int ti = TYPED ? stype[qs] : 0;
         const int nn = a.numneigh[i];
-        const uint16_t* __restrict__ col = a.nbr16 + (int64_t)a0 * a.cap + li;
+        const uint4* __restrict__ grp = base + li;
+        const int ng = (nn + kGroup - 1) / kGroup;
         double fxi = 0.0, fyi = 0.0, fzi = 0.0;
-        for (int k = 0; k < nn; ++k) {
-            const int q = col[k * nat];
-            const double dx = xi - sx[q];
-            const double dy = yi - sy[q];
-            const double dz = zi - sz[q];
-            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-            if (r2 < rc_sq) {
-                double eps24 = a.eps24, eps4 = a.eps4, sig6 = a.sig6, shift = a.vshift;
-                if (TYPED) {
-                    const double4 t = a.tab[ti * a.ntypes + stype[q]];
-                    eps24 = t.x;
-                    eps4 = t.y;
-                    sig6 = t.z;
-                    shift = t.w;
+        uint4 cur = grp[0]; // in bounds even when nn == 0: the brick's rows always exist
+        for (int g = 0; g < ng; ++g) {
+            // prefetch the next 8 indices (the last group re-reads itself)
+            const uint4 nxt = grp[(int64_t)min(g + 1, ng - 1) * nat];
+            const int rem = nn - g * kGroup;
+            auto pair = [&](int q) {
+                {
+                    const double dx = xi - sx[q];
+                    const double dy = yi - sy[q];
+                    const double dz = zi - sz[q];
+                    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                    if (r2 < rc_sq) {
+                        double eps24 = p_eps24, eps4 = p_eps4, sig6 = p_sig6, shift = p_shift;
+                        if (TYPED) {
+                            const double4 t = tab[ti * ntypes + stype[q]];
+                            eps24 = t.x;
+                            eps4 = t.y;
+                            sig6 = t.z;
+                            shift = t.w;
+                        }
+                        const double r2inv = 1.0 / r2;
+                        const double r6inv = r2inv * r2inv * r2inv * sig6;
+                        const double fpair = eps24 * r2inv * fma(2.0 * r6inv, r6inv, -r6inv);
+                        fxi = fma(fpair, dx, fxi);
+                        fyi = fma(fpair, dy, fyi);
+                        fzi = fma(fpair, dz, fzi);
+                        if (EF)
+                            acc[0] += fma(eps4, r6inv * r6inv - r6inv, -shift);
+                        if (VF) {
+                            const double fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
+                            acc[1] += dx * fxp;
+                            acc[2] += dy * fyp;
+                            acc[3] += dz * fzp;
+                            acc[4] += dx * fyp;
+                            acc[5] += dx * fzp;
+                            acc[6] += dy * fzp;
+                        }
+                    }
                 }
-                const double r2inv = 1.0 / r2;
-                const double r6inv = r2inv * r2inv * r2inv * sig6;
-                const double fpair = eps24 * r2inv * fma(2.0 * r6inv, r6inv, -r6inv);
-                fxi = fma(fpair, dx, fxi);
-                fyi = fma(fpair, dy, fyi);
-                fzi = fma(fpair, dz, fzi);
-                if (EF)
-                    acc[0] += fma(eps4, r6inv * r6inv - r6inv, -shift);
-                if (VF) {
-                    const double fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
-                    acc[1] += dx * fxp;
-                    acc[2] += dy * fyp;
-                    acc[3] += dz * fzp;
-                    acc[4] += dx * fyp;
-                    acc[5] += dx * fzp;
-                    acc[6] += dy * fzp;
-                }
+            };
+            if (rem >= kGroup) {
+                pair(cur.x & 0xffffu); pair(cur.x >> 16); pair(cur.y & 0xffffu); pair(cur.y >> 16);
+                pair(cur.z & 0xffffu); pair(cur.z >> 16); pair(cur.w & 0xffffu); pair(cur.w >> 16);
+            } else {
+                const unsigned long long lo = cur.x | (static_cast<unsigned long long>(cur.y) << 32);
+                const unsigned long long hi = cur.z | (static_cast<unsigned long long>(cur.w) << 32);
+                for (int u = 0; u < rem; ++u)
+                    pair(static_cast<int>(((u < 4 ? lo : hi) >> (16 * (u & 3))) & 0xffffu));
             }
+            cur = nxt;
         }
         a.fx[i] = fxi;
         a.fy[i] = fyi;
@@ -331,84 +341,103 @@ __global__ __launch_bounds__(kBlock) void k_lj_brick_fp64(BrickLJArgs a) {
 
 template <bool EF, bool VF, bool TYPED>
 __global__ __launch_bounds__(kBlock) void k_lj_brick_mixed(BrickLJArgs a) {
-    __shared__ int h_start[kHaloCellCap + 1];
-    __shared__ int h_off[kHaloCellCap + 1];
-    __shared__ int h_shift[kHaloCellCap];
-    __shared__ int scan_tmp[kBlock];
     __shared__ float4 sp[kHaloCap]; // x, y, z relative to the brick centre; w = type bits
     const int b = blockIdx.x;
-    HaloTable ht;
-    build_halo_table(a.g, a.start, b, ht, h_start, h_off, h_shift, scan_tmp);
+    const int nh = a.halo_n[b];
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (ht.natoms > kHaloCap) {
-        if (threadIdx.x == 0)
-            atomicAdd(&a.ctl->halo_overflow, 1);
+    if (nh > kHaloCap) {
         if (EF || VF)
             block_partials(acc, a.partials);
         return;
     }
-    const double bc0 = (ht.o[0] + 0.5 * ht.d[0]) * a.g.cs[0];
-    const double bc1 = (ht.o[1] + 0.5 * ht.d[1]) * a.g.cs[1];
-    const double bc2 = (ht.o[2] + 0.5 * ht.d[2]) * a.g.cs[2];
-    for (int q = threadIdx.x; q < ht.natoms; q += blockDim.x) {
+    const int2 br = a.brick_range[b];
+    const int a0 = br.x, nat = br.y;
+    const int* __restrict__ rows = a.halo_rows + (int64_t)b * kHaloCap;
+    // reference point: the brick centre (every halo row lies within the brick
+    // widened by the stencil reach, so |x - centre| < ~7 Å and the FP32
+    // rounding of the relative coordinate is <= 2.4e-7 Å)
+    const int bz = b % a.nb[2], by = (b / a.nb[2]) % a.nb[1], bx = b / (a.nb[1] * a.nb[2]);
+    const int ox = bx * a.bdim[0], oy = by * a.bdim[1], oz = bz * a.bdim[2];
+    const double r0x = (ox + 0.5 * min(a.bdim[0], a.nc[0] - ox)) * a.cs[0];
+    const double r0y = (oy + 0.5 * min(a.bdim[1], a.nc[1] - oy)) * a.cs[1];
+    const double r0z = (oz + 0.5 * min(a.bdim[2], a.nc[2] - oz)) * a.cs[2];
+    for (int q = threadIdx.x; q < nh; q += blockDim.x) {
         int gs;
         double xx, yy, zz;
-        halo_image(a, ht, h_start, h_off, h_shift, q, gs, xx, yy, zz);
+        row_image(a.x, a.y, a.z, a.L[0], a.L[1], a.L[2], rows[q], gs, xx, yy, zz);
         float4 v;
-        v.x = static_cast<float>(xx - bc0);
-        v.y = static_cast<float>(yy - bc1);
-        v.z = static_cast<float>(zz - bc2);
+        v.x = static_cast<float>(xx - r0x);
+        v.y = static_cast<float>(yy - r0y);
+        v.z = static_cast<float>(zz - r0z);
         v.w = TYPED ? __int_as_float(a.type[gs]) : 0.0f;
         sp[q] = v;
     }
     __syncthreads();
-    const int a0 = a.start[a.g.brick_cell_base[b]];
-    const int nat = a.start[a.g.brick_cell_base[b + 1]] - a0;
     const float rc_sq = static_cast<float>(a.rc_sq);
     const float eps24s = static_cast<float>(a.eps24), eps4s = static_cast<float>(a.eps4);
     const float sig6s = static_cast<float>(a.sig6), shifts = static_cast<float>(a.vshift);
+    const double4* __restrict__ tab = a.tab;
+    const int ntypes = a.ntypes;
+    const uint4* __restrict__ base = reinterpret_cast<const uint4*>(a.nbr16 + (int64_t)a0 * a.cap);
     for (int li = threadIdx.x; li < nat; li += blockDim.x) {
         const int i = a0 + li;
-        const int qs = a.self16[i];
-        const float4 pi = sp[qs];
+        const float4 pi = sp[a.self16[i]];
         const int ti = TYPED ? __float_as_int(pi.w) : 0;
         const int nn = a.numneigh[i];
-        const uint16_t* __restrict__ col = a.nbr16 + (int64_t)a0 * a.cap + li;
+        const uint4* __restrict__ grp = base + li;
+        const int ng = (nn + kGroup - 1) / kGroup;
         float fxi = 0.f, fyi = 0.f, fzi = 0.f;
-        for (int k = 0; k < nn; ++k) {
-            const int q = col[k * nat];
-            const float4 pj = sp[q];
-            const float dx = pi.x - pj.x;
-            const float dy = pi.y - pj.y;
-            const float dz = pi.z - pj.z;
-            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-            if (r2 < rc_sq) {
-                float eps24 = eps24s, eps4 = eps4s, sig6 = sig6s, shift = shifts;
-                if (TYPED) {
-                    const double4 t = a.tab[ti * a.ntypes + __float_as_int(pj.w)];
-                    eps24 = static_cast<float>(t.x);
-                    eps4 = static_cast<float>(t.y);
-                    sig6 = static_cast<float>(t.z);
-                    shift = static_cast<float>(t.w);
+        uint4 cur = grp[0]; // in bounds even when nn == 0: the brick's rows always exist
+        for (int g = 0; g < ng; ++g) {
+            // prefetch the next 8 indices (the last group re-reads itself)
+            const uint4 nxt = grp[(int64_t)min(g + 1, ng - 1) * nat];
+            const int rem = nn - g * kGroup;
+            auto pair = [&](int q) {
+                {
+                    const float4 pj = sp[q];
+                    const float dx = pi.x - pj.x;
+                    const float dy = pi.y - pj.y;
+                    const float dz = pi.z - pj.z;
+                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    if (r2 < rc_sq) {
+                        float eps24 = eps24s, eps4 = eps4s, sig6 = sig6s, shift = shifts;
+                        if (TYPED) {
+                            const double4 t = tab[ti * ntypes + __float_as_int(pj.w)];
+                            eps24 = static_cast<float>(t.x);
+                            eps4 = static_cast<float>(t.y);
+                            sig6 = static_cast<float>(t.z);
+                            shift = static_cast<float>(t.w);
+                        }
+                        const float r2inv = __builtin_amdgcn_rcpf(r2);
+                        const float r6inv = r2inv * r2inv * r2inv * sig6;
+                        const float fpair = eps24 * r2inv * __builtin_fmaf(2.0f * r6inv, r6inv, -r6inv);
+                        fxi = __builtin_fmaf(fpair, dx, fxi);
+                        fyi = __builtin_fmaf(fpair, dy, fyi);
+                        fzi = __builtin_fmaf(fpair, dz, fzi);
+                        if (EF)
+                            acc[0] += static_cast<double>(__builtin_fmaf(eps4, r6inv * r6inv - r6inv, -shift));
+                        if (VF) {
+                            const float fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
+                            acc[1] += static_cast<double>(dx * fxp);
+                            acc[2] += static_cast<double>(dy * fyp);
+                            acc[3] += static_cast<double>(dz * fzp);
+                            acc[4] += static_cast<double>(dx * fyp);
+                            acc[5] += static_cast<double>(dx * fzp);
+                            acc[6] += static_cast<double>(dy * fzp);
+                        }
+                    }
                 }
-                const float r2inv = __builtin_amdgcn_rcpf(r2);
-                const float r6inv = r2inv * r2inv * r2inv * sig6;
-                const float fpair = eps24 * r2inv * __builtin_fmaf(2.0f * r6inv, r6inv, -r6inv);
-                fxi = __builtin_fmaf(fpair, dx, fxi);
-                fyi = __builtin_fmaf(fpair, dy, fyi);
-                fzi = __builtin_fmaf(fpair, dz, fzi);
-                if (EF)
-                    acc[0] += static_cast<double>(__builtin_fmaf(eps4, r6inv * r6inv - r6inv, -shift));
-                if (VF) {
-                    const float fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
-                    acc[1] += static_cast<double>(dx * fxp);
-                    acc[2] += static_cast<double>(dy * fyp);
-                    acc[3] += static_cast<double>(dz * fzp);
-                    acc[4] += static_cast<double>(dx * fyp);
-                    acc[5] += static_cast<double>(dx * fzp);
-                    acc[6] += static_cast<double>(dy * fzp);
-                }
+            };
+            if (rem >= kGroup) {
+                pair(cur.x & 0xffffu); pair(cur.x >> 16); pair(cur.y & 0xffffu); pair(cur.y >> 16);
+                pair(cur.z & 0xffffu); pair(cur.z >> 16); pair(cur.w & 0xffffu); pair(cur.w >> 16);
+            } else {
+                const unsigned long long lo = cur.x | (static_cast<unsigned long long>(cur.y) << 32);
+                const unsigned long long hi = cur.z | (static_cast<unsigned long long>(cur.w) << 32);
+                for (int u = 0; u < rem; ++u)
+                    pair(static_cast<int>(((u < 4 ? lo : hi) >> (16 * (u & 3))) & 0xffffu));
             }
+            cur = nxt;
         }
         a.fx[i] = static_cast<double>(fxi);
         a.fy[i] = static_cast<double>(fyi);
@@ -468,8 +497,6 @@ __global__ __launch_bounds__(kBlock) void k_lj_brick_mixed(BrickLJArgs a) {
 
 namespace atvh {
 
-BrickGeom brick_geom(const atvh_system* s);
-
 int pair_grid(const atvh_system* s) { return s->brick_mode ? s->nbricks : grid_capped(s->n); }
 
 void pair_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
@@ -514,16 +541,24 @@ void pair_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
     hipStream_t st = s->ctx->stream;
     if (s->brick_mode) {
         BrickLJArgs b;
-        b.g = brick_geom(s);
         b.x = s->x;
         b.y = s->y;
         b.z = s->z;
         b.type = s->type;
-        b.start = s->cell_start;
+        b.halo_rows = s->halo_rows;
+        b.halo_n = s->halo_n;
+        b.brick_range = s->brick_range;
         b.nbr16 = s->nbr16;
         b.self16 = s->self16;
         b.numneigh = s->numneigh;
         b.cap = s->cap;
+        for (int d = 0; d < 3; ++d) {
+            b.L[d] = s->box[d];
+            b.cs[d] = s->cs[d];
+            b.nc[d] = s->nc[d];
+            b.nb[d] = s->nb[d];
+            b.bdim[d] = s->bdim[d];
+        }
         b.rc_sq = a.rc_sq;
         b.eps24 = a.eps24;
         b.eps4 = a.eps4;
@@ -535,7 +570,6 @@ void pair_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
         b.fy = s->fy;
         b.fz = s->fz;
         b.partials = s->partials;
-        b.ctl = s->ctl;
         if (precision == ATVH_PREC_MIXED)
             LJ_BRICK_DISPATCH(k_lj_brick_mixed, eflag, vflag, typed, g, st, b);
         else

@@ -5,12 +5,27 @@ classes only marshal numpy arrays through ctypes.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 
 import numpy as np
 
 from . import _abi
 from ._abi import dptr, iptr, lptr
+
+# Live handles, released in dependency order (systems, then contexts) before
+# the interpreter and the HIP runtime shut down.
+_live_systems = weakref.WeakSet()
+_live_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _release_all():
+    for s in list(_live_systems):
+        s.close()
+    for c in list(_live_contexts):
+        c.close()
 
 
 def _check(rc, ctx_handle):
@@ -29,10 +44,12 @@ class Context:
         if rc != _abi.OK:
             raise _abi.HipError(rc, f"atvh_ctx_create(device={device}) failed")
         self.h = h
+        _live_contexts.add(self)
 
     def close(self):
-        if self.h:
-            self.lib.atvh_ctx_destroy(self.h)
+        """Destroy the context; a no-op while systems created on it are alive
+        (the last System.close() or the garbage collector finishes it)."""
+        if self.h and self.lib.atvh_ctx_destroy(self.h) == _abi.OK:
             self.h = None
 
     def __del__(self):
@@ -68,6 +85,7 @@ class System:
         h = C.c_void_p()
         _check(self.lib.atvh_system_create(ctx.h, self.n, dptr(self.box), C.byref(h)), ctx.h)
         self.h = h
+        _live_systems.add(self)
 
     def close(self):
         if self.h:

@@ -43,6 +43,8 @@ typedef struct atvh_system atvh_system;
  * and the ATV_CUDA_CHECK error path (include/atomiverse/cuda/device/error.cuh).
  */
 int atvh_ctx_create(int device, atvh_ctx** out);
+/* Fails with ATVH_E_STATE (and frees nothing) while systems created on the
+ * context are still alive. */
 int atvh_ctx_destroy(atvh_ctx* ctx);
 const char* atvh_last_error(const atvh_ctx* ctx);
 /* Use an external stream (hipStream_t passed as void*); NULL restores the
