@@ -114,6 +114,8 @@ struct atvh_system {
     bool use_half = true;
     int nstencil = 0;
     int* d_stencil = nullptr; // int4 {di,dj,dk,kind}
+    int ncols = 0;
+    int* d_cols = nullptr;    // int4 {di,dj,kmax,0}: the stencil as z-columns
     int* cell_of = nullptr;   // [n] cell of each slot (after sort: of the sorted slot)
     int* acell_of = nullptr;
     int* cell_count = nullptr;

@@ -404,9 +404,10 @@ struct BrickBuildArgs {
     const double *x, *y, *z;
     const int* cell_of;
     const int* start;
-    const int4* stencil;
-    int nstencil;
+    const int4* rows;   // stencil rows {di, dj, kmax, 0}: offsets (di, dj, -kmax..kmax)
+    int nrows;
     double rc_sq;
+    double rcs;         // cutoff + skin (for the conservative culling only)
     uint16_t* nbr16;
     uint16_t* self16;
     int* numneigh;
@@ -419,10 +420,15 @@ struct BrickBuildArgs {
 
 // Brick-mode list build: one workgroup per brick (grid-stride over bricks so a
 // skipped rebuild costs one short launch). It records the brick's halo rows
-// (global slot + image code) for the pair kernels, stages the halo's RAW
+// (global slot + image code) for the pair kernels and stages the halo's RAW
 // positions in LDS (the predicate uses the reference's shift form on raw
-// coordinates, not images), and each brick atom scans its stencil cells inside
-// the halo, storing accepted neighbours as 16-bit halo rows in 8-wide groups.
+// coordinates). Each brick atom walks the stencil as columns (di, dj): the
+// cells dk = -kmax..kmax of one column are contiguous in the halo, so one loop
+// covers them. Columns and dk ranges that cannot hold a pair are culled with a
+// conservative geometric bound (1e-6 Å slack, far above rounding); every
+// candidate left is decided by the exact reference predicate, so culling never
+// changes the pair set. Accepted neighbours are stored as 16-bit halo rows in
+// 8-wide groups.
 __global__ __launch_bounds__(kBlock) void k_nl_build_brick(const int* __restrict__ pred,
                                                            BrickBuildArgs a) {
     if (!*pred)
@@ -432,11 +438,15 @@ __global__ __launch_bounds__(kBlock) void k_nl_build_brick(const int* __restrict
     __shared__ int h_shift[kHaloCellCap];
     __shared__ int scan_tmp[kBlock];
     __shared__ double sx[kHaloCap], sy[kHaloCap], sz[kHaloCap];
+    __shared__ unsigned char shz[kHaloCap]; // halo z index of the row's cell
     __shared__ long long sh_tot[kBlock / 64];
     __shared__ int sh_max[kBlock / 64], sh_over[kBlock / 64];
     long long my_cnt_sum = 0;
     int my_max = 0, my_over = 0;
     const int nz = a.g.nc[2], ny = a.g.nc[1];
+    const double csx = a.g.cs[0], csy = a.g.cs[1], csz = a.g.cs[2];
+    const double reach = a.rcs + 1e-6;
+    const double reach_sq = reach * reach;
     for (int b = blockIdx.x; b < a.nbricks; b += gridDim.x) {
         HaloTable ht;
         build_halo_table(a.g, a.start, b, ht, h_start, h_off, h_shift, scan_tmp);
@@ -460,40 +470,63 @@ __global__ __launch_bounds__(kBlock) void k_nl_build_brick(const int* __restrict
             sx[q] = a.x[gs];
             sy[q] = a.y[gs];
             sz[q] = a.z[gs];
+            shz[q] = static_cast<unsigned char>(h % ht.dims[2]);
         }
         __syncthreads();
-        const int hy = ht.dims[1], hz = ht.dims[2];
+        const int HY = ht.dims[1], HZ = ht.dims[2];
+        const int Rx = a.g.R[0], Ry = a.g.R[1], Rz = a.g.R[2];
         uint16_t* __restrict__ lst = a.nbr16 + (int64_t)a0 * a.cap;
         for (int li = threadIdx.x; li < nat; li += blockDim.x) {
             const int i = a0 + li;
             const int ci = a.cell_of[i];
-            const int lz = ci % nz - ht.o[2];
-            const int ly = (ci / nz) % ny - ht.o[1];
-            const int lx = ci / (ny * nz) - ht.o[0];
-            const int hs = ((lx + a.g.R[0]) * hy + (ly + a.g.R[1])) * hz + (lz + a.g.R[2]);
+            const int cz = ci % nz, cy = (ci / nz) % ny, cx = ci / (ny * nz);
+            const int lx = cx - ht.o[0], ly = cy - ht.o[1], lz = cz - ht.o[2];
+            const int hx0 = lx + Rx, hy0 = ly + Ry, hz0 = lz + Rz;
+            const int hs = (hx0 * HY + hy0) * HZ + hz0;
             const int qs = h_off[hs] + (i - h_start[hs]);
             const double xi = sx[qs], yi = sy[qs], zi = sz[qs];
+            // position inside the home cell (binning used the wrapped coordinate)
+            const double fx = (xi - a.g.L[0] * floor(xi / a.g.L[0])) - cx * csx;
+            const double fy = (yi - a.g.L[1] * floor(yi / a.g.L[1])) - cy * csy;
+            const double fz = (zi - a.g.L[2] * floor(zi / a.g.L[2])) - cz * csz;
             int cnt = 0;
-            for (int st = 0; st < a.nstencil; ++st) {
-                const int4 o = a.stencil[st];
-                const int h = ((lx + a.g.R[0] + o.x) * hy + (ly + a.g.R[1] + o.y)) * hz + (lz + a.g.R[2] + o.z);
-                const int code = h_shift[h];
+            for (int r = 0; r < a.nrows; ++r) {
+                const int4 rw = a.rows[r];
+                const int di = rw.x, dj = rw.y, kmax = rw.z;
+                // distance from the atom to the column's x/y slab
+                const double gx = fmax(0.0, fmax(di * csx - fx, fx - (di + 1) * csx));
+                const double gy = fmax(0.0, fmax(dj * csy - fy, fy - (dj + 1) * csy));
+                const double g2 = gx * gx + gy * gy;
+                if (g2 > reach_sq)
+                    continue;
+                const double zr = sqrt(reach_sq - g2);
+                const int klo = max(-kmax, static_cast<int>(floor((fz - zr) / csz)));
+                const int khi = min(kmax, static_cast<int>(floor((fz + zr) / csz)));
+                if (klo > khi)
+                    continue;
+                const int hcol = ((hx0 + di) * HY + (hy0 + dj)) * HZ;
+                const int code = h_shift[hcol + hz0]; // x/y shifts are shared by the column
                 const double shx = shift_of(code, 0, a.g.L[0]);
                 const double shy = shift_of(code, 1, a.g.L[1]);
-                const double shz = shift_of(code, 2, a.g.L[2]);
-                const int qb = h_off[h], qe = h_off[h + 1];
+                // column orientation: forward, backward, or decided per dk (di = dj = 0)
+                const int colkind = (di > 0 || (di == 0 && dj > 0)) ? 1 : ((di == 0 && dj == 0) ? 0 : 2);
+                const int qb = h_off[hcol + hz0 + klo], qe = h_off[hcol + hz0 + khi + 1];
                 for (int q = qb; q < qe; ++q) {
-                    if (q == qs)
+                    const int dk = static_cast<int>(shz[q]) - hz0;
+                    const int uz = cz + dk;
+                    const double shzv = uz < 0 ? -a.g.L[2] : (uz >= nz ? a.g.L[2] : 0.0);
+                    const int kind = colkind != 0 ? colkind : (dk > 0 ? 1 : (dk < 0 ? 2 : 0));
+                    if (kind == 0 && q == qs)
                         continue;
                     double dx, dy, dz;
-                    if (o.w == 2) {
+                    if (kind == 2) {
                         dx = sx[q] - (xi - shx);
                         dy = sy[q] - (yi - shy);
-                        dz = sz[q] - (zi - shz);
+                        dz = sz[q] - (zi - shzv);
                     } else {
                         dx = xi - (sx[q] + shx);
                         dy = yi - (sy[q] + shy);
-                        dz = zi - (sz[q] + shz);
+                        dz = zi - (sz[q] + shzv);
                     }
                     const double r2 = dx * dx + dy * dy + dz * dz;
                     if (r2 < a.rc_sq) {
@@ -768,7 +801,26 @@ void nl_setup_grid(atvh_system* s) {
                 st.push_back(home ? 0 : (forward ? 1 : 2));
             }
     s->nstencil = static_cast<int>(st.size() / 4);
+    // the stencil as columns (di, dj, kmax): for fixed (di, dj) the kept dk
+    // form the symmetric range -kmax..kmax (min distance grows with |dk|)
+    std::vector<int> cols;
+    for (int di = -s->reach[0]; di <= s->reach[0]; ++di)
+        for (int dj = -s->reach[1]; dj <= s->reach[1]; ++dj) {
+            int kmax = -1;
+            for (int dk = 0; dk <= s->reach[2]; ++dk)
+                if (min_dist_sq(di, dj, dk) <= rc_sq)
+                    kmax = dk;
+            if (kmax < 0)
+                continue;
+            cols.push_back(di);
+            cols.push_back(dj);
+            cols.push_back(kmax);
+            cols.push_back(0);
+        }
+    s->ncols = static_cast<int>(cols.size() / 4);
     nl_free(s);
+    dev_alloc(&s->d_cols, cols.size());
+    ATVH_HIP(hipMemcpy(s->d_cols, cols.data(), cols.size() * sizeof(int), hipMemcpyHostToDevice));
     // brick layout when the half-stencil arithmetic applies, else one brick
     // per cell and the global-slot fallback list
     s->brick_mode = s->use_half;
@@ -827,6 +879,7 @@ void nl_setup_grid(atvh_system* s) {
 
 void nl_free(atvh_system* s) {
     dev_free(s->d_stencil);
+    dev_free(s->d_cols);
     dev_free(s->cell_rank);
     dev_free(s->brick_cell_base);
     dev_free(s->cell_of);
@@ -885,9 +938,10 @@ void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
         ba.z = s->z;
         ba.cell_of = s->cell_of;
         ba.start = s->cell_start;
-        ba.stencil = reinterpret_cast<const int4*>(s->d_stencil);
-        ba.nstencil = s->nstencil;
+        ba.rows = reinterpret_cast<const int4*>(s->d_cols);
+        ba.nrows = s->ncols;
         ba.rc_sq = s->cutoff_skin * s->cutoff_skin;
+        ba.rcs = s->cutoff_skin;
         ba.nbr16 = s->nbr16;
         ba.self16 = s->self16;
         ba.numneigh = s->numneigh;
