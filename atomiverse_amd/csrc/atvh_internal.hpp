@@ -138,7 +138,9 @@ struct atvh_system {
     int* brick_cell_base = nullptr; // [nbricks+1]
     uint16_t* nbr16 = nullptr;      // per brick, 8-wide groups: [a0*cap + (g*nat + li)*8 + u]
     uint16_t* self16 = nullptr;     // [n] own row in the brick's halo
+    int* halo_cells = nullptr;      // [nbricks*kHaloCellCap] static: cell rank | image code << 26
     int* halo_rows = nullptr;       // [nbricks*kHaloCap] global slot | image code << 26
+    int* halo_off = nullptr;        // [nbricks*(kHaloCellCap+1)] halo row offset per halo cell
     int* halo_n = nullptr;          // [nbricks] rows in each halo
     int2* brick_range = nullptr;    // [nbricks] (first slot, atom count)
 

@@ -48,11 +48,14 @@ __device__ __forceinline__ void brick_coords(const BrickGeom& g, int b, int& bx,
 }
 
 // Fill the halo table of brick b into LDS: per halo cell its first global
-// slot (h_start), its LDS offset (h_off, exclusive scan of the counts) and
-// the periodic shift code of its image (h_shift: 3 x 2 bits, 0 = -L, 1 = 0,
-// 2 = +L). All threads of the block participate; ends with a barrier.
-__device__ __forceinline__ void build_halo_table(const BrickGeom& g, const int* __restrict__ cell_start,
-                                                 int b, HaloTable& ht, int* h_start, int* h_off,
+// slot (h_start), its LDS offset (h_off, exclusive scan of the counts) and the
+// periodic shift code of its image (h_shift: 3 x 2 bits, 0 = -L, 1 = 0,
+// 2 = +L). `cells` is the static per-brick table (rank | code << 26) built on
+// the host, so every halo cell costs two independent loads of cell_start.
+// All threads of the block participate; ends with a barrier.
+__device__ __forceinline__ void build_halo_table(const BrickGeom& g, const int* __restrict__ cells,
+                                                 const int* __restrict__ cell_start, int b,
+                                                 HaloTable& ht, int* h_start, int* h_off,
                                                  int* h_shift, int* scan_tmp) {
     int bx, by, bz;
     brick_coords(g, b, bx, by, bz);
@@ -65,37 +68,23 @@ __device__ __forceinline__ void build_halo_table(const BrickGeom& g, const int* 
     }
     ht.ncell = ht.dims[0] * ht.dims[1] * ht.dims[2];
     const int nthr = blockDim.x;
-    // per-thread chunk of consecutive halo cells, then a block scan of chunk sums
+    const int* __restrict__ tab = cells + (int64_t)b * kHaloCellCap;
+    for (int h = threadIdx.x; h < ht.ncell; h += nthr) {
+        const int e = tab[h];
+        const int r = e & kRowSlotMask;
+        const int s0 = cell_start[r], s1 = cell_start[r + 1];
+        h_start[h] = s0;
+        h_shift[h] = static_cast<unsigned>(e) >> kRowSlotBits;
+        h_off[h] = s1 - s0; // count, scanned below
+    }
+    __syncthreads();
+    // block exclusive scan: per-thread chunk of consecutive cells, then chunk sums
     const int per = (ht.ncell + nthr - 1) / nthr;
     const int h0 = threadIdx.x * per;
     const int h1 = min(ht.ncell, h0 + per);
     int local = 0;
-    for (int h = h0; h < h1; ++h) {
-        const int hz = h % ht.dims[2];
-        const int hy = (h / ht.dims[2]) % ht.dims[1];
-        const int hx = h / (ht.dims[1] * ht.dims[2]);
-        int u[3] = {ht.o[0] + hx - g.R[0], ht.o[1] + hy - g.R[1], ht.o[2] + hz - g.R[2]};
-        int code = 0;
-        for (int a = 0; a < 3; ++a) {
-            int s = 1;
-            if (u[a] < 0) {
-                u[a] += g.nc[a];
-                s = 0;
-            } else if (u[a] >= g.nc[a]) {
-                u[a] -= g.nc[a];
-                s = 2;
-            }
-            code |= s << (2 * a);
-        }
-        const int c = u[0] * g.nc[1] * g.nc[2] + u[1] * g.nc[2] + u[2];
-        const int r = g.cell_rank[c];
-        const int s0 = cell_start[r], s1 = cell_start[r + 1];
-        h_start[h] = s0;
-        h_shift[h] = code;
-        h_off[h] = s1 - s0; // count, scanned below
-        local += s1 - s0;
-    }
-    // block exclusive scan of `local`
+    for (int h = h0; h < h1; ++h)
+        local += h_off[h];
     scan_tmp[threadIdx.x] = local;
     __syncthreads();
     for (int off = 1; off < nthr; off <<= 1) {
@@ -114,19 +103,6 @@ __device__ __forceinline__ void build_halo_table(const BrickGeom& g, const int* 
     if (threadIdx.x == 0)
         h_off[ht.ncell] = ht.natoms;
     __syncthreads();
-}
-
-// Halo cell of LDS row q (binary search over h_off).
-__device__ __forceinline__ int halo_cell_of(const int* h_off, int ncell, int q) {
-    int lo = 0, hi = ncell; // h_off[lo] <= q < h_off[hi]
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (h_off[mid] <= q)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
 }
 
 __device__ __forceinline__ double shift_of(int code, int axis, double L) {

@@ -400,6 +400,8 @@ __global__ __launch_bounds__(kBlock) void k_nl_build(const int* __restrict__ pre
 
 struct BrickBuildArgs {
     BrickGeom g;
+    const int* halo_cells; // static per-brick halo cell table (rank | code << 26)
+    int* halo_off;         // [nbricks*(kHaloCellCap+1)] halo row offset of every halo cell
     int nbricks;
     const double *x, *y, *z;
     const int* cell_of;
@@ -418,132 +420,202 @@ struct BrickBuildArgs {
     DevCtl* ctl;
 };
 
-// Brick-mode list build: one workgroup per brick (grid-stride over bricks so a
-// skipped rebuild costs one short launch). It records the brick's halo rows
-// (global slot + image code) for the pair kernels and stages the halo's RAW
-// positions in LDS (the predicate uses the reference's shift form on raw
-// coordinates). Each brick atom walks the stencil as columns (di, dj): the
-// cells dk = -kmax..kmax of one column are contiguous in the halo, so one loop
-// covers them. Columns and dk ranges that cannot hold a pair are culled with a
-// conservative geometric bound (1e-6 Å slack, far above rounding); every
-// candidate left is decided by the exact reference predicate, so culling never
-// changes the pair set. Accepted neighbours are stored as 16-bit halo rows in
-// 8-wide groups.
-__global__ __launch_bounds__(kBlock) void k_nl_build_brick(const int* __restrict__ pred,
-                                                           BrickBuildArgs a) {
+constexpr int kMaxCols = 25;     // stencil columns: (2*reach+1)^2 with reach <= 2
+constexpr int kMaxBrickAtoms = 384;
+
+// Rebuild step 1 (one workgroup per brick, small LDS, high occupancy): the
+// brick's halo table from the static halo-cell table and the new cell_start —
+// halo rows (global slot | image code), per-halo-cell row offsets, the halo
+// size and the brick's slot range.
+__global__ __launch_bounds__(kBlock) void k_nl_halo(const int* __restrict__ pred, BrickBuildArgs a) {
     if (!*pred)
         return;
     __shared__ int h_start[kHaloCellCap + 1];
     __shared__ int h_off[kHaloCellCap + 1];
     __shared__ int h_shift[kHaloCellCap];
     __shared__ int scan_tmp[kBlock];
-    __shared__ double sx[kHaloCap], sy[kHaloCap], sz[kHaloCap];
-    __shared__ unsigned char shz[kHaloCap]; // halo z index of the row's cell
-    __shared__ long long sh_tot[kBlock / 64];
-    __shared__ int sh_max[kBlock / 64], sh_over[kBlock / 64];
-    long long my_cnt_sum = 0;
-    int my_max = 0, my_over = 0;
-    const int nz = a.g.nc[2], ny = a.g.nc[1];
-    const double csx = a.g.cs[0], csy = a.g.cs[1], csz = a.g.cs[2];
-    const double reach = a.rcs + 1e-6;
-    const double reach_sq = reach * reach;
     for (int b = blockIdx.x; b < a.nbricks; b += gridDim.x) {
         HaloTable ht;
-        build_halo_table(a.g, a.start, b, ht, h_start, h_off, h_shift, scan_tmp);
+        build_halo_table(a.g, a.halo_cells, a.start, b, ht, h_start, h_off, h_shift, scan_tmp);
         const int a0 = a.start[a.g.brick_cell_base[b]];
         const int nat = a.start[a.g.brick_cell_base[b + 1]] - a0;
         if (threadIdx.x == 0) {
             a.halo_n[b] = ht.natoms;
             a.brick_range[b] = make_int2(a0, nat);
-            if (ht.natoms > kHaloCap)
+            if (ht.natoms > kHaloCap || nat > kMaxBrickAtoms)
                 atomicAdd(&a.ctl->halo_overflow, 1);
         }
-        if (ht.natoms > kHaloCap) {
-            __syncthreads();
-            continue;
+        int* __restrict__ hoff = a.halo_off + (int64_t)b * (kHaloCellCap + 1);
+        for (int h = threadIdx.x; h <= ht.ncell; h += blockDim.x)
+            hoff[h] = h_off[h];
+        if (ht.natoms <= kHaloCap) {
+            int* __restrict__ rows = a.halo_rows + (int64_t)b * kHaloCap;
+            for (int h = threadIdx.x; h < ht.ncell; h += blockDim.x) {
+                const int q0 = h_off[h], nq = h_off[h + 1] - q0, g0 = h_start[h];
+                const int tag = h_shift[h] << kRowSlotBits;
+                for (int k = 0; k < nq; ++k)
+                    rows[q0 + k] = (g0 + k) | tag;
+            }
         }
-        int* __restrict__ rows = a.halo_rows + (int64_t)b * kHaloCap;
-        for (int q = threadIdx.x; q < ht.natoms; q += blockDim.x) {
-            const int h = halo_cell_of(h_off, ht.ncell, q);
-            const int gs = h_start[h] + (q - h_off[h]);
-            rows[q] = gs | (h_shift[h] << kRowSlotBits);
-            sx[q] = a.x[gs];
-            sy[q] = a.y[gs];
-            sz[q] = a.z[gs];
-            shz[q] = static_cast<unsigned char>(h % ht.dims[2]);
-        }
+        __syncthreads(); // LDS reused by the next brick
+    }
+}
+
+// Rebuild step 2 (one workgroup per brick): the brick's halo RAW positions in
+// LDS (the predicate uses the reference's shift form on raw coordinates);
+// work items are (brick atom, stencil column) pairs laid out column-major so
+// the lanes of a wave share one column (same x/y shift and orientation). Each
+// item culls the column's dk range with a conservative geometric bound (1e-6
+// Å slack, far above rounding) and tests the remaining candidates with the
+// exact reference predicate. Pass A counts per item, a per-atom prefix over the
+// columns fixes every item's output offset (column order, then halo row
+// order: deterministic), pass B writes the 16-bit rows in 8-wide groups.
+__global__ __launch_bounds__(kBlock) void k_nl_pairs(const int* __restrict__ pred, BrickBuildArgs a) {
+    if (!*pred)
+        return;
+    __shared__ double sx[kHaloCap], sy[kHaloCap], sz[kHaloCap];
+    __shared__ unsigned char shz[kHaloCap];
+    __shared__ int h_off[kHaloCellCap + 1];
+    __shared__ unsigned short item_off[kMaxBrickAtoms * kMaxCols]; // counts, then offsets
+    __shared__ unsigned short self_q[kMaxBrickAtoms];
+    __shared__ unsigned char self_cell[kMaxBrickAtoms][3];
+    __shared__ long long sh_tot[kBlock / 64];
+    __shared__ int sh_max[kBlock / 64], sh_over[kBlock / 64];
+    long long my_cnt_sum = 0;
+    int my_max = 0, my_over = 0;
+    const int nz = a.g.nc[2], ny = a.g.nc[1];
+    const double Lx = a.g.L[0], Ly = a.g.L[1], Lz = a.g.L[2];
+    const double csx = a.g.cs[0], csy = a.g.cs[1], csz = a.g.cs[2];
+    const double icsz = 1.0 / csz;
+    const double reach = a.rcs + 1e-6;
+    const double reach_sq = reach * reach;
+    const int ncols = a.nrows;
+    for (int b = blockIdx.x; b < a.nbricks; b += gridDim.x) {
+        const int nh = a.halo_n[b];
+        const int2 br = a.brick_range[b];
+        const int a0 = br.x, nat = br.y;
+        if (nh > kHaloCap || nat > kMaxBrickAtoms)
+            continue; // reported by k_nl_halo (block-uniform)
+        int bx, by, bz;
+        brick_coords(a.g, b, bx, by, bz);
+        const int o0 = bx * a.g.bdim[0], o1 = by * a.g.bdim[1], o2 = bz * a.g.bdim[2];
+        const int d1 = min(a.g.bdim[1], a.g.nc[1] - o1), d2 = min(a.g.bdim[2], a.g.nc[2] - o2);
+        const int HY = d1 + 2 * a.g.R[1], HZ = d2 + 2 * a.g.R[2];
+        const int d0 = min(a.g.bdim[0], a.g.nc[0] - o0);
+        const int ncell = (d0 + 2 * a.g.R[0]) * HY * HZ;
+        const int* __restrict__ hoff = a.halo_off + (int64_t)b * (kHaloCellCap + 1);
+        for (int h = threadIdx.x; h <= ncell; h += blockDim.x)
+            h_off[h] = hoff[h];
+        const int* __restrict__ rows = a.halo_rows + (int64_t)b * kHaloCap;
         __syncthreads();
-        const int HY = ht.dims[1], HZ = ht.dims[2];
-        const int Rx = a.g.R[0], Ry = a.g.R[1], Rz = a.g.R[2];
-        uint16_t* __restrict__ lst = a.nbr16 + (int64_t)a0 * a.cap;
+        for (int h = threadIdx.x; h < ncell; h += blockDim.x) {
+            const int q0 = h_off[h], q1 = h_off[h + 1];
+            const unsigned char hzc = static_cast<unsigned char>(h % HZ);
+            for (int q = q0; q < q1; ++q) {
+                const int gs = rows[q] & kRowSlotMask;
+                sx[q] = a.x[gs];
+                sy[q] = a.y[gs];
+                sz[q] = a.z[gs];
+                shz[q] = hzc;
+            }
+        }
+        // brick atoms: own halo row and home cell
         for (int li = threadIdx.x; li < nat; li += blockDim.x) {
             const int i = a0 + li;
             const int ci = a.cell_of[i];
-            const int cz = ci % nz, cy = (ci / nz) % ny, cx = ci / (ny * nz);
-            const int lx = cx - ht.o[0], ly = cy - ht.o[1], lz = cz - ht.o[2];
-            const int hx0 = lx + Rx, hy0 = ly + Ry, hz0 = lz + Rz;
-            const int hs = (hx0 * HY + hy0) * HZ + hz0;
-            const int qs = h_off[hs] + (i - h_start[hs]);
-            const double xi = sx[qs], yi = sy[qs], zi = sz[qs];
-            // position inside the home cell (binning used the wrapped coordinate)
-            const double fx = (xi - a.g.L[0] * floor(xi / a.g.L[0])) - cx * csx;
-            const double fy = (yi - a.g.L[1] * floor(yi / a.g.L[1])) - cy * csy;
-            const double fz = (zi - a.g.L[2] * floor(zi / a.g.L[2])) - cz * csz;
-            int cnt = 0;
-            for (int r = 0; r < a.nrows; ++r) {
-                const int4 rw = a.rows[r];
+            const int lz = ci % nz - o2, ly = (ci / nz) % ny - o1, lx = ci / (ny * nz) - o0;
+            const int hs = ((lx + a.g.R[0]) * HY + (ly + a.g.R[1])) * HZ + (lz + a.g.R[2]);
+            // first slot of the home cell = the slot of its first halo row
+            const int q0 = h_off[hs];
+            const int first = rows[q0] & kRowSlotMask;
+            self_q[li] = static_cast<unsigned short>(q0 + (i - first));
+            self_cell[li][0] = static_cast<unsigned char>(lx);
+            self_cell[li][1] = static_cast<unsigned char>(ly);
+            self_cell[li][2] = static_cast<unsigned char>(lz);
+        }
+        __syncthreads();
+        const int nitems = nat * ncols;
+        uint16_t* __restrict__ lst = a.nbr16 + (int64_t)a0 * a.cap;
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int t = threadIdx.x; t < nitems; t += blockDim.x) {
+                const int col = t / nat, li = t - col * nat;
+                const int4 rw = a.rows[col];
                 const int di = rw.x, dj = rw.y, kmax = rw.z;
-                // distance from the atom to the column's x/y slab
+                const int lx = self_cell[li][0], ly = self_cell[li][1], lz = self_cell[li][2];
+                const int qs = self_q[li];
+                const double xi = sx[qs], yi = sy[qs], zi = sz[qs];
+                const int cx = o0 + lx, cy = o1 + ly, cz = o2 + lz;
+                // position inside the home cell (binning used the wrapped coordinate)
+                const double fx = (xi - Lx * floor(xi / Lx)) - cx * csx;
+                const double fy = (yi - Ly * floor(yi / Ly)) - cy * csy;
+                const double fz = (zi - Lz * floor(zi / Lz)) - cz * csz;
                 const double gx = fmax(0.0, fmax(di * csx - fx, fx - (di + 1) * csx));
                 const double gy = fmax(0.0, fmax(dj * csy - fy, fy - (dj + 1) * csy));
                 const double g2 = gx * gx + gy * gy;
-                if (g2 > reach_sq)
-                    continue;
-                const double zr = sqrt(reach_sq - g2);
-                const int klo = max(-kmax, static_cast<int>(floor((fz - zr) / csz)));
-                const int khi = min(kmax, static_cast<int>(floor((fz + zr) / csz)));
-                if (klo > khi)
-                    continue;
-                const int hcol = ((hx0 + di) * HY + (hy0 + dj)) * HZ;
-                const int code = h_shift[hcol + hz0]; // x/y shifts are shared by the column
-                const double shx = shift_of(code, 0, a.g.L[0]);
-                const double shy = shift_of(code, 1, a.g.L[1]);
-                // column orientation: forward, backward, or decided per dk (di = dj = 0)
-                const int colkind = (di > 0 || (di == 0 && dj > 0)) ? 1 : ((di == 0 && dj == 0) ? 0 : 2);
-                const int qb = h_off[hcol + hz0 + klo], qe = h_off[hcol + hz0 + khi + 1];
-                for (int q = qb; q < qe; ++q) {
-                    const int dk = static_cast<int>(shz[q]) - hz0;
-                    const int uz = cz + dk;
-                    const double shzv = uz < 0 ? -a.g.L[2] : (uz >= nz ? a.g.L[2] : 0.0);
-                    const int kind = colkind != 0 ? colkind : (dk > 0 ? 1 : (dk < 0 ? 2 : 0));
-                    if (kind == 0 && q == qs)
-                        continue;
-                    double dx, dy, dz;
-                    if (kind == 2) {
-                        dx = sx[q] - (xi - shx);
-                        dy = sy[q] - (yi - shy);
-                        dz = sz[q] - (zi - shzv);
-                    } else {
-                        dx = xi - (sx[q] + shx);
-                        dy = yi - (sy[q] + shy);
-                        dz = zi - (sz[q] + shzv);
-                    }
-                    const double r2 = dx * dx + dy * dy + dz * dz;
-                    if (r2 < a.rc_sq) {
-                        if (cnt < a.cap)
-                            lst[((int64_t)(cnt >> 3) * nat + li) * kGroup + (cnt & 7)] = static_cast<uint16_t>(q);
-                        ++cnt;
+                int cnt = 0;
+                int outpos = pass ? item_off[li * kMaxCols + col] : 0;
+                if (g2 <= reach_sq) {
+                    const double zr = static_cast<double>(sqrtf(static_cast<float>(reach_sq - g2))) + 1e-6;
+                    const int klo = max(-kmax, static_cast<int>(floor((fz - zr) * icsz)));
+                    const int khi = min(kmax, static_cast<int>(floor((fz + zr) * icsz)));
+                    if (klo <= khi) {
+                        const int hz0 = lz + a.g.R[2];
+                        const int hcol = ((lx + a.g.R[0] + di) * HY + (ly + a.g.R[1] + dj)) * HZ;
+                        const int ux = cx + di, uy = cy + dj;
+                        const double shx = ux < 0 ? -Lx : (ux >= a.g.nc[0] ? Lx : 0.0);
+                        const double shy = uy < 0 ? -Ly : (uy >= a.g.nc[1] ? Ly : 0.0);
+                        const int colkind = (di > 0 || (di == 0 && dj > 0)) ? 1 : ((di == 0 && dj == 0) ? 0 : 2);
+                        const int qb = h_off[hcol + hz0 + klo], qe = h_off[hcol + hz0 + khi + 1];
+                        for (int q = qb; q < qe; ++q) {
+                            const int dk = static_cast<int>(shz[q]) - hz0;
+                            const int uz = cz + dk;
+                            const double shzv = uz < 0 ? -Lz : (uz >= nz ? Lz : 0.0);
+                            const int kind = colkind != 0 ? colkind : (dk > 0 ? 1 : (dk < 0 ? 2 : 0));
+                            double dx, dy, dz;
+                            if (kind == 2) {
+                                dx = sx[q] - (xi - shx);
+                                dy = sy[q] - (yi - shy);
+                                dz = sz[q] - (zi - shzv);
+                            } else {
+                                dx = xi - (sx[q] + shx);
+                                dy = yi - (sy[q] + shy);
+                                dz = zi - (sz[q] + shzv);
+                            }
+                            const double r2 = dx * dx + dy * dy + dz * dz;
+                            if (r2 < a.rc_sq && q != qs) {
+                                if (pass && outpos < a.cap)
+                                    lst[((int64_t)(outpos >> 3) * nat + li) * kGroup + (outpos & 7)] =
+                                        static_cast<uint16_t>(q);
+                                ++outpos;
+                                ++cnt;
+                            }
+                        }
                     }
                 }
+                if (!pass)
+                    item_off[li * kMaxCols + col] = static_cast<unsigned short>(cnt);
             }
-            a.numneigh[i] = cnt < a.cap ? cnt : a.cap;
-            a.self16[i] = static_cast<uint16_t>(qs);
-            my_cnt_sum += cnt;
-            my_max = max(my_max, cnt);
-            if (cnt > a.cap)
-                ++my_over;
+            __syncthreads();
+            if (!pass) {
+                // per-atom exclusive prefix over the columns
+                for (int li = threadIdx.x; li < nat; li += blockDim.x) {
+                    int run = 0;
+                    for (int col = 0; col < ncols; ++col) {
+                        const int c = item_off[li * kMaxCols + col];
+                        item_off[li * kMaxCols + col] = static_cast<unsigned short>(run);
+                        run += c;
+                    }
+                    const int i = a0 + li;
+                    a.numneigh[i] = run < a.cap ? run : a.cap;
+                    a.self16[i] = self_q[li];
+                    my_cnt_sum += run;
+                    my_max = max(my_max, run);
+                    if (run > a.cap)
+                        ++my_over;
+                }
+                __syncthreads();
+            }
         }
-        __syncthreads(); // LDS reused by the next brick
     }
     for (int off = 32; off > 0; off >>= 1) {
         my_cnt_sum += __shfl_down(my_cnt_sum, off, 64);
@@ -699,7 +771,7 @@ static void choose_bricks(atvh_system* s, int first) {
             hc *= d[a] + 2 * s->reach[a];
             bc *= d[a];
         }
-        if (hc <= kHaloCellCap && per_cell * hc <= 0.7 * kHaloCap && per_cell * bc <= 1.25 * kBlock) {
+        if (hc <= kHaloCellCap && per_cell * hc <= 0.7 * kHaloCap && per_cell * bc <= kBlock) {
             pick = k;
             break;
         }
@@ -734,6 +806,46 @@ static void build_rank_tables(atvh_system* s, std::vector<int>& rank, std::vecto
                         }
             }
     base[s->nbricks] = next;
+}
+
+// Static halo table of every brick: for each halo cell (z-fastest over the
+// brick widened by the reach) the layout rank of the periodic cell it images
+// and the shift code of that image (per axis 0: -L, 1: none, 2: +L).
+static void build_halo_cell_tables(atvh_system* s, const std::vector<int>& rank, std::vector<int>& out) {
+    if (s->ncells > kRowSlotMask)
+        throw Error(ATVH_E_ARG, "brick layout supports at most 2^26 cells");
+    out.assign(size_t(s->nbricks) * kHaloCellCap, 0);
+    for (int bx = 0; bx < s->nb[0]; ++bx)
+        for (int by = 0; by < s->nb[1]; ++by)
+            for (int bz = 0; bz < s->nb[2]; ++bz) {
+                const int b = (bx * s->nb[1] + by) * s->nb[2] + bz;
+                const int o[3] = {bx * s->bdim[0], by * s->bdim[1], bz * s->bdim[2]};
+                int dims[3];
+                for (int a = 0; a < 3; ++a)
+                    dims[a] = std::min(s->bdim[a], s->nc[a] - o[a]) + 2 * s->reach[a];
+                if (dims[0] * dims[1] * dims[2] > kHaloCellCap)
+                    throw Error(ATVH_E_ARG, "brick halo has too many cells");
+                int h = 0;
+                for (int hx = 0; hx < dims[0]; ++hx)
+                    for (int hy = 0; hy < dims[1]; ++hy)
+                        for (int hz = 0; hz < dims[2]; ++hz, ++h) {
+                            int u[3] = {o[0] + hx - s->reach[0], o[1] + hy - s->reach[1], o[2] + hz - s->reach[2]};
+                            int code = 0;
+                            for (int a = 0; a < 3; ++a) {
+                                int sc = 1;
+                                if (u[a] < 0) {
+                                    u[a] += s->nc[a];
+                                    sc = 0;
+                                } else if (u[a] >= s->nc[a]) {
+                                    u[a] -= s->nc[a];
+                                    sc = 2;
+                                }
+                                code |= sc << (2 * a);
+                            }
+                            const int c = (u[0] * s->nc[1] + u[1]) * s->nc[2] + u[2];
+                            out[size_t(b) * kHaloCellCap + h] = rank[c] | (code << kRowSlotBits);
+                        }
+            }
 }
 
 BrickGeom brick_geom(const atvh_system* s) {
@@ -818,6 +930,8 @@ void nl_setup_grid(atvh_system* s) {
             cols.push_back(0);
         }
     s->ncols = static_cast<int>(cols.size() / 4);
+    if (s->ncols > kMaxCols)
+        throw Error(ATVH_E_ARG, "stencil has more columns than the list build supports");
     nl_free(s);
     dev_alloc(&s->d_cols, cols.size());
     ATVH_HIP(hipMemcpy(s->d_cols, cols.data(), cols.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -835,6 +949,9 @@ void nl_setup_grid(atvh_system* s) {
     }
     std::vector<int> rank, base;
     build_rank_tables(s, rank, base);
+    std::vector<int> hcells;
+    if (s->brick_mode)
+        build_halo_cell_tables(s, rank, hcells);
     // per-block reduction partials: one row per pair workgroup (bricks) or grid block
     const int need = std::max(kMaxGrid, s->nbricks);
     if (need > s->npartials) {
@@ -844,6 +961,10 @@ void nl_setup_grid(atvh_system* s) {
     }
     dev_alloc(&s->d_stencil, st.size());
     ATVH_HIP(hipMemcpy(s->d_stencil, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (s->brick_mode) {
+        dev_alloc(&s->halo_cells, hcells.size());
+        ATVH_HIP(hipMemcpy(s->halo_cells, hcells.data(), hcells.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
     dev_alloc(&s->cell_rank, s->ncells);
     ATVH_HIP(hipMemcpy(s->cell_rank, rank.data(), rank.size() * sizeof(int), hipMemcpyHostToDevice));
     dev_alloc(&s->brick_cell_base, s->nbricks + 1);
@@ -864,6 +985,7 @@ void nl_setup_grid(atvh_system* s) {
         if (s->n > kRowSlotMask)
             throw Error(ATVH_E_ARG, "brick layout supports at most 2^26 atoms per system");
         dev_alloc(&s->halo_rows, size_t(s->nbricks) * kHaloCap);
+        dev_alloc(&s->halo_off, size_t(s->nbricks) * (kHaloCellCap + 1));
         dev_alloc(&s->halo_n, s->nbricks);
         dev_alloc(&s->brick_range, s->nbricks);
     }
@@ -879,6 +1001,7 @@ void nl_setup_grid(atvh_system* s) {
 
 void nl_free(atvh_system* s) {
     dev_free(s->d_stencil);
+    dev_free(s->halo_cells);
     dev_free(s->d_cols);
     dev_free(s->cell_rank);
     dev_free(s->brick_cell_base);
@@ -894,6 +1017,7 @@ void nl_free(atvh_system* s) {
     dev_free(s->nbr);
     dev_free(s->nbr16);
     dev_free(s->halo_rows);
+    dev_free(s->halo_off);
     dev_free(s->halo_n);
     dev_free(s->brick_range);
 }
@@ -929,6 +1053,7 @@ void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
     if (s->brick_mode) {
         BrickBuildArgs ba;
         ba.g = brick_geom(s);
+        ba.halo_cells = s->halo_cells;
         ba.nbricks = s->nbricks;
         ba.halo_rows = s->halo_rows;
         ba.halo_n = s->halo_n;
@@ -947,7 +1072,9 @@ void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
         ba.numneigh = s->numneigh;
         ba.cap = s->cap;
         ba.ctl = s->ctl;
-        k_nl_build_brick<<<std::min(s->nbricks, 2048), kBlock, 0, st>>>(pred, ba);
+        ba.halo_off = s->halo_off;
+        k_nl_halo<<<std::min(s->nbricks, 4096), kBlock, 0, st>>>(pred, ba);
+        k_nl_pairs<<<std::min(s->nbricks, 2048), kBlock, 0, st>>>(pred, ba);
     } else {
         BuildArgs ba{s->x, s->y, s->z, s->cell_of, s->cell_rank, s->cell_start,
                      reinterpret_cast<const int4*>(s->d_stencil), s->nstencil, s->nc[0], s->nc[1],
