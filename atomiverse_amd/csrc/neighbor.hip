@@ -421,7 +421,7 @@ struct BrickBuildArgs {
 };
 
 constexpr int kMaxCols = 25;     // stencil columns: (2*reach+1)^2 with reach <= 2
-constexpr int kMaxBrickAtoms = 384;
+constexpr int kMaxBrickAtoms = 320;
 
 // Rebuild step 1 (one workgroup per brick, small LDS, high occupancy): the
 // brick's halo table from the static halo-cell table and the new cell_start —
@@ -461,34 +461,62 @@ __global__ __launch_bounds__(kBlock) void k_nl_halo(const int* __restrict__ pred
     }
 }
 
-// Rebuild step 2 (one workgroup per brick): the brick's halo RAW positions in
-// LDS (the predicate uses the reference's shift form on raw coordinates);
-// work items are (brick atom, stencil column) pairs laid out column-major so
-// the lanes of a wave share one column (same x/y shift and orientation). Each
-// item culls the column's dk range with a conservative geometric bound (1e-6
-// Å slack, far above rounding) and tests the remaining candidates with the
-// exact reference predicate. Pass A counts per item, a per-atom prefix over the
-// columns fixes every item's output offset (column order, then halo row
-// order: deterministic), pass B writes the 16-bit rows in 8-wide groups.
+// Rebuild step 2 (one workgroup per brick). Work items are (brick atom,
+// stencil column) pairs laid out column-major so the lanes of a wave share one
+// column (same x/y image and orientation). Each item culls the column's dk
+// range with a conservative geometric bound, then screens its candidates with
+// FP32 image coordinates (relative to the brick centre: |error of r^2| <=
+// 1e-5 Å^2): a candidate is accepted below (rc+skin)^2 - kBand, rejected above
+// (rc+skin)^2 + kBand, and only inside the band decided by the exact reference
+// predicate — raw FP64 positions, the shift form of the half-stencil offset in
+// its canonical orientation (refresh_half, cell_list.cpp:680-727). The pair
+// set is therefore exactly the reference's. Pass A counts per item, a
+// per-atom prefix over the columns fixes every item's output offset (column
+// order, then halo row order: deterministic), pass B writes the 16-bit rows in
+// 8-wide groups.
+constexpr float kBand = 1e-4f;
+
+__device__ __noinline__ bool exact_pair(const double* __restrict__ x, const double* __restrict__ y,
+                                        const double* __restrict__ z, double rc_sq, int gi, int gj,
+                                        int kind, double shx, double shy, double shz) {
+    const double xi = x[gi], yi = y[gi], zi = z[gi];
+    const double xj = x[gj], yj = y[gj], zj = z[gj];
+    double dx, dy, dz;
+    if (kind == 2) {
+        dx = xj - (xi - shx);
+        dy = yj - (yi - shy);
+        dz = zj - (zi - shz);
+    } else {
+        dx = xi - (xj + shx);
+        dy = yi - (yj + shy);
+        dz = zi - (zj + shz);
+    }
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    return r2 < rc_sq;
+}
+
 __global__ __launch_bounds__(kBlock) void k_nl_pairs(const int* __restrict__ pred, BrickBuildArgs a) {
     if (!*pred)
         return;
-    __shared__ double sx[kHaloCap], sy[kHaloCap], sz[kHaloCap];
-    __shared__ unsigned char shz[kHaloCap];
+    __shared__ float4 sp[kHaloCap]; // image x, y, z relative to the brick centre; w = halo z index
     __shared__ int h_off[kHaloCellCap + 1];
     __shared__ unsigned short item_off[kMaxBrickAtoms * kMaxCols]; // counts, then offsets
     __shared__ unsigned short self_q[kMaxBrickAtoms];
-    __shared__ unsigned char self_cell[kMaxBrickAtoms][3];
+    __shared__ unsigned char self_cell[kMaxBrickAtoms][4];
     __shared__ long long sh_tot[kBlock / 64];
     __shared__ int sh_max[kBlock / 64], sh_over[kBlock / 64];
     long long my_cnt_sum = 0;
     int my_max = 0, my_over = 0;
     const int nz = a.g.nc[2], ny = a.g.nc[1];
     const double Lx = a.g.L[0], Ly = a.g.L[1], Lz = a.g.L[2];
-    const double csx = a.g.cs[0], csy = a.g.cs[1], csz = a.g.cs[2];
-    const double icsz = 1.0 / csz;
-    const double reach = a.rcs + 1e-6;
-    const double reach_sq = reach * reach;
+    const float csxf = static_cast<float>(a.g.cs[0]), csyf = static_cast<float>(a.g.cs[1]);
+    const float cszf = static_cast<float>(a.g.cs[2]);
+    const float icszf = 1.0f / cszf;
+    const float rcsf = static_cast<float>(a.rcs);
+    const float reach = rcsf + 1e-4f; // culling slack >> FP32 rounding of the coordinates
+    const float reach_sq = reach * reach;
+    const float rc2f = static_cast<float>(a.rc_sq);
+    const float lo_acc = rc2f - kBand, hi_rej = rc2f + kBand;
     const int ncols = a.nrows;
     for (int b = blockIdx.x; b < a.nbricks; b += gridDim.x) {
         const int nh = a.halo_n[b];
@@ -499,10 +527,13 @@ __global__ __launch_bounds__(kBlock) void k_nl_pairs(const int* __restrict__ pre
         int bx, by, bz;
         brick_coords(a.g, b, bx, by, bz);
         const int o0 = bx * a.g.bdim[0], o1 = by * a.g.bdim[1], o2 = bz * a.g.bdim[2];
+        const int d0 = min(a.g.bdim[0], a.g.nc[0] - o0);
         const int d1 = min(a.g.bdim[1], a.g.nc[1] - o1), d2 = min(a.g.bdim[2], a.g.nc[2] - o2);
         const int HY = d1 + 2 * a.g.R[1], HZ = d2 + 2 * a.g.R[2];
-        const int d0 = min(a.g.bdim[0], a.g.nc[0] - o0);
         const int ncell = (d0 + 2 * a.g.R[0]) * HY * HZ;
+        const double bc0 = (o0 + 0.5 * d0) * a.g.cs[0];
+        const double bc1 = (o1 + 0.5 * d1) * a.g.cs[1];
+        const double bc2 = (o2 + 0.5 * d2) * a.g.cs[2];
         const int* __restrict__ hoff = a.halo_off + (int64_t)b * (kHaloCellCap + 1);
         for (int h = threadIdx.x; h <= ncell; h += blockDim.x)
             h_off[h] = hoff[h];
@@ -510,13 +541,17 @@ __global__ __launch_bounds__(kBlock) void k_nl_pairs(const int* __restrict__ pre
         __syncthreads();
         for (int h = threadIdx.x; h < ncell; h += blockDim.x) {
             const int q0 = h_off[h], q1 = h_off[h + 1];
-            const unsigned char hzc = static_cast<unsigned char>(h % HZ);
+            const float hzc = __int_as_float(h % HZ);
             for (int q = q0; q < q1; ++q) {
-                const int gs = rows[q] & kRowSlotMask;
-                sx[q] = a.x[gs];
-                sy[q] = a.y[gs];
-                sz[q] = a.z[gs];
-                shz[q] = hzc;
+                const int row = rows[q];
+                const int gs = row & kRowSlotMask;
+                const int code = static_cast<unsigned>(row) >> kRowSlotBits;
+                float4 v;
+                v.x = static_cast<float>((a.x[gs] + shift_of(code, 0, Lx)) - bc0);
+                v.y = static_cast<float>((a.y[gs] + shift_of(code, 1, Ly)) - bc1);
+                v.z = static_cast<float>((a.z[gs] + shift_of(code, 2, Lz)) - bc2);
+                v.w = hzc;
+                sp[q] = v;
             }
         }
         // brick atoms: own halo row and home cell
@@ -525,9 +560,8 @@ __global__ __launch_bounds__(kBlock) void k_nl_pairs(const int* __restrict__ pre
             const int ci = a.cell_of[i];
             const int lz = ci % nz - o2, ly = (ci / nz) % ny - o1, lx = ci / (ny * nz) - o0;
             const int hs = ((lx + a.g.R[0]) * HY + (ly + a.g.R[1])) * HZ + (lz + a.g.R[2]);
-            // first slot of the home cell = the slot of its first halo row
             const int q0 = h_off[hs];
-            const int first = rows[q0] & kRowSlotMask;
+            const int first = rows[q0] & kRowSlotMask; // slot of the home cell's first atom
             self_q[li] = static_cast<unsigned short>(q0 + (i - first));
             self_cell[li][0] = static_cast<unsigned char>(lx);
             self_cell[li][1] = static_cast<unsigned char>(ly);
@@ -543,46 +577,43 @@ __global__ __launch_bounds__(kBlock) void k_nl_pairs(const int* __restrict__ pre
                 const int di = rw.x, dj = rw.y, kmax = rw.z;
                 const int lx = self_cell[li][0], ly = self_cell[li][1], lz = self_cell[li][2];
                 const int qs = self_q[li];
-                const double xi = sx[qs], yi = sy[qs], zi = sz[qs];
-                const int cx = o0 + lx, cy = o1 + ly, cz = o2 + lz;
-                // position inside the home cell (binning used the wrapped coordinate)
-                const double fx = (xi - Lx * floor(xi / Lx)) - cx * csx;
-                const double fy = (yi - Ly * floor(yi / Ly)) - cy * csy;
-                const double fz = (zi - Lz * floor(zi / Lz)) - cz * csz;
-                const double gx = fmax(0.0, fmax(di * csx - fx, fx - (di + 1) * csx));
-                const double gy = fmax(0.0, fmax(dj * csy - fy, fy - (dj + 1) * csy));
-                const double g2 = gx * gx + gy * gy;
+                const float4 pi = sp[qs];
+                // position inside the home cell, from the brick-centre frame
+                const float fx = pi.x - (lx - 0.5f * d0) * csxf;
+                const float fy = pi.y - (ly - 0.5f * d1) * csyf;
+                const float fz = pi.z - (lz - 0.5f * d2) * cszf;
+                const float gx = fmaxf(0.f, fmaxf(di * csxf - fx, fx - (di + 1) * csxf));
+                const float gy = fmaxf(0.f, fmaxf(dj * csyf - fy, fy - (dj + 1) * csyf));
+                const float g2 = gx * gx + gy * gy;
                 int cnt = 0;
                 int outpos = pass ? item_off[li * kMaxCols + col] : 0;
                 if (g2 <= reach_sq) {
-                    const double zr = static_cast<double>(sqrtf(static_cast<float>(reach_sq - g2))) + 1e-6;
-                    const int klo = max(-kmax, static_cast<int>(floor((fz - zr) * icsz)));
-                    const int khi = min(kmax, static_cast<int>(floor((fz + zr) * icsz)));
+                    const float zr = sqrtf(reach_sq - g2) + 1e-4f;
+                    const int klo = max(-kmax, static_cast<int>(floorf((fz - zr) * icszf)));
+                    const int khi = min(kmax, static_cast<int>(floorf((fz + zr) * icszf)));
                     if (klo <= khi) {
                         const int hz0 = lz + a.g.R[2];
                         const int hcol = ((lx + a.g.R[0] + di) * HY + (ly + a.g.R[1] + dj)) * HZ;
-                        const int ux = cx + di, uy = cy + dj;
-                        const double shx = ux < 0 ? -Lx : (ux >= a.g.nc[0] ? Lx : 0.0);
-                        const double shy = uy < 0 ? -Ly : (uy >= a.g.nc[1] ? Ly : 0.0);
-                        const int colkind = (di > 0 || (di == 0 && dj > 0)) ? 1 : ((di == 0 && dj == 0) ? 0 : 2);
                         const int qb = h_off[hcol + hz0 + klo], qe = h_off[hcol + hz0 + khi + 1];
                         for (int q = qb; q < qe; ++q) {
-                            const int dk = static_cast<int>(shz[q]) - hz0;
-                            const int uz = cz + dk;
-                            const double shzv = uz < 0 ? -Lz : (uz >= nz ? Lz : 0.0);
-                            const int kind = colkind != 0 ? colkind : (dk > 0 ? 1 : (dk < 0 ? 2 : 0));
-                            double dx, dy, dz;
-                            if (kind == 2) {
-                                dx = sx[q] - (xi - shx);
-                                dy = sy[q] - (yi - shy);
-                                dz = sz[q] - (zi - shzv);
-                            } else {
-                                dx = xi - (sx[q] + shx);
-                                dy = yi - (sy[q] + shy);
-                                dz = zi - (sz[q] + shzv);
+                            const float4 pj = sp[q];
+                            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                            bool acc = r2 < lo_acc;
+                            if (!acc && r2 <= hi_rej && q != qs) {
+                                // inside the band: the reference predicate, exactly
+                                const int dk = __float_as_int(pj.w) - hz0;
+                                const int cx = o0 + lx, cy = o1 + ly, cz = o2 + lz;
+                                const int ux = cx + di, uy = cy + dj, uz = cz + dk;
+                                const double shx = ux < 0 ? -Lx : (ux >= a.g.nc[0] ? Lx : 0.0);
+                                const double shy = uy < 0 ? -Ly : (uy >= a.g.nc[1] ? Ly : 0.0);
+                                const double shzv = uz < 0 ? -Lz : (uz >= nz ? Lz : 0.0);
+                                const int colkind = (di > 0 || (di == 0 && dj > 0)) ? 1 : ((di == 0 && dj == 0) ? 0 : 2);
+                                const int kind = colkind != 0 ? colkind : (dk > 0 ? 1 : (dk < 0 ? 2 : 0));
+                                acc = exact_pair(a.x, a.y, a.z, a.rc_sq, a0 + li, rows[q] & kRowSlotMask, kind,
+                                                 shx, shy, shzv);
                             }
-                            const double r2 = dx * dx + dy * dy + dz * dz;
-                            if (r2 < a.rc_sq && q != qs) {
+                            if (acc && q != qs) {
                                 if (pass && outpos < a.cap)
                                     lst[((int64_t)(outpos >> 3) * nat + li) * kGroup + (outpos & 7)] =
                                         static_cast<uint16_t>(q);
