@@ -72,3 +72,12 @@ def test_rocksalt_generator():
     i = 0
     j = 1  # +x neighbour
     assert q[i] == -q[j] and x[j] - x[i] == 1.5
+
+
+def test_hip_cpp_layer_compiles():
+    """The hip:: component header compiles stand-alone as C++20 (no HIP headers)."""
+    src = "#include \"atomiverse/hip/md.hpp\"\nint main(){ ATV::hip::md::MD* p = nullptr; (void)p; return 0; }\n"
+    out = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                          "-I", os.path.join(ROOT, "include"), "-x", "c++", "-"], input=src, text=True,
+                         capture_output=True)
+    assert out.returncode == 0, out.stderr
