@@ -543,6 +543,17 @@ int orc_md_run(int64_t n, double* x, double* y, double* z, double* vx, double* v
                double* fx, double* fy, double* fz, const double* mass, const double box[3],
                const orc_md_params* p, orc_md_state* st, int nsteps, double* pe, double* ke,
                double* virial_out) {
+    return orc_md_run_ff(n, x, y, z, vx, vy, vz, fx, fy, fz, mass, box, p, nullptr, st, nsteps, pe,
+                         ke, virial_out);
+}
+
+// Same loop; the force phase is PairLJCut, or PairLJCutCoulLong followed by
+// KSpacePME when `ff` is given (evaluators run in insertion order at
+// Phase::force, include/atomiverse/core/plan.h stable sort).
+int orc_md_run_ff(int64_t n, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                  double* fx, double* fy, double* fz, const double* mass, const double box[3],
+                  const orc_md_params* p, const orc_md_ff* ff, orc_md_state* st, int nsteps,
+                  double* pe, double* ke, double* virial_out) {
     if (n <= 0 || !p || !st)
         return -1;
     const int nthreads = p->nthreads < 1 ? 1 : p->nthreads;
@@ -625,9 +636,19 @@ int orc_md_run(int64_t n, double* x, double* y, double* z, double* vx, double* v
         std::memset(fy, 0, n * sizeof(double));
         std::memset(fz, 0, n * sizeof(double));
         double e = 0.0, vir[6] = {0, 0, 0, 0, 0, 0};
-        pair_lj(n, x, y, z, box, hl.numneigh.data(), hl.firstneigh.data(), hl.neighbors.data(),
-                p->eps, p->sigma, p->cutoff, 0, nullptr, nullptr, nullptr, nthreads, fx, fy, fz, &e,
-                vir);
+        if (!ff) {
+            pair_lj(n, x, y, z, box, hl.numneigh.data(), hl.firstneigh.data(), hl.neighbors.data(),
+                    p->eps, p->sigma, p->cutoff, 0, nullptr, nullptr, nullptr, nthreads, fx, fy, fz,
+                    &e, vir);
+        } else {
+            orc_pair_lj_coul_long(n, x, y, z, ff->q, ff->type, box, hl.numneigh.data(),
+                                  hl.firstneigh.data(), hl.neighbors.data(), ff->ntypes, ff->lj1,
+                                  ff->lj2, ff->lj3, ff->lj4, ff->shift, p->cutoff, ff->e_tab,
+                                  ff->f_tab, ff->nbin, nthreads, fx, fy, fz, &e, vir);
+            if (ff->pme_order > 0)
+                orc_pme(n, x, y, z, ff->q, box, ff->alpha, ff->pme_K, ff->pme_order, fx, fy, fz,
+                        &e, vir, nullptr);
+        }
         // final_integrate
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; ++i) {

@@ -79,6 +79,71 @@ void orc_pair_lj(int64_t n, const double* x, const double* y, const double* z,
  * whose diagonal is set (PairLJCut::finalize_type_table_). */
 void orc_lj_mix(int ntypes, double* eps_tab, double* sigma_tab);
 
+
+/* ---- long-range electrostatics (atv_pme.cpp) ------------------------------ */
+
+/* good_fft_size: smallest m >= max(n, 4) factoring into {2,3,5,7}. */
+int orc_good_fft_size(int n);
+/* bspline: cardinal B-spline weights theta[order] and derivatives. */
+void orc_bspline(double frac, int order, double* theta, double* dtheta);
+/* spline_moduli: Euler moduli mod[K]. */
+void orc_spline_moduli(int K, int order, double* mod);
+/* EwaldSum::recommend_alpha */
+double orc_ewald_alpha(double cutoff, double accuracy);
+/* KSpacePME mesh dimensions for an orthorhombic box and grid spacing. */
+void orc_pme_grid(const double box[3], double spacing, int order, int K[3]);
+/* Unnormalised 3-D complex DFT, data interleaved (re, im), layout
+ * k2 + K2 (k1 + K1 k0); sign -1 forward, +1 backward. */
+void orc_fft3d(int K0, int K1, int K2, double* data, int sign);
+/* KSpacePME::compute (orthorhombic, one rank). Accumulates forces, energy
+ * (self + recip + net) and the reciprocal virial; parts[3] (optional) gets
+ * e_recip, e_self, e_net. Returns 0, or -1 on bad arguments. */
+int orc_pme(int64_t n, const double* x, const double* y, const double* z,
+            const double* q, const double box[3], double alpha, const int K[3],
+            int order, double* fx, double* fy, double* fz, double* energy,
+            double* virial, double* parts);
+/* ErfcTable::build: e_tab[nbin+1], f_tab[nbin+1]. */
+void orc_erfc_table(double alpha, double cutoff, int nbin, double* e_tab,
+                    double* f_tab);
+/* PairLJCutCoulLong::finalize_type_table_: mixes unset entries in place and
+ * fills lj1..lj4 and the energy shift (ntypes*ntypes each). */
+void orc_ljcoul_tables(int ntypes, double cutoff, double* eps, double* sigma,
+                       const int* set, double* lj1, double* lj2, double* lj3,
+                       double* lj4, double* shift);
+/* PairLJCutCoulLong::compute on a half CSR (accumulating). */
+void orc_pair_lj_coul_long(int64_t n, const double* x, const double* y,
+                           const double* z, const double* q, const int* type,
+                           const double box[3], const int* numneigh,
+                           const int64_t* firstneigh, const int* neighbors,
+                           int ntypes, const double* lj1, const double* lj2,
+                           const double* lj3, const double* lj4,
+                           const double* shift, double cutoff,
+                           const double* e_tab, const double* f_tab, int nbin,
+                           int nthreads, double* fx, double* fy, double* fz,
+                           double* energy, double* virial);
+/* CoulLong::compute (closed-form erfc real space, accumulating). */
+void orc_coul_long(int64_t n, const double* x, const double* y, const double* z,
+                   const double* q, const double box[3], const int* numneigh,
+                   const int64_t* firstneigh, const int* neighbors,
+                   double alpha, double cutoff, double* fx, double* fy,
+                   double* fz, double* energy, double* virial);
+
+/* Force field of a charged run for orc_md_run_ff: PairLJCutCoulLong with
+ * per-type tables (lj1..lj4 and shift from orc_ljcoul_tables) and the erfc table,
+ * followed by KSpacePME when pme_order > 0 (the reference's evaluator
+ * order: pair first, then k-space). */
+typedef struct orc_md_ff {
+    int ntypes;
+    const int* type;
+    const double* q;
+    const double *lj1, *lj2, *lj3, *lj4, *shift;
+    const double *e_tab, *f_tab;
+    int nbin;
+    double alpha;
+    int pme_K[3];
+    int pme_order;
+} orc_md_ff;
+
 /* ---- integrators ----------------------------------------------------------- */
 
 typedef struct orc_md_params {
@@ -112,6 +177,15 @@ int orc_md_run(int64_t n, double* x, double* y, double* z, double* vx,
                const double* mass, const double box[3],
                const orc_md_params* p, orc_md_state* state, int nsteps,
                double* pe, double* ke, double* virial);
+
+/* orc_md_run with the charged force field `ff` in place of single-type LJ
+ * (p->eps / p->sigma unused; p->cutoff is the real-space cutoff). */
+int orc_md_run_ff(int64_t n, double* x, double* y, double* z, double* vx,
+                  double* vy, double* vz, double* fx, double* fy, double* fz,
+                  const double* mass, const double box[3],
+                  const orc_md_params* p, const orc_md_ff* ff,
+                  orc_md_state* state, int nsteps, double* pe, double* ke,
+                  double* virial);
 
 /* KineticEnergy::launch: 0.5 * mvv2e * sum m v^2 */
 double orc_kinetic_energy(int64_t n, const double* mass, const double* vx,

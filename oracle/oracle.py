@@ -40,6 +40,25 @@ class MDParams(C.Structure):
     ]
 
 
+class MDForceField(C.Structure):
+    _fields_ = [
+        ("ntypes", C.c_int),
+        ("type", _ip),
+        ("q", _dp),
+        ("lj1", _dp),
+        ("lj2", _dp),
+        ("lj3", _dp),
+        ("lj4", _dp),
+        ("shift", _dp),
+        ("e_tab", _dp),
+        ("f_tab", _dp),
+        ("nbin", C.c_int),
+        ("alpha", C.c_double),
+        ("pme_K", C.c_int * 3),
+        ("pme_order", C.c_int),
+    ]
+
+
 class MDState(C.Structure):
     _fields_ = [("xi", C.c_double), ("step", C.c_int64), ("nrebuilds", C.c_int64), ("ago", C.c_int64)]
 
@@ -81,6 +100,36 @@ def lib():
         L.orc_md_run.restype = C.c_int
         L.orc_md_run.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                  C.POINTER(MDParams), C.POINTER(MDState), C.c_int, _dp, _dp, _dp]
+        L.orc_md_run_ff.restype = C.c_int
+        L.orc_md_run_ff.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                    C.POINTER(MDParams), C.POINTER(MDForceField), C.POINTER(MDState),
+                                    C.c_int, _dp, _dp, _dp]
+        L.orc_good_fft_size.restype = C.c_int
+        L.orc_good_fft_size.argtypes = [C.c_int]
+        L.orc_bspline.restype = None
+        L.orc_bspline.argtypes = [C.c_double, C.c_int, _dp, _dp]
+        L.orc_spline_moduli.restype = None
+        L.orc_spline_moduli.argtypes = [C.c_int, C.c_int, _dp]
+        L.orc_ewald_alpha.restype = C.c_double
+        L.orc_ewald_alpha.argtypes = [C.c_double, C.c_double]
+        L.orc_pme_grid.restype = None
+        L.orc_pme_grid.argtypes = [_dp, C.c_double, C.c_int, _ip]
+        L.orc_fft3d.restype = None
+        L.orc_fft3d.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_int]
+        L.orc_pme.restype = C.c_int
+        L.orc_pme.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp, C.c_double, _ip, C.c_int, _dp, _dp, _dp,
+                              _dp, _dp, _dp]
+        L.orc_erfc_table.restype = None
+        L.orc_erfc_table.argtypes = [C.c_double, C.c_double, C.c_int, _dp, _dp]
+        L.orc_ljcoul_tables.restype = None
+        L.orc_ljcoul_tables.argtypes = [C.c_int, C.c_double, _dp, _dp, _ip, _dp, _dp, _dp, _dp, _dp]
+        L.orc_pair_lj_coul_long.restype = None
+        L.orc_pair_lj_coul_long.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _ip, _dp, _ip, _lp, _ip, C.c_int,
+                                            _dp, _dp, _dp, _dp, _dp, C.c_double, _dp, _dp, C.c_int, C.c_int,
+                                            _dp, _dp, _dp, _dp, _dp]
+        L.orc_coul_long.restype = None
+        L.orc_coul_long.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp, _ip, _lp, _ip, C.c_double,
+                                    C.c_double, _dp, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -225,4 +274,155 @@ def md_run(state_arrays, box, params: MDParams, nsteps, md_state: MDState | None
                           C.byref(params), C.byref(md_state), nsteps, _d(pe), _d(ke), _d(vir))
     if rc != 0:
         raise RuntimeError(f"orc_md_run failed ({rc})")
+    return pe, ke, vir, md_state
+
+
+# ---- long-range electrostatics ------------------------------------------------
+
+def good_fft_size(n):
+    return lib().orc_good_fft_size(int(n))
+
+
+def bspline(frac, order):
+    th = np.zeros(order)
+    dth = np.zeros(order)
+    lib().orc_bspline(float(frac), order, _d(th), _d(dth))
+    return th, dth
+
+
+def spline_moduli(K, order):
+    m = np.zeros(K)
+    lib().orc_spline_moduli(K, order, _d(m))
+    return m
+
+
+def ewald_alpha(cutoff, accuracy=1e-5):
+    return lib().orc_ewald_alpha(cutoff, accuracy)
+
+
+def pme_grid(box, spacing, order):
+    box = np.ascontiguousarray(box, dtype=np.float64)
+    K = np.zeros(3, np.int32)
+    lib().orc_pme_grid(_d(box), spacing, order, _i(K))
+    return K
+
+
+def fft3d(a, sign):
+    """Unnormalised 3-D DFT of a complex array (sign -1 forward, +1 backward)."""
+    a = np.ascontiguousarray(a, dtype=np.complex128).copy()
+    K0, K1, K2 = a.shape
+    lib().orc_fft3d(K0, K1, K2, a.ctypes.data_as(_dp), sign)
+    return a
+
+
+def pme(x, y, z, q, box, alpha, K, order):
+    """KSpacePME::compute -> (fx, fy, fz, energy, virial[6], (e_recip, e_self, e_net))."""
+    x, y, z, q = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, q))
+    box = np.ascontiguousarray(box, dtype=np.float64)
+    K = np.ascontiguousarray(K, dtype=np.int32)
+    n = x.size
+    fx, fy, fz = (np.zeros(n) for _ in range(3))
+    e = np.zeros(1)
+    vir = np.zeros(6)
+    parts = np.zeros(3)
+    rc = lib().orc_pme(n, _d(x), _d(y), _d(z), _d(q), _d(box), alpha, _i(K), order, _d(fx), _d(fy), _d(fz),
+                       _d(e), _d(vir), _d(parts))
+    if rc != 0:
+        raise ValueError("orc_pme: bad arguments")
+    return fx, fy, fz, float(e[0]), vir, tuple(parts)
+
+
+def erfc_table(alpha, cutoff, nbin=10000):
+    e = np.zeros(nbin + 1)
+    f = np.zeros(nbin + 1)
+    lib().orc_erfc_table(alpha, cutoff, nbin, _d(e), _d(f))
+    return e, f
+
+
+class LJCoulTables:
+    """finalize_type_table_ output for a coefficient table (ntypes x ntypes;
+    `set` marks entries given explicitly, the rest are mixed)."""
+
+    def __init__(self, eps, sigma, set_, cutoff, alpha, nbin=10000):
+        self.eps = np.ascontiguousarray(eps, dtype=np.float64).reshape(-1).copy()
+        self.sigma = np.ascontiguousarray(sigma, dtype=np.float64).reshape(-1).copy()
+        self.set = np.ascontiguousarray(set_, dtype=np.int32).reshape(-1)
+        self.ntypes = int(round(np.sqrt(self.eps.size)))
+        self.lj1, self.lj2, self.lj3, self.lj4, self.shift = (np.zeros(self.eps.size) for _ in range(5))
+        lib().orc_ljcoul_tables(self.ntypes, cutoff, _d(self.eps), _d(self.sigma), _i(self.set), _d(self.lj1),
+                                _d(self.lj2), _d(self.lj3), _d(self.lj4), _d(self.shift))
+        self.cutoff = cutoff
+        self.alpha = alpha
+        self.nbin = nbin
+        self.e_tab, self.f_tab = erfc_table(alpha, cutoff, nbin)
+
+
+def pair_lj_coul_long(x, y, z, q, type_, box, half, tab: LJCoulTables, nthreads=1):
+    """PairLJCutCoulLong::compute on a half CSR -> (fx, fy, fz, energy, virial[6])."""
+    numneigh, first, nb = half
+    x, y, z, q = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, q))
+    type_ = np.ascontiguousarray(type_, dtype=np.int32)
+    box = np.ascontiguousarray(box, dtype=np.float64)
+    n = x.size
+    fx, fy, fz = (np.zeros(n) for _ in range(3))
+    e = np.zeros(1)
+    vir = np.zeros(6)
+    lib().orc_pair_lj_coul_long(n, _d(x), _d(y), _d(z), _d(q), _i(type_), _d(box), _i(numneigh), _l(first),
+                                _i(nb), tab.ntypes, _d(tab.lj1), _d(tab.lj2), _d(tab.lj3), _d(tab.lj4),
+                                _d(tab.shift), tab.cutoff, _d(tab.e_tab), _d(tab.f_tab), tab.nbin, nthreads,
+                                _d(fx), _d(fy), _d(fz), _d(e), _d(vir))
+    return fx, fy, fz, float(e[0]), vir
+
+
+def coul_long(x, y, z, q, box, half, alpha, cutoff):
+    """CoulLong::compute (closed-form erfc) -> (fx, fy, fz, energy, virial[6])."""
+    numneigh, first, nb = half
+    x, y, z, q = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, q))
+    box = np.ascontiguousarray(box, dtype=np.float64)
+    n = x.size
+    fx, fy, fz = (np.zeros(n) for _ in range(3))
+    e = np.zeros(1)
+    vir = np.zeros(6)
+    lib().orc_coul_long(n, _d(x), _d(y), _d(z), _d(q), _d(box), _i(numneigh), _l(first), _i(nb), alpha,
+                        cutoff, _d(fx), _d(fy), _d(fz), _d(e), _d(vir))
+    return fx, fy, fz, float(e[0]), vir
+
+
+def all_pairs_half(n):
+    """Half CSR holding every pair j > i (the reference tests' build_full_halflist)."""
+    numneigh = (n - 1 - np.arange(n)).astype(np.int32)
+    first = np.concatenate([[0], np.cumsum(numneigh)[:-1]]).astype(np.int64)
+    nb = np.concatenate([np.arange(i + 1, n) for i in range(n)] + [np.zeros(0, np.int64)]).astype(np.int32)
+    return numneigh, first, nb
+
+
+def md_run_ff(state_arrays, box, params: MDParams, tab: LJCoulTables, type_, q, pme_K, pme_order, nsteps,
+              md_state: MDState | None = None):
+    """orc_md_run_ff: the MD loop with PairLJCutCoulLong (+ KSpacePME when pme_order > 0)."""
+    s = state_arrays
+    box = np.ascontiguousarray(box, dtype=np.float64)
+    type_ = np.ascontiguousarray(type_, dtype=np.int32)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    ff = MDForceField()
+    ff.ntypes = tab.ntypes
+    ff.type = _i(type_)
+    ff.q = _d(q)
+    ff.lj1, ff.lj2, ff.lj3, ff.lj4, ff.shift = (_d(a) for a in (tab.lj1, tab.lj2, tab.lj3, tab.lj4, tab.shift))
+    ff.e_tab, ff.f_tab = _d(tab.e_tab), _d(tab.f_tab)
+    ff.nbin = tab.nbin
+    ff.alpha = tab.alpha
+    for d in range(3):
+        ff.pme_K[d] = int(pme_K[d]) if pme_order > 0 else 0
+    ff.pme_order = int(pme_order)
+    if md_state is None:
+        md_state = MDState()
+    pe = np.zeros(nsteps)
+    ke = np.zeros(nsteps)
+    vir = np.zeros(6)
+    keep = (type_, q)  # noqa: F841  (alive across the call)
+    rc = lib().orc_md_run_ff(s["x"].size, _d(s["x"]), _d(s["y"]), _d(s["z"]), _d(s["vx"]), _d(s["vy"]),
+                             _d(s["vz"]), _d(s["fx"]), _d(s["fy"]), _d(s["fz"]), _d(s["mass"]), _d(box),
+                             C.byref(params), C.byref(ff), C.byref(md_state), nsteps, _d(pe), _d(ke), _d(vir))
+    if rc != 0:
+        raise RuntimeError(f"orc_md_run_ff failed ({rc})")
     return pe, ke, vir, md_state
