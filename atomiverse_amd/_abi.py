@@ -47,7 +47,11 @@ SIGNATURES = {
     "atvh_nlist_stats": (C.c_int, [_vp, _lp, _lp, _ip, _ip]),
     "atvh_nlist_export_pairs": (C.c_int, [_vp, _lp, C.c_int64, _lp]),
     "atvh_pair_lj_configure": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp]),
+    "atvh_pair_ljcoul_configure": (C.c_int, [_vp, C.c_int, _dp, _dp, _ip, C.c_double, C.c_double, C.c_int]),
     "atvh_pair_compute": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "atvh_pme_configure": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int]),
+    "atvh_pme_compute": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "atvh_pme_info": (C.c_int, [_vp, _ip, _dp]),
     "atvh_energy": (C.c_int, [_vp, _dp, _dp, _dp]),
     "atvh_integrator_configure": (C.c_int, [_vp, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int64]),
     "atvh_step_vvnh": (C.c_int, [_vp, C.c_int]),

@@ -54,7 +54,11 @@ struct DevCtl {
     int always;       // constant 1: predicate for unconditional launches
     int halo_overflow; // bricks whose halo exceeded the LDS capacity
     long long nbr_total; // full-list entries of the last build
+    double e_kspace[3];  // last k-space energy parts: reciprocal, self, net charge
 };
+
+struct CoulState; // PairLJCutCoulLong tables (coul_pme.hip)
+struct PmeState;  // KSpacePME mesh, plans and tables (coul_pme.hip)
 
 struct TimerSlot {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
@@ -144,8 +148,14 @@ struct atvh_system {
     int* halo_n = nullptr;          // [nbricks] rows in each halo
     int2* brick_range = nullptr;    // [nbricks] (first slot, atom count)
 
-    // pair style
+    // pair style: 0 = PairLJCut, 1 = PairLJCutCoulLong
     bool pair_configured = false;
+    int pair_style = 0;
+    atvh::CoulState* coul = nullptr;
+    // k-space (KSpacePME), evaluated after the pair style in the force phase
+    atvh::PmeState* pme = nullptr;
+    // charge sums of the uploaded frame (ascending atom order)
+    double q_sum = 0, q_sq = 0, q_abs = 0;
     double lj_eps = 1, lj_sigma = 1, lj_cutoff = 0;
     int lj_ntypes = 0;
     std::vector<double> lj_eps_tab, lj_sigma_tab;
@@ -221,9 +231,17 @@ int pair_grid(const atvh_system* s);
 void pair_enqueue(atvh_system* s, int precision, bool eflag, bool vflag);
 void pair_free(atvh_system* s);
 
+// coul_pme.hip
+void ljc_enqueue(atvh_system* s, int precision, bool eflag, bool vflag);
+void pme_enqueue(atvh_system* s, int precision, bool ev);
+int pme_partial_blocks(const atvh_system* s);
+const double* pme_partials(const atvh_system* s);
+void pme_energy_consts(const atvh_system* s, double& escale, double& e_self, double& e_net);
+void coul_free(atvh_system* s);
+
 // integrate.hip
 void integ_enqueue_initial(atvh_system* s);
 void integ_enqueue_final(atvh_system* s);
-void energy_enqueue_finalize(atvh_system* s, double* record_pe, double* record_ke);
+void energy_enqueue_finalize(atvh_system* s, double* record_pe, int step, bool with_pair = true);
 
 } // namespace atvh

@@ -1,0 +1,1151 @@
+// coul_pme.hip — charged systems: the fused LJ + real-space Coulomb pair
+// style and smooth particle-mesh Ewald.
+//
+// PairLJCutCoulLong (src/cpu/potentials/pair_lj_cut_coul_long.cpp):
+//   per type pair lj1..lj4 = 48 eps s^12, 24 eps s^6, 4 eps s^12, 4 eps s^6 and
+//   the energy shift at the cutoff (:132-186, Lorentz-Berthelot for unset
+//   off-diagonals); per pair inside r_c (r2 != 0):
+//     r6inv = r2inv^3, fpair = r2inv (lj1 r6inv^2 - lj2 r6inv) + qqr2e qi qj f(r2),
+//     E = lj3 r6inv^2 - lj4 r6inv - shift + qqr2e qi qj e(r2),
+//   e, f linearly interpolated in r^2 over the ErfcTable (erfc_table.h:44-90,
+//   10000 bins, built on the host with libm erfc/exp exactly as the reference).
+//   Kernels: one workgroup per brick (same list and halo as the LJ kernels),
+//   full list, energy/virial halved. The table is stored per bin as
+//   {f_i, f_{i+1} - f_i, e_i, e_{i+1} - e_i}: one 32-byte (FP64) or 16-byte
+//   (FP32) load per pair, with the differences formed on the host exactly as
+//   the reference forms them per pair.
+//
+// KSpacePME (src/cpu/longrange/pme.cpp:147-253, pme_spline.h, pme_influence.h):
+//   1. spread   one thread per atom: fractional coordinate and base index in
+//               FP64 (the reference's floor/wrap), B-spline weights (Essmann
+//               recursion) in the compute precision, n^3 contributions added to
+//               a 64-bit fixed-point grid with integer atomics — exact and
+//               order-independent, so the grid is bitwise reproducible. The
+//               scale is 2^(61 - ceil(log2 sum|q|)), which no grid point can
+//               overflow (a point holds at most sum|q_i|).
+//   2. unfix    fixed point -> real grid (FP64 or FP32), clearing the
+//               fixed-point grid for the next call.
+//   3. hipFFT   R2C (D2Z / R2C), unnormalised.
+//   4. influence  H(m) (host-built table, FP64, half spectrum) times Q(m);
+//               energy and virial summed in FP64 over the half spectrum with
+//               the Hermitian partner of each interior mode folded in.
+//   5. hipFFT   C2R (Z2D / C2R).
+//   6. gather   one thread per atom: weights and derivatives again, n^3 reads,
+//               F = -(k_e / (pi V)) q sum_d g_d K_d / L_d, added to the forces.
+//   Self and net-charge energies are host constants of the uploaded charges.
+
+#include "atvh_internal.hpp"
+#include "brick.hpp"
+#include "pair_common.hpp"
+
+#include <hipfft/hipfft.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace atvh {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kEwaldF = 1.1283791670955126; // 2/sqrt(pi), erfc_table.h:24
+constexpr int kPmeMaxOrder = 8;
+constexpr int kPmeGridCap = 1024; // influence kernel blocks (energy partials)
+
+struct CoulState {
+    int ntypes = 0;
+    double cutoff = 0, alpha = 0;
+    int nbin = 0;
+    double inv_dr2 = 0;
+    double* tab = nullptr;   // [ntypes^2][8] lj1 lj2 lj3 lj4 shift 0 0 0
+    float* tab32 = nullptr;  // same, FP32
+    double4* erfc = nullptr; // [nbin] {f, df, e, de}
+    float4* erfc32 = nullptr;
+    std::vector<double> eps_mixed, sigma_mixed; // finalize_type_table_ result (host copy)
+};
+
+struct PmeState {
+    double alpha = 0, spacing = 0;
+    int order = 0;
+    int K[3] = {0, 0, 0};
+    int Kh = 0; // K2/2 + 1
+    double vol = 0, ic[3] = {0, 0, 0};
+    double fix_scale = 0;
+    unsigned long long* fix = nullptr;
+    double* real64 = nullptr;
+    double2* cplx64 = nullptr;
+    float* real32 = nullptr;
+    float2* cplx32 = nullptr;
+    double* H = nullptr; // [K0][K1][Kh]
+    double* partials = nullptr;
+    int nb = 0;
+    hipfftHandle f64 = nullptr, b64 = nullptr, f32 = nullptr, b32 = nullptr;
+};
+
+} // namespace atvh
+
+using namespace atvh;
+
+namespace {
+
+// ---------------------------------------------------------------- pair style
+
+struct LjcArgs {
+    const double *x, *y, *z, *q;
+    const int* type;
+    const int* halo_rows;
+    const int* halo_n;
+    const int2* brick_range;
+    const uint16_t* nbr16;
+    const uint16_t* self16;
+    const int* numneigh;
+    int cap;
+    double L[3], cs[3];
+    int nc[3], nb[3], bdim[3];
+    double rc_sq;
+    int ntypes;
+    const double* tab;
+    const float* tab32;
+    const double4* erfc;
+    const float4* erfc32;
+    int nbin;
+    double inv_dr2;
+    double *fx, *fy, *fz;
+    double* partials;
+    // small-grid form (global slots, per-pair minimum image)
+    const int* nbr;
+    int64_t npad;
+    double il[3];
+    int64_t n;
+};
+
+template <bool EF, bool VF>
+__global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
+#pragma clang fp contract(fast)
+    __shared__ double sx[kHaloCap], sy[kHaloCap], sz[kHaloCap], sq[kHaloCap];
+    __shared__ int stype[kHaloCap];
+    const int b = blockIdx.x;
+    const int nh = a.halo_n[b];
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (nh > kHaloCap) { // reported by the build; nothing valid to compute
+        if (EF || VF)
+            block_partials(acc, a.partials);
+        return;
+    }
+    const int* __restrict__ rows = a.halo_rows + (int64_t)b * kHaloCap;
+    for (int r = threadIdx.x; r < nh; r += blockDim.x) {
+        int gs;
+        double xx, yy, zz;
+        row_image(a.x, a.y, a.z, a.L[0], a.L[1], a.L[2], rows[r], gs, xx, yy, zz);
+        sx[r] = xx;
+        sy[r] = yy;
+        sz[r] = zz;
+        sq[r] = a.q[gs];
+        stype[r] = a.type[gs];
+    }
+    __syncthreads();
+    const int2 br = a.brick_range[b];
+    const int a0 = br.x, nat = br.y;
+    const double rc_sq = a.rc_sq, inv_dr2 = a.inv_dr2;
+    const int ntypes = a.ntypes, nbin1 = a.nbin - 1;
+    const double* __restrict__ tab = a.tab;
+    const double4* __restrict__ etab = a.erfc;
+    const uint4* __restrict__ base = reinterpret_cast<const uint4*>(a.nbr16 + (int64_t)a0 * a.cap);
+    for (int li = threadIdx.x; li < nat; li += blockDim.x) {
+        const int i = a0 + li;
+        const int qs = a.self16[i];
+        const double xi = sx[qs], yi = sy[qs], zi = sz[qs];
+        const double qi_scale = coulomb * sq[qs];
+        const double* __restrict__ trow = tab + stype[qs] * ntypes * 8;
+        const int nn = a.numneigh[i];
+        const uint4* __restrict__ grp = base + li;
+        const int ng = (nn + kGroup - 1) / kGroup;
+        double fxi = 0.0, fyi = 0.0, fzi = 0.0;
+        uint4 cur = grp[0]; // in bounds even when nn == 0: the brick's rows always exist
+        for (int g = 0; g < ng; ++g) {
+            const uint4 nxt = grp[(int64_t)min(g + 1, ng - 1) * nat];
+            const int rem = min(nn - g * kGroup, kGroup);
+            for (int u = 0; u < rem; ++u) {
+                const int q = group_entry(cur, u);
+                const double dx = xi - sx[q];
+                const double dy = yi - sy[q];
+                const double dz = zi - sz[q];
+                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                if (r2 < rc_sq && r2 != 0.0) {
+                    const double* __restrict__ t = trow + stype[q] * 8;
+                    const double r2inv = 1.0 / r2;
+                    const double r6inv = r2inv * r2inv * r2inv;
+                    const double xb = r2 * inv_dr2;
+                    const int bi = min(static_cast<int>(xb), nbin1);
+                    const double tt = xb - static_cast<double>(bi);
+                    const double4 ev = etab[bi];
+                    const double qq = qi_scale * sq[q];
+                    const double fpair = r2inv * (t[0] * r6inv * r6inv - t[1] * r6inv) + qq * (ev.x + tt * ev.y);
+                    fxi = fma(fpair, dx, fxi);
+                    fyi = fma(fpair, dy, fyi);
+                    fzi = fma(fpair, dz, fzi);
+                    if (EF)
+                        acc[0] += (t[2] * r6inv * r6inv - t[3] * r6inv - t[4]) + qq * (ev.z + tt * ev.w);
+                    if (VF) {
+                        const double fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
+                        acc[1] += dx * fxp;
+                        acc[2] += dy * fyp;
+                        acc[3] += dz * fzp;
+                        acc[4] += dx * fyp;
+                        acc[5] += dx * fzp;
+                        acc[6] += dy * fzp;
+                    }
+                }
+            }
+            cur = nxt;
+        }
+        a.fx[i] = fxi;
+        a.fy[i] = fyi;
+        a.fz[i] = fzi;
+    }
+    if (EF || VF) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            acc[k] *= 0.5; // full list: every pair seen from both ends
+        block_partials(acc, a.partials);
+    }
+}
+
+template <bool EF, bool VF>
+__global__ __launch_bounds__(kBlock) void k_ljc_brick_mixed(LjcArgs a) {
+    __shared__ float4 sp[kHaloCap]; // x, y, z relative to the brick centre; w = charge
+    __shared__ unsigned char stype[kHaloCap];
+    const int b = blockIdx.x;
+    const int nh = a.halo_n[b];
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (nh > kHaloCap) {
+        if (EF || VF)
+            block_partials(acc, a.partials);
+        return;
+    }
+    const int2 br = a.brick_range[b];
+    const int a0 = br.x, nat = br.y;
+    const int* __restrict__ rows = a.halo_rows + (int64_t)b * kHaloCap;
+    double r0x, r0y, r0z;
+    brick_centre(b, a.nb, a.bdim, a.nc, a.cs, r0x, r0y, r0z);
+    for (int r = threadIdx.x; r < nh; r += blockDim.x) {
+        int gs;
+        double xx, yy, zz;
+        row_image(a.x, a.y, a.z, a.L[0], a.L[1], a.L[2], rows[r], gs, xx, yy, zz);
+        float4 v;
+        v.x = static_cast<float>(xx - r0x);
+        v.y = static_cast<float>(yy - r0y);
+        v.z = static_cast<float>(zz - r0z);
+        v.w = static_cast<float>(a.q[gs]);
+        sp[r] = v;
+        stype[r] = static_cast<unsigned char>(a.type[gs]);
+    }
+    __syncthreads();
+    const float rc_sq = static_cast<float>(a.rc_sq), inv_dr2 = static_cast<float>(a.inv_dr2);
+    const int ntypes = a.ntypes, nbin1 = a.nbin - 1;
+    const float* __restrict__ tab = a.tab32;
+    const float4* __restrict__ etab = a.erfc32;
+    const uint4* __restrict__ base = reinterpret_cast<const uint4*>(a.nbr16 + (int64_t)a0 * a.cap);
+    for (int li = threadIdx.x; li < nat; li += blockDim.x) {
+        const int i = a0 + li;
+        const int qs = a.self16[i];
+        const float4 pi = sp[qs];
+        const float qi_scale = static_cast<float>(coulomb) * pi.w;
+        const float* __restrict__ trow = tab + stype[qs] * ntypes * 8;
+        const int nn = a.numneigh[i];
+        const uint4* __restrict__ grp = base + li;
+        const int ng = (nn + kGroup - 1) / kGroup;
+        float fxi = 0.f, fyi = 0.f, fzi = 0.f;
+        uint4 cur = grp[0];
+        for (int g = 0; g < ng; ++g) {
+            const uint4 nxt = grp[(int64_t)min(g + 1, ng - 1) * nat];
+            const int rem = min(nn - g * kGroup, kGroup);
+            for (int u = 0; u < rem; ++u) {
+                const int q = group_entry(cur, u);
+                const float4 pj = sp[q];
+                const float dx = pi.x - pj.x;
+                const float dy = pi.y - pj.y;
+                const float dz = pi.z - pj.z;
+                const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                if (r2 < rc_sq && r2 != 0.0f) {
+                    const float4 t = *reinterpret_cast<const float4*>(trow + stype[q] * 8);
+                    const float r2inv = __builtin_amdgcn_rcpf(r2);
+                    const float r6inv = r2inv * r2inv * r2inv;
+                    const float xb = r2 * inv_dr2;
+                    const int bi = min(static_cast<int>(xb), nbin1);
+                    const float tt = xb - static_cast<float>(bi);
+                    const float4 ev = etab[bi];
+                    const float qq = qi_scale * pj.w;
+                    const float fpair = r2inv * __builtin_fmaf(t.x * r6inv, r6inv, -t.y * r6inv) +
+                                        qq * __builtin_fmaf(tt, ev.y, ev.x);
+                    fxi = __builtin_fmaf(fpair, dx, fxi);
+                    fyi = __builtin_fmaf(fpair, dy, fyi);
+                    fzi = __builtin_fmaf(fpair, dz, fzi);
+                    if (EF) {
+                        const float sh = trow[stype[q] * 8 + 4];
+                        acc[0] += static_cast<double>(__builtin_fmaf(t.z * r6inv, r6inv, -t.w * r6inv) - sh) +
+                                  static_cast<double>(qq * __builtin_fmaf(tt, ev.w, ev.z));
+                    }
+                    if (VF) {
+                        const float fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
+                        acc[1] += static_cast<double>(dx * fxp);
+                        acc[2] += static_cast<double>(dy * fyp);
+                        acc[3] += static_cast<double>(dz * fzp);
+                        acc[4] += static_cast<double>(dx * fyp);
+                        acc[5] += static_cast<double>(dx * fzp);
+                        acc[6] += static_cast<double>(dy * fzp);
+                    }
+                }
+            }
+            cur = nxt;
+        }
+        a.fx[i] = static_cast<double>(fxi);
+        a.fy[i] = static_cast<double>(fyi);
+        a.fz[i] = static_cast<double>(fzi);
+    }
+    if (EF || VF) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            acc[k] *= 0.5;
+        block_partials(acc, a.partials);
+    }
+}
+
+// Small grids (< 2*reach+1 cells on an axis): global-slot ELL list with the
+// reference's per-pair minimum image d -= L nearbyint(d / L) (1/L hoisted).
+template <bool EF, bool VF, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_ljc_global(LjcArgs a) {
+#pragma clang fp contract(fast)
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    const int ntypes = a.ntypes, nbin1 = a.nbin - 1;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
+        const double qi_scale = coulomb * a.q[i];
+        const int ti = a.type[i];
+        const int nn = a.numneigh[i];
+        double fxi = 0.0, fyi = 0.0, fzi = 0.0;
+        for (int k = 0; k < nn; ++k) {
+            const int j = a.nbr[(int64_t)k * a.npad + i];
+            double dx = xi - a.x[j];
+            double dy = yi - a.y[j];
+            double dz = zi - a.z[j];
+            dx -= a.L[0] * rint(dx * a.il[0]);
+            dy -= a.L[1] * rint(dy * a.il[1]);
+            dz -= a.L[2] * rint(dz * a.il[2]);
+            const double r2 = dx * dx + dy * dy + dz * dz; // FP64 in both modes: same survivors
+            if (!(r2 < a.rc_sq && r2 != 0.0))
+                continue;
+            const int ab = (ti * ntypes + a.type[j]) * 8;
+            double fpair, epair = 0.0;
+            if (MIXED) {
+                const float r2f = static_cast<float>(r2);
+                const float r2inv = __builtin_amdgcn_rcpf(r2f);
+                const float r6inv = r2inv * r2inv * r2inv;
+                const float xb = r2f * static_cast<float>(a.inv_dr2);
+                const int bi = min(static_cast<int>(xb), nbin1);
+                const float tt = xb - static_cast<float>(bi);
+                const float4 ev = a.erfc32[bi];
+                const float qq = static_cast<float>(qi_scale) * static_cast<float>(a.q[j]);
+                const float* t = a.tab32 + ab;
+                fpair = static_cast<double>(r2inv * (t[0] * r6inv * r6inv - t[1] * r6inv) + qq * (ev.x + tt * ev.y));
+                if (EF)
+                    epair = static_cast<double>((t[2] * r6inv * r6inv - t[3] * r6inv - t[4]) + qq * (ev.z + tt * ev.w));
+            } else {
+                const double r2inv = 1.0 / r2;
+                const double r6inv = r2inv * r2inv * r2inv;
+                const double xb = r2 * a.inv_dr2;
+                const int bi = min(static_cast<int>(xb), nbin1);
+                const double tt = xb - static_cast<double>(bi);
+                const double4 ev = a.erfc[bi];
+                const double qq = qi_scale * a.q[j];
+                const double* t = a.tab + ab;
+                fpair = r2inv * (t[0] * r6inv * r6inv - t[1] * r6inv) + qq * (ev.x + tt * ev.y);
+                if (EF)
+                    epair = (t[2] * r6inv * r6inv - t[3] * r6inv - t[4]) + qq * (ev.z + tt * ev.w);
+            }
+            const double fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
+            fxi += fxp;
+            fyi += fyp;
+            fzi += fzp;
+            if (EF)
+                acc[0] += epair;
+            if (VF) {
+                acc[1] += dx * fxp;
+                acc[2] += dy * fyp;
+                acc[3] += dz * fzp;
+                acc[4] += dx * fyp;
+                acc[5] += dx * fzp;
+                acc[6] += dy * fzp;
+            }
+        }
+        a.fx[i] = fxi;
+        a.fy[i] = fyi;
+        a.fz[i] = fzi;
+    }
+    if (EF || VF) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            acc[k] *= 0.5;
+        block_partials(acc, a.partials);
+    }
+}
+
+// ---------------------------------------------------------------- PME
+
+struct PmeArgs {
+    int64_t n;
+    const double *x, *y, *z, *q;
+    double *fx, *fy, *fz;
+    int K0, K1, K2, Kh;
+    double ic0, ic1, ic2;
+    double fix_scale, inv_fix;
+    unsigned long long* fix;
+    double fscale;      // k_e / (pi V)
+    double pi2_a2;      // pi^2 / alpha^2
+    const double* H;
+    double* partials;
+};
+
+__device__ __forceinline__ int pme_wrap(int g, int K) {
+    // pme_spline.h:149-160: at most one period either side
+    if (g >= K) g -= K;
+    if (g < 0) g += K;
+    if (g >= K) g -= K;
+    if (g < 0) g += K;
+    return g;
+}
+
+// bspline (pme_spline.h:29-55) in precision T, order N.
+template <class T, int N>
+__device__ __forceinline__ void bspline_t(T frac, T (&th)[N], T (&dth)[N]) {
+    th[N - 1] = T(0);
+    th[1] = frac;
+    th[0] = T(1) - frac;
+#pragma unroll
+    for (int k = 3; k < N; ++k) {
+        const T div = T(1) / T(k - 1);
+        th[k - 1] = div * frac * th[k - 2];
+#pragma unroll
+        for (int j = 1; j < k - 1; ++j)
+            th[k - 1 - j] = div * ((frac + T(j)) * th[k - 2 - j] + (T(k - j) - frac) * th[k - 1 - j]);
+        th[0] = div * (T(1) - frac) * th[0];
+    }
+    dth[0] = -th[0];
+#pragma unroll
+    for (int j = 1; j < N; ++j)
+        dth[j] = th[j - 1] - th[j];
+    const T div = T(1) / T(N - 1);
+    th[N - 1] = div * frac * th[N - 2];
+#pragma unroll
+    for (int j = 1; j < N - 1; ++j)
+        th[N - 1 - j] = div * ((frac + T(j)) * th[N - 2 - j] + (T(N - j) - frac) * th[N - 1 - j]);
+    th[0] = div * (T(1) - frac) * th[0];
+}
+
+// pme_fill_atom_splines (pme_spline.h:131-146), one axis: u = (x/L) K wrapped
+// into [0, K) by u - floor(u/K) K, in FP64; the weights in precision T.
+template <class T, int N>
+__device__ __forceinline__ int axis_spline(double xv, double ic, int K, T (&th)[N], T (&dth)[N]) {
+    const double s = ic * xv;
+    double u = s * K;
+    u -= floor(u / K) * K;
+    // u lies in [0, K]; the clamp only keeps a non-finite coordinate in bounds
+    const int u0 = min(max(static_cast<int>(floor(u)), 0), K);
+    const double frac = u - u0;
+    bspline_t<T, N>(static_cast<T>(frac), th, dth);
+    return u0 - (N - 1);
+}
+
+template <class T, int N>
+__global__ __launch_bounds__(kBlock) void k_pme_spread(PmeArgs a) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double qi = a.q[i];
+        if (qi == 0.0)
+            continue;
+        T tx[N], ty[N], tz[N], dx[N], dy[N], dz[N];
+        const int b0 = axis_spline<T, N>(a.x[i], a.ic0, a.K0, tx, dx);
+        const int b1 = axis_spline<T, N>(a.y[i], a.ic1, a.K1, ty, dy);
+        const int b2 = axis_spline<T, N>(a.z[i], a.ic2, a.K2, tz, dz);
+        const T qt = static_cast<T>(qi);
+        int g2s[N];
+#pragma unroll
+        for (int c = 0; c < N; ++c)
+            g2s[c] = pme_wrap(b2 + c, a.K2);
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+            const int g0 = pme_wrap(b0 + p, a.K0);
+            const T wa = qt * tx[p];
+#pragma unroll
+            for (int r = 0; r < N; ++r) {
+                const int g1 = pme_wrap(b1 + r, a.K1);
+                const T wab = wa * ty[r];
+                unsigned long long* __restrict__ row =
+                    a.fix + (int64_t)a.K2 * (g1 + (int64_t)a.K1 * g0);
+#pragma unroll
+                for (int c = 0; c < N; ++c) {
+                    const double v = static_cast<double>(wab * tz[c]);
+                    const long long fx = static_cast<long long>(rint(v * a.fix_scale));
+                    atomicAdd(row + g2s[c], static_cast<unsigned long long>(fx));
+                }
+            }
+        }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pme_unfix(int64_t ntot, unsigned long long* fix, T* real,
+                                                      double inv_fix) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ntot;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const long long v = static_cast<long long>(fix[i]);
+        real[i] = static_cast<T>(static_cast<double>(v) * inv_fix);
+        fix[i] = 0ull;
+    }
+}
+
+// pme_apply_influence_tables (pme_influence.h:179-222) on the R2C half
+// spectrum. Modes with 0 < m2 < K2/2 stand for themselves and their Hermitian
+// partner (-m): energy and diagonal virial count twice; an off-diagonal term
+// m_a m_b of the partner changes sign unless one of the two indices is the
+// Nyquist plane of its axis (h = K/2 maps to itself), so it is folded in as
+// m_a m_b (1 + s_a s_b) with s = -1, or +1 on a Nyquist plane (s_z = -1).
+template <class T2, bool EV>
+__global__ __launch_bounds__(kBlock) void k_pme_influence(PmeArgs a, T2* __restrict__ cplx) {
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    const int64_t nmodes = (int64_t)a.K0 * a.K1 * a.Kh;
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < nmodes;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const double h = a.H[idx];
+        T2 c = cplx[idx];
+        if (EV && h != 0.0) {
+            const int m2 = static_cast<int>(idx % a.Kh);
+            const int64_t r = idx / a.Kh;
+            const int m1 = static_cast<int>(r % a.K1);
+            const int m0 = static_cast<int>(r / a.K1);
+            const int h0 = (m0 <= a.K0 / 2) ? m0 : m0 - a.K0;
+            const int h1 = (m1 <= a.K1 / 2) ? m1 : m1 - a.K1;
+            const int h2 = (m2 <= a.K2 / 2) ? m2 : m2 - a.K2;
+            const double mx = h0 * a.ic0, my = h1 * a.ic1, mz = h2 * a.ic2;
+            const double msq = mx * mx + my * my + mz * mz;
+            const double re = static_cast<double>(c.x), im = static_cast<double>(c.y);
+            const double q2 = re * re + im * im;
+            const double vt = -2.0 * (1.0 / msq + a.pi2_a2);
+            const double ug = h * q2;
+            const bool pair = m2 != 0 && 2 * m2 != a.K2;
+            const double w = pair ? 2.0 : 1.0;
+            const double sx = (2 * m0 == a.K0) ? 1.0 : -1.0;
+            const double sy = (2 * m1 == a.K1) ? 1.0 : -1.0;
+            const double oxy = pair ? 1.0 + sx * sy : 1.0;
+            const double oxz = pair ? 1.0 - sx : 1.0;
+            const double oyz = pair ? 1.0 - sy : 1.0;
+            acc[0] += w * ug;
+            acc[1] += w * ug * (1.0 + vt * mx * mx);
+            acc[2] += w * ug * (1.0 + vt * my * my);
+            acc[3] += w * ug * (1.0 + vt * mz * mz);
+            acc[4] += oxy * ug * (vt * mx * my);
+            acc[5] += oxz * ug * (vt * mx * mz);
+            acc[6] += oyz * ug * (vt * my * mz);
+        }
+        using TS = decltype(c.x);
+        c.x *= static_cast<TS>(h);
+        c.y *= static_cast<TS>(h);
+        cplx[idx] = c;
+    }
+    if (EV)
+        block_partials(acc, a.partials);
+}
+
+template <class T, int N>
+__global__ __launch_bounds__(kBlock) void k_pme_gather(PmeArgs a, const T* __restrict__ real) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double qi = a.q[i];
+        if (qi == 0.0)
+            continue;
+        T tx[N], ty[N], tz[N], dx[N], dy[N], dz[N];
+        const int b0 = axis_spline<T, N>(a.x[i], a.ic0, a.K0, tx, dx);
+        const int b1 = axis_spline<T, N>(a.y[i], a.ic1, a.K1, ty, dy);
+        const int b2 = axis_spline<T, N>(a.z[i], a.ic2, a.K2, tz, dz);
+        int g2s[N];
+#pragma unroll
+        for (int c = 0; c < N; ++c)
+            g2s[c] = pme_wrap(b2 + c, a.K2);
+        T gx = T(0), gy = T(0), gz = T(0);
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+            const int g0 = pme_wrap(b0 + p, a.K0);
+#pragma unroll
+            for (int r = 0; r < N; ++r) {
+                const int g1 = pme_wrap(b1 + r, a.K1);
+                const T* __restrict__ row = real + (int64_t)a.K2 * (g1 + (int64_t)a.K1 * g0);
+#pragma unroll
+                for (int c = 0; c < N; ++c) {
+                    const T cval = row[g2s[c]];
+                    gx += cval * dx[p] * ty[r] * tz[c];
+                    gy += cval * tx[p] * dy[r] * tz[c];
+                    gz += cval * tx[p] * ty[r] * dz[c];
+                }
+            }
+        }
+        const double fxi = static_cast<double>(gx) * a.K0 * a.ic0;
+        const double fyi = static_cast<double>(gy) * a.K1 * a.ic1;
+        const double fzi = static_cast<double>(gz) * a.K2 * a.ic2;
+        const double pre = -a.fscale * qi;
+        a.fx[i] += pre * fxi;
+        a.fy[i] += pre * fyi;
+        a.fz[i] += pre * fzi;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero3(int64_t n, double* a, double* b, double* c) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        a[i] = 0.0;
+        b[i] = 0.0;
+        c[i] = 0.0;
+    }
+}
+
+#define LJC_DISPATCH2(KERNEL, EF, VF, GRID, ST, ARGS)                                           \
+    do {                                                                                       \
+        if (EF) {                                                                              \
+            if (VF) KERNEL<true, true><<<GRID, kBlock, 0, ST>>>(ARGS);                         \
+            else KERNEL<true, false><<<GRID, kBlock, 0, ST>>>(ARGS);                           \
+        } else {                                                                               \
+            if (VF) KERNEL<false, true><<<GRID, kBlock, 0, ST>>>(ARGS);                        \
+            else KERNEL<false, false><<<GRID, kBlock, 0, ST>>>(ARGS);                          \
+        }                                                                                      \
+    } while (0)
+
+#define LJC_DISPATCH3(KERNEL, EF, VF, M, GRID, ST, ARGS)                                        \
+    do {                                                                                       \
+        if (M) {                                                                               \
+            if (EF) {                                                                          \
+                if (VF) KERNEL<true, true, true><<<GRID, kBlock, 0, ST>>>(ARGS);               \
+                else KERNEL<true, false, true><<<GRID, kBlock, 0, ST>>>(ARGS);                 \
+            } else {                                                                           \
+                if (VF) KERNEL<false, true, true><<<GRID, kBlock, 0, ST>>>(ARGS);              \
+                else KERNEL<false, false, true><<<GRID, kBlock, 0, ST>>>(ARGS);                \
+            }                                                                                  \
+        } else {                                                                               \
+            if (EF) {                                                                          \
+                if (VF) KERNEL<true, true, false><<<GRID, kBlock, 0, ST>>>(ARGS);              \
+                else KERNEL<true, false, false><<<GRID, kBlock, 0, ST>>>(ARGS);                \
+            } else {                                                                           \
+                if (VF) KERNEL<false, true, false><<<GRID, kBlock, 0, ST>>>(ARGS);             \
+                else KERNEL<false, false, false><<<GRID, kBlock, 0, ST>>>(ARGS);               \
+            }                                                                                  \
+        }                                                                                      \
+    } while (0)
+
+template <class T, int N>
+void launch_spread_gather(bool spread, int grid, hipStream_t st, const PmeArgs& a, const T* real) {
+    if (spread)
+        k_pme_spread<T, N><<<grid, kBlock, 0, st>>>(a);
+    else
+        k_pme_gather<T, N><<<grid, kBlock, 0, st>>>(a, real);
+}
+
+template <class T>
+void dispatch_order(int order, bool spread, int grid, hipStream_t st, const PmeArgs& a, const T* real) {
+    switch (order) {
+    case 3: launch_spread_gather<T, 3>(spread, grid, st, a, real); break;
+    case 4: launch_spread_gather<T, 4>(spread, grid, st, a, real); break;
+    case 5: launch_spread_gather<T, 5>(spread, grid, st, a, real); break;
+    case 6: launch_spread_gather<T, 6>(spread, grid, st, a, real); break;
+    case 7: launch_spread_gather<T, 7>(spread, grid, st, a, real); break;
+    case 8: launch_spread_gather<T, 8>(spread, grid, st, a, real); break;
+    default: throw Error(ATVH_E_ARG, "KSpacePME: order must be in [3, 8]");
+    }
+}
+
+#define HIPFFT_CHECK(call)                                                                 \
+    do {                                                                                   \
+        hipfftResult r_ = (call);                                                          \
+        if (r_ != HIPFFT_SUCCESS)                                                          \
+            throw Error(ATVH_E_HIP, std::string(#call) + ": hipFFT error " + std::to_string(int(r_))); \
+    } while (0)
+
+// Host restatements of the mesh helpers (pme_spline.h:63-126).
+int good_fft_size(int n) {
+    auto ok = [](int m) {
+        for (int p : {2, 3, 5, 7})
+            while (m % p == 0)
+                m /= p;
+        return m == 1;
+    };
+    if (n < 4)
+        n = 4;
+    while (!ok(n))
+        ++n;
+    return n;
+}
+
+void bspline_host(double frac, int n, double* theta, double* dtheta) {
+    theta[n - 1] = 0.0;
+    theta[1] = frac;
+    theta[0] = 1.0 - frac;
+    for (int k = 3; k < n; ++k) {
+        const double div = 1.0 / (k - 1);
+        theta[k - 1] = div * frac * theta[k - 2];
+        for (int j = 1; j < k - 1; ++j)
+            theta[k - 1 - j] = div * ((frac + j) * theta[k - 2 - j] + (k - j - frac) * theta[k - 1 - j]);
+        theta[0] = div * (1.0 - frac) * theta[0];
+    }
+    dtheta[0] = -theta[0];
+    for (int j = 1; j < n; ++j)
+        dtheta[j] = theta[j - 1] - theta[j];
+    const double div = 1.0 / (n - 1);
+    theta[n - 1] = div * frac * theta[n - 2];
+    for (int j = 1; j < n - 1; ++j)
+        theta[n - 1 - j] = div * ((frac + j) * theta[n - 2 - j] + (n - j - frac) * theta[n - 1 - j]);
+    theta[0] = div * (1.0 - frac) * theta[0];
+}
+
+std::vector<double> spline_moduli(int K, int n) {
+    std::vector<double> data(n, 0.0), dd(n, 0.0), mod(K, 0.0);
+    bspline_host(0.0, n, data.data(), dd.data());
+    for (int i = 0; i < K; ++i) {
+        double sr = 0, si = 0;
+        for (int k = 0; k < n - 1; ++k) {
+            const double arg = 2.0 * kPi * i * k / K;
+            sr += data[k] * std::cos(arg);
+            si += data[k] * std::sin(arg);
+        }
+        const double denom = sr * sr + si * si;
+        mod[i] = denom > 1e-12 ? 1.0 / denom : 0.0;
+    }
+    for (int i = 0; i < K; ++i)
+        if (mod[i] == 0.0)
+            mod[i] = 0.5 * (mod[(i - 1 + K) % K] + mod[(i + 1) % K]);
+    return mod;
+}
+
+void pme_release(PmeState* p) {
+    if (!p)
+        return;
+    if (p->f64) hipfftDestroy(p->f64);
+    if (p->b64) hipfftDestroy(p->b64);
+    if (p->f32) hipfftDestroy(p->f32);
+    if (p->b32) hipfftDestroy(p->b32);
+    dev_free(p->fix);
+    dev_free(p->real64);
+    dev_free(p->cplx64);
+    dev_free(p->real32);
+    dev_free(p->cplx32);
+    dev_free(p->H);
+    dev_free(p->partials);
+    delete p;
+}
+
+void coul_release(CoulState* c) {
+    if (!c)
+        return;
+    dev_free(c->tab);
+    dev_free(c->tab32);
+    dev_free(c->erfc);
+    dev_free(c->erfc32);
+    delete c;
+}
+
+PmeArgs make_pme_args(atvh_system* s) {
+    PmeState* p = s->pme;
+    PmeArgs a;
+    a.n = s->n;
+    a.x = s->x;
+    a.y = s->y;
+    a.z = s->z;
+    a.q = s->q;
+    a.fx = s->fx;
+    a.fy = s->fy;
+    a.fz = s->fz;
+    a.K0 = p->K[0];
+    a.K1 = p->K[1];
+    a.K2 = p->K[2];
+    a.Kh = p->Kh;
+    a.ic0 = p->ic[0];
+    a.ic1 = p->ic[1];
+    a.ic2 = p->ic[2];
+    // fixed-point scale: a grid point holds at most sum|q_i| (weights <= 1)
+    const double qabs = std::max(s->q_abs, 1.0);
+    p->fix_scale = std::ldexp(1.0, 61 - static_cast<int>(std::ceil(std::log2(qabs))));
+    a.fix_scale = p->fix_scale;
+    a.inv_fix = 1.0 / p->fix_scale;
+    a.fix = p->fix;
+    a.fscale = coulomb / (kPi * p->vol);
+    a.pi2_a2 = kPi * kPi / (p->alpha * p->alpha);
+    a.H = p->H;
+    a.partials = p->partials;
+    return a;
+}
+
+} // namespace
+
+namespace atvh {
+
+void ljc_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
+    CoulState* c = s->coul;
+    LjcArgs a{};
+    a.x = s->x;
+    a.y = s->y;
+    a.z = s->z;
+    a.q = s->q;
+    a.type = s->type;
+    a.halo_rows = s->halo_rows;
+    a.halo_n = s->halo_n;
+    a.brick_range = s->brick_range;
+    a.nbr16 = s->nbr16;
+    a.self16 = s->self16;
+    a.numneigh = s->numneigh;
+    a.cap = s->cap;
+    for (int d = 0; d < 3; ++d) {
+        a.L[d] = s->box[d];
+        a.cs[d] = s->cs[d];
+        a.nc[d] = s->nc[d];
+        a.nb[d] = s->nb[d];
+        a.bdim[d] = s->bdim[d];
+        a.il[d] = 1.0 / s->box[d];
+    }
+    a.rc_sq = c->cutoff * c->cutoff;
+    a.ntypes = c->ntypes;
+    a.tab = c->tab;
+    a.tab32 = c->tab32;
+    a.erfc = c->erfc;
+    a.erfc32 = c->erfc32;
+    a.nbin = c->nbin;
+    a.inv_dr2 = c->inv_dr2;
+    a.fx = s->fx;
+    a.fy = s->fy;
+    a.fz = s->fz;
+    a.partials = s->partials;
+    a.nbr = s->nbr;
+    a.npad = s->npad;
+    a.n = s->n;
+    hipStream_t st = s->ctx->stream;
+    const int g = pair_grid(s);
+    if (s->brick_mode) {
+        if (precision == ATVH_PREC_MIXED)
+            LJC_DISPATCH2(k_ljc_brick_mixed, eflag, vflag, g, st, a);
+        else
+            LJC_DISPATCH2(k_ljc_brick_fp64, eflag, vflag, g, st, a);
+    } else {
+        LJC_DISPATCH3(k_ljc_global, eflag, vflag, precision == ATVH_PREC_MIXED, g, st, a);
+    }
+    ATVH_HIP(hipGetLastError());
+}
+
+void pme_enqueue(atvh_system* s, int precision, bool ev) {
+    PmeState* p = s->pme;
+    if (!s->has_charge)
+        throw Error(ATVH_E_STATE, "KSpacePME: the frame carries no charges");
+    ScopedTimer tm(s->ctx, "pme");
+    hipStream_t st = s->ctx->stream;
+    const bool mixed = precision == ATVH_PREC_MIXED;
+    const int64_t ntot = (int64_t)p->K[0] * p->K[1] * p->K[2];
+    const int64_t nmodes = (int64_t)p->K[0] * p->K[1] * p->Kh;
+    // buffers and plans of this precision, made on first use
+    if (mixed && !p->real32) {
+        dev_alloc(&p->real32, ntot);
+        dev_alloc(&p->cplx32, nmodes);
+        HIPFFT_CHECK(hipfftPlan3d(&p->f32, p->K[0], p->K[1], p->K[2], HIPFFT_R2C));
+        HIPFFT_CHECK(hipfftPlan3d(&p->b32, p->K[0], p->K[1], p->K[2], HIPFFT_C2R));
+    }
+    if (!mixed && !p->real64) {
+        dev_alloc(&p->real64, ntot);
+        dev_alloc(&p->cplx64, nmodes);
+        HIPFFT_CHECK(hipfftPlan3d(&p->f64, p->K[0], p->K[1], p->K[2], HIPFFT_D2Z));
+        HIPFFT_CHECK(hipfftPlan3d(&p->b64, p->K[0], p->K[1], p->K[2], HIPFFT_Z2D));
+    }
+    PmeArgs a = make_pme_args(s);
+    const int ga = grid_capped(s->n);
+    const int gg = grid_capped(ntot);
+    const int gm = std::min(grid_for(nmodes), kPmeGridCap);
+    if (mixed) {
+        dispatch_order<float>(p->order, true, ga, st, a, nullptr);
+        k_pme_unfix<float><<<gg, kBlock, 0, st>>>(ntot, p->fix, p->real32, a.inv_fix);
+        HIPFFT_CHECK(hipfftSetStream(p->f32, st));
+        HIPFFT_CHECK(hipfftExecR2C(p->f32, p->real32, reinterpret_cast<hipfftComplex*>(p->cplx32)));
+        if (ev)
+            k_pme_influence<float2, true><<<gm, kBlock, 0, st>>>(a, p->cplx32);
+        else
+            k_pme_influence<float2, false><<<gm, kBlock, 0, st>>>(a, p->cplx32);
+        HIPFFT_CHECK(hipfftSetStream(p->b32, st));
+        HIPFFT_CHECK(hipfftExecC2R(p->b32, reinterpret_cast<hipfftComplex*>(p->cplx32), p->real32));
+        dispatch_order<float>(p->order, false, ga, st, a, p->real32);
+    } else {
+        dispatch_order<double>(p->order, true, ga, st, a, nullptr);
+        k_pme_unfix<double><<<gg, kBlock, 0, st>>>(ntot, p->fix, p->real64, a.inv_fix);
+        HIPFFT_CHECK(hipfftSetStream(p->f64, st));
+        HIPFFT_CHECK(hipfftExecD2Z(p->f64, p->real64, reinterpret_cast<hipfftDoubleComplex*>(p->cplx64)));
+        if (ev)
+            k_pme_influence<double2, true><<<gm, kBlock, 0, st>>>(a, p->cplx64);
+        else
+            k_pme_influence<double2, false><<<gm, kBlock, 0, st>>>(a, p->cplx64);
+        HIPFFT_CHECK(hipfftSetStream(p->b64, st));
+        HIPFFT_CHECK(hipfftExecZ2D(p->b64, reinterpret_cast<hipfftDoubleComplex*>(p->cplx64), p->real64));
+        dispatch_order<double>(p->order, false, ga, st, a, p->real64);
+    }
+    p->nb = gm;
+    ATVH_HIP(hipGetLastError());
+}
+
+int pme_partial_blocks(const atvh_system* s) { return s->pme ? s->pme->nb : 0; }
+const double* pme_partials(const atvh_system* s) { return s->pme ? s->pme->partials : nullptr; }
+
+// escale = k_e / (2 pi V) (pme_influence.h:215); self and net-charge energies
+// (pme.cpp:222-246) from the charge sums of the uploaded frame.
+void pme_energy_consts(const atvh_system* s, double& escale, double& e_self, double& e_net) {
+    const PmeState* p = s->pme;
+    escale = coulomb / (2.0 * kPi * p->vol);
+    e_self = -coulomb * p->alpha / std::sqrt(kPi) * s->q_sq;
+    e_net = -coulomb * kPi / (2.0 * p->vol * p->alpha * p->alpha) * s->q_sum * s->q_sum;
+}
+
+void coul_free(atvh_system* s) {
+    coul_release(s->coul);
+    s->coul = nullptr;
+    pme_release(s->pme);
+    s->pme = nullptr;
+}
+
+} // namespace atvh
+
+#define API_TRY try {
+#define API_CATCH(C)                                                                    \
+    }                                                                                   \
+    catch (const atvh::Error& e_) {                                                     \
+        if (C)                                                                          \
+            (C)->err = e_.what();                                                       \
+        return e_.code;                                                                 \
+    }                                                                                   \
+    catch (const std::exception& e_) {                                                  \
+        if (C)                                                                          \
+            (C)->err = e_.what();                                                       \
+        return ATVH_E_ARG;                                                              \
+    }
+
+extern "C" {
+
+int atvh_pair_ljcoul_configure(atvh_system* s, int ntypes, const double* eps_tab, const double* sigma_tab,
+                               const int* set_tab, double alpha, double cutoff, int nbin) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    // constructor checks (pair_lj_cut_coul_long.cpp:34-44)
+    if (ntypes < 1 || ntypes > kMaxTypes || !eps_tab || !sigma_tab || !set_tab)
+        throw Error(ATVH_E_ARG, "PairLJCutCoulLong: empty or oversized coeff table");
+    if (!(alpha > 0.0))
+        throw Error(ATVH_E_ARG, "PairLJCutCoulLong: alpha must be > 0");
+    if (!(cutoff > 0.0))
+        throw Error(ATVH_E_ARG, "PairLJCutCoulLong: cutoff must be > 0");
+    if (nbin < 16)
+        throw Error(ATVH_E_ARG, "ErfcTable::build: bad alpha/cut/nbin");
+    for (int ab = 0; ab < ntypes * ntypes; ++ab)
+        if (set_tab[ab] && (!(sigma_tab[ab] > 0.0) || !(eps_tab[ab] >= 0.0)))
+            throw Error(ATVH_E_ARG, "PairLJCutCoulLong: bad epsilon/sigma");
+    ATVH_HIP(hipSetDevice(c->device));
+    auto* cs = new CoulState();
+    try {
+        cs->ntypes = ntypes;
+        cs->alpha = alpha;
+        cs->cutoff = cutoff;
+        cs->nbin = nbin;
+        // finalize_type_table_ (pair_lj_cut_coul_long.cpp:132-186)
+        std::vector<double> eps(eps_tab, eps_tab + ntypes * ntypes), sig(sigma_tab, sigma_tab + ntypes * ntypes);
+        std::vector<double> tab(size_t(8) * ntypes * ntypes, 0.0);
+        std::vector<float> tab32(tab.size(), 0.0f);
+        for (int ta = 0; ta < ntypes; ++ta)
+            for (int tb = 0; tb < ntypes; ++tb) {
+                const int ab = ta * ntypes + tb;
+                double e = eps[ab], sg = sig[ab];
+                if (!set_tab[ab]) {
+                    const int aa = ta * ntypes + ta, bb = tb * ntypes + tb;
+                    if (set_tab[aa] && set_tab[bb]) {
+                        e = std::sqrt(eps_tab[aa] * eps_tab[bb]);
+                        sg = 0.5 * (sigma_tab[aa] + sigma_tab[bb]);
+                    } else {
+                        e = 0.0;
+                        sg = 1.0;
+                    }
+                    eps[ab] = e;
+                    sig[ab] = sg;
+                }
+                const double s2 = sg * sg;
+                const double s6 = s2 * s2 * s2;
+                const double s12 = s6 * s6;
+                double* t = &tab[size_t(8) * ab];
+                t[0] = 48.0 * e * s12;
+                t[1] = 24.0 * e * s6;
+                t[2] = 4.0 * e * s12;
+                t[3] = 4.0 * e * s6;
+                if (e > 0.0) {
+                    const double rc2 = cutoff * cutoff;
+                    const double r6 = rc2 * rc2 * rc2;
+                    const double r6inv = 1.0 / r6;
+                    t[4] = t[2] * r6inv * r6inv - t[3] * r6inv;
+                }
+                for (int k = 0; k < 8; ++k)
+                    tab32[size_t(8) * ab + k] = static_cast<float>(t[k]);
+            }
+        cs->eps_mixed = eps;
+        cs->sigma_mixed = sig;
+        // ErfcTable::build (erfc_table.h:44-72)
+        const double cutoff_sq = cutoff * cutoff;
+        const double dr2 = cutoff_sq / static_cast<double>(nbin);
+        cs->inv_dr2 = 1.0 / dr2;
+        std::vector<double> et(nbin + 1), ft(nbin + 1);
+        for (int i = 1; i <= nbin; ++i) {
+            const double r2 = i * dr2;
+            const double r = std::sqrt(r2);
+            const double grij = alpha * r;
+            const double expm2 = std::exp(-grij * grij);
+            const double erfcv = std::erfc(grij);
+            et[i] = erfcv / r;
+            ft[i] = (erfcv + kEwaldF * grij * expm2) / (r2 * r);
+        }
+        et[0] = et[1];
+        ft[0] = ft[1];
+        std::vector<double4> e4(nbin);
+        std::vector<float4> e4f(nbin);
+        for (int i = 0; i < nbin; ++i) {
+            e4[i] = make_double4(ft[i], ft[i + 1] - ft[i], et[i], et[i + 1] - et[i]);
+            e4f[i] = make_float4(static_cast<float>(e4[i].x), static_cast<float>(e4[i].y),
+                                 static_cast<float>(e4[i].z), static_cast<float>(e4[i].w));
+        }
+        dev_alloc(&cs->tab, tab.size());
+        dev_alloc(&cs->tab32, tab32.size());
+        dev_alloc(&cs->erfc, e4.size());
+        dev_alloc(&cs->erfc32, e4f.size());
+        ATVH_HIP(hipMemcpy(cs->tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+        ATVH_HIP(hipMemcpy(cs->tab32, tab32.data(), tab32.size() * sizeof(float), hipMemcpyHostToDevice));
+        ATVH_HIP(hipMemcpy(cs->erfc, e4.data(), e4.size() * sizeof(double4), hipMemcpyHostToDevice));
+        ATVH_HIP(hipMemcpy(cs->erfc32, e4f.data(), e4f.size() * sizeof(float4), hipMemcpyHostToDevice));
+    } catch (...) {
+        coul_release(cs);
+        throw;
+    }
+    coul_release(s->coul);
+    s->coul = cs;
+    s->pair_style = 1;
+    s->lj_cutoff = cutoff;
+    s->pair_configured = true;
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+int atvh_pme_configure(atvh_system* s, double alpha, double spacing, int order) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    // constructor checks (pme.cpp:31-39), plus this backend's order limit
+    if (!(alpha > 0.0))
+        throw Error(ATVH_E_ARG, "KSpacePME: alpha must be > 0");
+    if (!(spacing > 0.0))
+        throw Error(ATVH_E_ARG, "KSpacePME: grid_spacing must be > 0");
+    if (order < 3)
+        throw Error(ATVH_E_ARG, "KSpacePME: order must be >= 3");
+    if (order > kPmeMaxOrder)
+        throw Error(ATVH_E_ARG, "KSpacePME: this backend supports spline orders up to 8");
+    ATVH_HIP(hipSetDevice(c->device));
+    auto* p = new PmeState();
+    try {
+        p->alpha = alpha;
+        p->spacing = spacing;
+        p->order = order;
+        // ensure_grid (pme.cpp:66-74)
+        for (int d = 0; d < 3; ++d) {
+            p->K[d] = good_fft_size(static_cast<int>(std::ceil(s->box[d] / spacing)));
+            if (p->K[d] < order + 1)
+                p->K[d] = good_fft_size(order + 1);
+            p->ic[d] = 1.0 / s->box[d];
+        }
+        p->Kh = p->K[2] / 2 + 1;
+        p->vol = s->box[0] * s->box[1] * s->box[2];
+        const int64_t ntot = (int64_t)p->K[0] * p->K[1] * p->K[2];
+        const int64_t nmodes = (int64_t)p->K[0] * p->K[1] * p->Kh;
+        // pme_build_influence_tables (pme_influence.h:115-175) on the half spectrum
+        std::vector<double> mod[3];
+        for (int d = 0; d < 3; ++d)
+            mod[d] = spline_moduli(p->K[d], order);
+        const double inv_pi2_a2 = kPi * kPi / (alpha * alpha);
+        std::vector<double> H(nmodes, 0.0);
+        for (int m0 = 0; m0 < p->K[0]; ++m0) {
+            const int h0 = (m0 <= p->K[0] / 2) ? m0 : m0 - p->K[0];
+            const double bm0 = mod[0][m0];
+            for (int m1 = 0; m1 < p->K[1]; ++m1) {
+                const int h1 = (m1 <= p->K[1] / 2) ? m1 : m1 - p->K[1];
+                const double bm01 = bm0 * mod[1][m1];
+                for (int m2 = 0; m2 < p->Kh; ++m2) {
+                    if (m0 == 0 && m1 == 0 && m2 == 0)
+                        continue;
+                    const int h2 = (m2 <= p->K[2] / 2) ? m2 : m2 - p->K[2];
+                    const double mx = h0 * p->ic[0], my = h1 * p->ic[1], mz = h2 * p->ic[2];
+                    const double msq = mx * mx + my * my + mz * mz;
+                    const double inv_msq = 1.0 / msq;
+                    const double B = bm01 * mod[2][m2];
+                    H[m2 + (int64_t)p->Kh * (m1 + (int64_t)p->K[1] * m0)] =
+                        std::exp(-inv_pi2_a2 * msq) * inv_msq * B;
+                }
+            }
+        }
+        dev_alloc(&p->H, H.size());
+        ATVH_HIP(hipMemcpy(p->H, H.data(), H.size() * sizeof(double), hipMemcpyHostToDevice));
+        dev_alloc(&p->fix, ntot);
+        ATVH_HIP(hipMemset(p->fix, 0, ntot * sizeof(unsigned long long)));
+        dev_alloc(&p->partials, size_t(kPmeGridCap) * kPartialStride);
+        ATVH_HIP(hipMemset(p->partials, 0, size_t(kPmeGridCap) * kPartialStride * sizeof(double)));
+    } catch (...) {
+        pme_release(p);
+        throw;
+    }
+    pme_release(s->pme);
+    s->pme = p;
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+int atvh_pme_info(atvh_system* s, int* K, double* e_parts) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    if (!s->pme)
+        throw Error(ATVH_E_STATE, "KSpacePME not configured");
+    if (K)
+        for (int d = 0; d < 3; ++d)
+            K[d] = s->pme->K[d];
+    if (e_parts) {
+        ATVH_HIP(hipStreamSynchronize(c->stream));
+        DevCtl h;
+        ATVH_HIP(hipMemcpy(&h, s->ctl, sizeof(h), hipMemcpyDeviceToHost));
+        for (int k = 0; k < 3; ++k)
+            e_parts[k] = h.e_kspace[k];
+    }
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+int atvh_pme_compute(atvh_system* s, int precision, int eflag, int vflag) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    if (precision != ATVH_PREC_FP64 && precision != ATVH_PREC_MIXED)
+        throw Error(ATVH_E_ARG, "atvh_pme_compute: unknown precision");
+    if (!s->pme)
+        throw Error(ATVH_E_STATE, "KSpacePME not configured");
+    ATVH_HIP(hipSetDevice(c->device));
+    k_zero3<<<grid_capped(s->n), kBlock, 0, c->stream>>>(s->n, s->fx, s->fy, s->fz);
+    ATVH_HIP(hipGetLastError());
+    const bool ev = eflag || vflag;
+    pme_enqueue(s, precision, ev);
+    if (ev)
+        energy_enqueue_finalize(s, nullptr, 0, false);
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+} // extern "C"

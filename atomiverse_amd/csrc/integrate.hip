@@ -203,17 +203,43 @@ __global__ __launch_bounds__(kBlock) void k_post(PostArgs a) {
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_energy_finalize(DevCtl* ctl, const double* partials,
-                                                            int nb, double* record_pe, int step) {
-    double v[7];
-    for (int k = 0; k < 7; ++k)
-        v[k] = block_fixed_sum(partials + k, nb, kPartialStride);
+struct FinalizeArgs {
+    DevCtl* ctl;
+    const double* partials; // pair style, per workgroup
+    int nb;
+    const double* kpartials; // k-space influence, per workgroup (kspace only)
+    int knb;
+    double escale, e_self, e_net;
+    double* record_pe;
+    int step;
+};
+
+// Fixed-order totals of the force phase: pair energy and virial, then the
+// k-space terms in the reference's order (self first, then reciprocal + net
+// charge; pme.cpp:222-246).
+__global__ __launch_bounds__(kBlock) void k_energy_finalize(FinalizeArgs a) {
+    double v[7], k[7];
+    for (int c = 0; c < 7; ++c)
+        v[c] = a.nb > 0 ? block_fixed_sum(a.partials + c, a.nb, kPartialStride) : 0.0;
+    for (int c = 0; c < 7; ++c)
+        k[c] = a.knb > 0 ? block_fixed_sum(a.kpartials + c, a.knb, kPartialStride) : 0.0;
     if (threadIdx.x == 0) {
-        ctl->pe = v[0];
-        for (int k = 0; k < 6; ++k)
-            ctl->vir[k] = v[k + 1];
-        if (record_pe)
-            record_pe[step] = v[0];
+        double pe = v[0];
+        if (a.knb > 0) {
+            const double e_recip = k[0] * a.escale;
+            pe += a.e_self;
+            pe += e_recip + a.e_net;
+            for (int c = 0; c < 6; ++c)
+                v[c + 1] += k[c + 1] * a.escale;
+            a.ctl->e_kspace[0] = e_recip;
+            a.ctl->e_kspace[1] = a.e_self;
+            a.ctl->e_kspace[2] = a.e_net;
+        }
+        a.ctl->pe = pe;
+        for (int c = 0; c < 6; ++c)
+            a.ctl->vir[c] = v[c + 1];
+        if (a.record_pe)
+            a.record_pe[a.step] = pe;
     }
 }
 
@@ -295,10 +321,19 @@ void integ_enqueue_final(atvh_system* s) {
     ATVH_HIP(hipGetLastError());
 }
 
-void energy_enqueue_finalize(atvh_system* s, double* record_pe, double* record_ke) {
-    (void)record_ke;
-    k_energy_finalize<<<1, kBlock, 0, s->ctx->stream>>>(s->ctl, s->partials, pair_grid(s),
-                                                        record_pe, 0);
+void energy_enqueue_finalize(atvh_system* s, double* record_pe, int step, bool with_pair) {
+    FinalizeArgs a;
+    a.ctl = s->ctl;
+    a.partials = s->partials;
+    a.nb = with_pair ? pair_grid(s) : 0;
+    a.kpartials = pme_partials(s);
+    a.knb = pme_partial_blocks(s);
+    a.escale = a.e_self = a.e_net = 0.0;
+    if (a.knb > 0)
+        pme_energy_consts(s, a.escale, a.e_self, a.e_net);
+    a.record_pe = record_pe;
+    a.step = step;
+    k_energy_finalize<<<1, kBlock, 0, s->ctx->stream>>>(a);
     ATVH_HIP(hipGetLastError());
 }
 
@@ -417,8 +452,7 @@ int atvh_md_run(atvh_system* s, int nsteps, int precision, int energy_every) {
         const bool ef = (step == nsteps - 1) || (energy_every > 0 && (step % energy_every) == 0);
         pair_enqueue(s, precision, ef, ef);
         if (ef)
-            k_energy_finalize<<<1, kBlock, 0, st>>>(s->ctl, s->partials, pair_grid(s), rec_pe,
-                                                    step);
+            energy_enqueue_finalize(s, rec_pe, step);
         {
             ScopedTimer tm(c, "integrate");
             k_final<<<g, kBlock, 0, st>>>(s->n, ia);

@@ -24,6 +24,7 @@
 
 #include "atvh_internal.hpp"
 #include "brick.hpp"
+#include "pair_common.hpp"
 
 #include <cmath>
 
@@ -47,33 +48,6 @@ struct LJArgs {
     double *fx, *fy, *fz;
     double* partials;
 };
-
-template <class T>
-__device__ __forceinline__ T wave_sum(T v) {
-    for (int off = 32; off > 0; off >>= 1)
-        v += __shfl_down(v, off, 64);
-    return v;
-}
-
-// Fixed-order block reduction of 7 doubles into partials[block*8 + k].
-__device__ __forceinline__ void block_partials(double (&acc)[7], double* partials) {
-    __shared__ double sh[kBlock / 64][7];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < 7; ++k)
-        acc[k] = wave_sum(acc[k]);
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < 7; ++k)
-            sh[wid][k] = acc[k];
-    __syncthreads();
-    if (threadIdx.x < 7) {
-        double t = 0.0;
-        for (int w = 0; w < kBlock / 64; ++w)
-            t += sh[w][threadIdx.x];
-        partials[blockIdx.x * kPartialStride + threadIdx.x] = t;
-    }
-}
 
 template <bool EF, bool VF, bool TYPED>
 __global__ __launch_bounds__(kBlock) void k_lj_fp64(int64_t n, LJArgs a) {
@@ -222,18 +196,6 @@ struct BrickLJArgs {
     double* partials;
 };
 
-// Image position of a halo row: the global slot's position plus the periodic
-// shift recorded at the build (positions are not re-wrapped between builds).
-__device__ __forceinline__ void row_image(const double* __restrict__ x, const double* __restrict__ y,
-                                          const double* __restrict__ z, double lx, double ly, double lz,
-                                          int row, int& gs, double& xx, double& yy, double& zz) {
-    gs = row & kRowSlotMask;
-    const int code = static_cast<unsigned>(row) >> kRowSlotBits;
-    xx = x[gs] + shift_of(code, 0, lx);
-    yy = y[gs] + shift_of(code, 1, ly);
-    zz = z[gs] + shift_of(code, 2, lz);
-}
-
 template <bool EF, bool VF, bool TYPED>
 __global__ __launch_bounds__(kBlock) void k_lj_brick_fp64(BrickLJArgs a) {
     __shared__ double sx[kHaloCap], sy[kHaloCap], sz[kHaloCap];
@@ -356,11 +318,8 @@ __global__ __launch_bounds__(kBlock) void k_lj_brick_mixed(BrickLJArgs a) {
     // reference point: the brick centre (every halo row lies within the brick
     // widened by the stencil reach, so |x - centre| < ~7 Å and the FP32
     // rounding of the relative coordinate is <= 2.4e-7 Å)
-    const int bz = b % a.nb[2], by = (b / a.nb[2]) % a.nb[1], bx = b / (a.nb[1] * a.nb[2]);
-    const int ox = bx * a.bdim[0], oy = by * a.bdim[1], oz = bz * a.bdim[2];
-    const double r0x = (ox + 0.5 * min(a.bdim[0], a.nc[0] - ox)) * a.cs[0];
-    const double r0y = (oy + 0.5 * min(a.bdim[1], a.nc[1] - oy)) * a.cs[1];
-    const double r0z = (oz + 0.5 * min(a.bdim[2], a.nc[2] - oz)) * a.cs[2];
+    double r0x, r0y, r0z;
+    brick_centre(b, a.nb, a.bdim, a.nc, a.cs, r0x, r0y, r0z);
     for (int q = threadIdx.x; q < nh; q += blockDim.x) {
         int gs;
         double xx, yy, zz;
@@ -499,14 +458,30 @@ namespace atvh {
 
 int pair_grid(const atvh_system* s) { return s->brick_mode ? s->nbricks : grid_capped(s->n); }
 
+static void lj_enqueue(atvh_system* s, int precision, bool eflag, bool vflag);
+
+// The force phase: the pair style (PairLJCut or PairLJCutCoulLong) writes the
+// forces, then KSpacePME adds its part when configured (the reference's
+// evaluator order at Phase::force).
 void pair_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
     if (!s->pair_configured)
         throw Error(ATVH_E_STATE, "pair style not configured");
     if (!s->nl_built)
-        throw Error(ATVH_E_STATE, "PairLJCut: no neighbor list (add a NeighborList and build it)");
+        throw Error(ATVH_E_STATE, "pair style: no neighbor list (add a NeighborList and build it)");
     if (s->lj_cutoff > s->cutoff + 1e-12)
-        throw Error(ATVH_E_ARG, "PairLJCut: pair cutoff exceeds the neighbor-list cutoff");
-    ScopedTimer tm(s->ctx, "pair");
+        throw Error(ATVH_E_ARG, "pair style: pair cutoff exceeds the neighbor-list cutoff");
+    {
+        ScopedTimer tm(s->ctx, "pair");
+        if (s->pair_style == 1)
+            ljc_enqueue(s, precision, eflag, vflag);
+        else
+            lj_enqueue(s, precision, eflag, vflag);
+    }
+    if (s->pme)
+        pme_enqueue(s, precision, eflag || vflag);
+}
+
+static void lj_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
     LJArgs a;
     a.x = s->x;
     a.y = s->y;
@@ -621,6 +596,7 @@ int atvh_pair_lj_configure(atvh_system* s, double eps, double sigma, double cuto
     s->lj_sigma = sigma;
     s->lj_cutoff = cutoff;
     s->lj_ntypes = ntypes;
+    s->pair_style = 0;
     dev_free(s->d_ljtab);
     if (ntypes > 0) {
         // finalize_type_table_ (pair_lj_cut.cpp:257-298): sigma^6 and the
@@ -656,7 +632,7 @@ int atvh_pair_compute(atvh_system* s, int precision, int eflag, int vflag) {
     ATVH_HIP(hipSetDevice(c->device));
     pair_enqueue(s, precision, eflag != 0, vflag != 0);
     if (eflag || vflag)
-        energy_enqueue_finalize(s, nullptr, nullptr);
+        energy_enqueue_finalize(s, nullptr, 0);
     return ATVH_OK;
     API_CATCH(c)
 }

@@ -162,6 +162,29 @@ class System:
             nt = int(round(np.sqrt(et.size)))
             self._ck(self.lib.atvh_pair_lj_configure(self.h, eps, sigma, cutoff, nt, dptr(et), dptr(st)))
 
+    def pair_ljcoul(self, eps_tab, sigma_tab, set_tab, alpha, cutoff, nbin=10000):
+        """PairLJCutCoulLong: ntypes x ntypes coefficient tables (set_tab marks given entries)."""
+        et = np.ascontiguousarray(eps_tab, dtype=np.float64).reshape(-1)
+        st = np.ascontiguousarray(sigma_tab, dtype=np.float64).reshape(-1)
+        mt = np.ascontiguousarray(set_tab, dtype=np.int32).reshape(-1)
+        nt = int(round(np.sqrt(et.size)))
+        self._ck(self.lib.atvh_pair_ljcoul_configure(self.h, nt, dptr(et), dptr(st), iptr(mt), alpha, cutoff,
+                                                     int(nbin)))
+
+    def pme(self, alpha, spacing, order):
+        """KSpacePME(alpha, grid_spacing, order)."""
+        self._ck(self.lib.atvh_pme_configure(self.h, alpha, spacing, int(order)))
+
+    def pme_compute(self, precision=_abi.PREC_FP64, eflag=True, vflag=True):
+        self._ck(self.lib.atvh_pme_compute(self.h, precision, 1 if eflag else 0, 1 if vflag else 0))
+
+    def pme_info(self):
+        """(K[3], (e_recip, e_self, e_net) of the last k-space energy evaluation)."""
+        K = np.zeros(3, np.int32)
+        parts = np.zeros(3)
+        self._ck(self.lib.atvh_pme_info(self.h, iptr(K), dptr(parts)))
+        return K, tuple(parts)
+
     def compute(self, precision=_abi.PREC_FP64, eflag=True, vflag=True):
         self._ck(self.lib.atvh_pair_compute(self.h, precision, 1 if eflag else 0, 1 if vflag else 0))
 

@@ -49,6 +49,14 @@ class LJWorkload:
     check: int = 1
     type_: np.ndarray | None = None
     charge: np.ndarray | None = None
+    # charged systems: PairLJCutCoulLong tables and KSpacePME
+    pair: str = "lj"
+    eps_tab: np.ndarray | None = None
+    sigma_tab: np.ndarray | None = None
+    set_tab: np.ndarray | None = None
+    alpha: float = 0.0
+    pme_spacing: float = 0.0
+    pme_order: int = 0
     meta: dict = field(default_factory=dict)
 
     @property
@@ -108,12 +116,48 @@ def lj_c1(thermostat=1):
     return lj_fcc(80, seed=87288, thermostat=thermostat, name="lj_fcc_2048000_nvt")
 
 
+def ewald_alpha(cutoff, accuracy=1e-5):
+    """EwaldSum::recommend_alpha (ewald_sum.h:58-60)."""
+    return (1.35 - 0.15 * math.log(accuracy)) / cutoff
+
+
+def charged_melt(nside, density=0.9, T_star=1.2, seed=87289, thermostat=1, cutoff=3.0, skin=0.3,
+                 accuracy=1e-5, pme_spacing=0.65, pme_order=4, name=None):
+    """Charged LJ binary melt: nside^3 simple-cubic sites (rock salt), alternating
+    +-1 reduced charges, species A (sigma 1.0) and B (sigma 0.8), eps 1,
+    Lorentz-Berthelot for A-B, unit masses, seeded MB velocities."""
+    a = (1.0 / density) ** (1.0 / 3.0)
+    x, y, z, t, q = rocksalt_lattice(nside, nside, nside, a, 1.0 / math.sqrt(QQR2E))
+    n = x.size
+    L = nside * a
+    mass = np.ones(n)
+    T = T_star / KB
+    vx, vy, vz = init_velocity(mass, T, seed)
+    return LJWorkload(name or f"charged_melt_{n}", x, y, z, vx, vy, vz, mass, np.array([L, L, L]),
+                      cutoff=cutoff, skin=skin, T=T, thermostat=thermostat, type_=t, charge=q, pair="ljcoul",
+                      eps_tab=np.array([[1.0, 0.0], [0.0, 1.0]]), sigma_tab=np.array([[1.0, 0.0], [0.0, 0.8]]),
+                      set_tab=np.array([[1, 0], [0, 1]], np.int32), alpha=ewald_alpha(cutoff, accuracy),
+                      pme_spacing=pme_spacing, pme_order=pme_order,
+                      meta=dict(nside=nside, density=density, T_star=T_star, seed=seed, accuracy=accuracy))
+
+
+def lj_c2():
+    """512,000-ion charged LJ melt (80^3 rock salt, rho*=0.9, T*=1.2, seed 87289), NVT
+    tau*=0.5, r_c 3.0 + skin 0.3, Ewald tolerance 1e-5, PME 128^3 order 4."""
+    return charged_melt(80, name="charged_melt_512000_nvt")
+
+
 def upload(sysobj, w: LJWorkload):
     sysobj.upload(w.x, w.y, w.z, w.mass, w.vx, w.vy, w.vz, type_=w.type_, charge=w.charge)
 
 
 def configure(sysobj, w: LJWorkload):
-    """NeighborList + PairLJCut + VelocityVerlet(+NoseHoover), as the configs state."""
+    """NeighborList + PairLJCut (or PairLJCutCoulLong + KSpacePME) + VelocityVerlet(+NoseHoover)."""
     sysobj.nlist_configure(w.cutoff, w.skin, w.every, w.delay, bool(w.check))
-    sysobj.pair_lj(w.eps, w.sigma, w.cutoff)
+    if w.pair == "ljcoul":
+        sysobj.pair_ljcoul(w.eps_tab, w.sigma_tab, w.set_tab, w.alpha, w.cutoff)
+        if w.pme_order > 0:
+            sysobj.pme(w.alpha, w.pme_spacing, w.pme_order)
+    else:
+        sysobj.pair_lj(w.eps, w.sigma, w.cutoff)
     sysobj.integrator(w.dt, w.thermostat, w.T, w.tau_T, 0)

@@ -117,9 +117,36 @@ int atvh_nlist_export_pairs(atvh_system* sys, int64_t* pairs, int64_t cap,
 int atvh_pair_lj_configure(atvh_system* sys, double eps, double sigma,
                            double cutoff, int ntypes, const double* eps_tab,
                            const double* sigma_tab);
-/* One force evaluation (ZeroForces folded in: forces are overwritten).
+/* PairLJCutCoulLong(coeffs, alpha, cutoff)
+ * (src/cpu/potentials/pair_lj_cut_coul_long.cpp:34-70, compute :276-571;
+ * CUDA counterpart cuda::CoulLong, src/cuda/potentials/coul_long.cu).
+ * The coefficient table is ntypes x ntypes, row-major: entries with
+ * set_tab != 0 are given (both orders, as the constructor's `put` stores
+ * them), the rest are Lorentz-Berthelot mixed from set diagonals or zero.
+ * Real-space Coulomb from the ErfcTable of nbin bins (10000 in the
+ * reference). Replaces any PairLJCut of the system. */
+int atvh_pair_ljcoul_configure(atvh_system* sys, int ntypes,
+                               const double* eps_tab, const double* sigma_tab,
+                               const int* set_tab, double alpha, double cutoff,
+                               int nbin);
+/* One force-phase evaluation: the pair style overwrites the forces, then the
+ * k-space solver (if configured) adds its forces (ZeroForces folded in).
  * eflag/vflag request the energy and 6-component virial. */
 int atvh_pair_compute(atvh_system* sys, int precision, int eflag, int vflag);
+
+/* ---- k-space -----------------------------------------------------------------------
+ * KSpacePME(alpha, grid_spacing, order) (src/cpu/longrange/pme.cpp:31-39,
+ * compute :147-253; CUDA counterpart cuda::KSpacePME,
+ * src/cuda/longrange/pme.cu). Orders 3..8. Mesh K_d = good_fft_size(
+ * ceil(L_d / spacing)) (pme.cpp:66-74). Needs per-atom charges in the frame.
+ * Once configured it runs in every force phase after the pair style. */
+int atvh_pme_configure(atvh_system* sys, double alpha, double grid_spacing,
+                       int order);
+/* KSpacePME alone into zeroed forces; energy/virial are the k-space terms. */
+int atvh_pme_compute(atvh_system* sys, int precision, int eflag, int vflag);
+/* Mesh size K[3] (may be null) and the last k-space energy parts
+ * e_parts[3] = {reciprocal, self, net charge} (may be null; synchronises). */
+int atvh_pme_info(atvh_system* sys, int* K, double* e_parts);
 /* Last computed potential energy (eV) and Voigt virial (eV); synchronises. */
 int atvh_energy(atvh_system* sys, double* pe, double* virial6, double* ke);
 
