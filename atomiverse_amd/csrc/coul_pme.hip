@@ -786,6 +786,8 @@ namespace atvh {
 
 void ljc_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
     CoulState* c = s->coul;
+    if (s->ntypes > c->ntypes)
+        throw Error(ATVH_E_ARG, "PairLJCutCoulLong: the frame has atom types beyond the coefficient table");
     LjcArgs a{};
     a.x = s->x;
     a.y = s->y;

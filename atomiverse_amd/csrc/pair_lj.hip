@@ -482,6 +482,8 @@ void pair_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
 }
 
 static void lj_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
+    if (s->lj_ntypes > 0 && s->ntypes > s->lj_ntypes)
+        throw Error(ATVH_E_ARG, "PairLJCut: the frame has atom types beyond the coefficient table");
     LJArgs a;
     a.x = s->x;
     a.y = s->y;

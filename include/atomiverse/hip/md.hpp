@@ -235,6 +235,119 @@ class PairLJCut : public Component {
     bool stress_ = false;
 };
 
+/// One coefficient row of PairLJCutCoulLong (types, epsilon [eV], sigma [Å]).
+struct LJTypeCoeff {
+    int type_i = 0, type_j = 0;
+    double epsilon = 0.0, sigma = 1.0;
+};
+
+/// PairLJCutCoulLong(coeffs, alpha, cutoff): per-type shifted LJ + real-space
+/// Ewald Coulomb from the erfc table (src/cpu/potentials/
+/// pair_lj_cut_coul_long.cpp:34-70); unset off-diagonal pairs are
+/// Lorentz-Berthelot mixed from set diagonals. With a KSpacePME in the plan
+/// its reciprocal part runs in the same force-phase call.
+class PairLJCutCoulLong : public Component {
+  public:
+    PairLJCutCoulLong(std::vector<LJTypeCoeff> coeffs, double alpha, double cutoff)
+        : coeffs_(std::move(coeffs)), alpha_(alpha), cutoff_(cutoff) {
+        if (coeffs_.empty())
+            throw std::invalid_argument("PairLJCutCoulLong: empty coeff table");
+        if (!(alpha > 0.0))
+            throw std::invalid_argument("PairLJCutCoulLong: alpha must be > 0");
+        if (!(cutoff > 0.0))
+            throw std::invalid_argument("PairLJCutCoulLong: cutoff must be > 0");
+        for (const auto& c : coeffs_) {
+            if (c.type_i < 0 || c.type_j < 0)
+                throw std::invalid_argument("PairLJCutCoulLong: negative type");
+            if (!(c.sigma > 0.0) || !(c.epsilon >= 0.0))
+                throw std::invalid_argument("PairLJCutCoulLong: bad epsilon/sigma");
+            ntypes_ = std::max(ntypes_, std::max(c.type_i, c.type_j) + 1);
+        }
+    }
+    std::string name() const override { return "lj/cut/coul/long"; }
+    bool provides_force() const override { return true; }
+    void set_stress_required(bool v) override { stress_ = v; }
+    void build(const BuildContext& c) override {
+        const size_t n2 = size_t(ntypes_) * ntypes_;
+        std::vector<double> eps(n2, 0.0), sig(n2, 0.0);
+        std::vector<int> set(n2, 0);
+        for (const auto& cf : coeffs_)
+            for (auto [a, b] : {std::pair{cf.type_i, cf.type_j}, std::pair{cf.type_j, cf.type_i}}) {
+                eps[size_t(a) * ntypes_ + b] = cf.epsilon;
+                sig[size_t(a) * ntypes_ + b] = cf.sigma;
+                set[size_t(a) * ntypes_ + b] = 1;
+            }
+        check(atvh_pair_ljcoul_configure(c.runtime.sys(), ntypes_, eps.data(), sig.data(), set.data(), alpha_,
+                                         cutoff_, 10000),
+              c.runtime.ctx(), "PairLJCutCoulLong::build");
+    }
+    void launch(const LaunchContext& c) override {
+        const int ef = c.runtime.energy_this_step ? 1 : 0;
+        const int vf = (stress_ || c.runtime.stress_required) && ef ? 1 : 0;
+        check(atvh_pair_compute(c.runtime.sys(), static_cast<int>(c.runtime.precision), ef, vf), c.runtime.ctx(),
+              "PairLJCutCoulLong::launch");
+    }
+    double pair_cutoff() const { return cutoff_; }
+    double alpha() const { return alpha_; }
+    int ntypes() const { return ntypes_; }
+
+  private:
+    std::vector<LJTypeCoeff> coeffs_;
+    double alpha_, cutoff_;
+    int ntypes_ = 0;
+    bool stress_ = false;
+};
+
+/// CoulLong(alpha, cutoff): real-space Ewald Coulomb alone
+/// (src/cpu/potentials/coul_long.cpp), here the LJ-free PairLJCutCoulLong of
+/// one type (the erfc table stands in for the closed form, within the
+/// table's interpolation error).
+class CoulLong : public PairLJCutCoulLong {
+  public:
+    CoulLong(double alpha, double cutoff) : PairLJCutCoulLong({LJTypeCoeff{0, 0, 0.0, 1.0}}, alpha, cutoff) {}
+    std::string name() const override { return "coul/long"; }
+};
+
+/// KSpacePME(alpha, grid_spacing, order) (src/cpu/longrange/pme.cpp:31-253).
+/// With a pair style in the plan the k-space part runs inside that style's
+/// force-phase call, after it (the reference's evaluator order); alone it
+/// evaluates the reciprocal sum into zeroed forces.
+class KSpacePME : public Component {
+  public:
+    KSpacePME(double alpha, double grid_spacing, int order = 5)
+        : alpha_(alpha), spacing_(grid_spacing), order_(order) {
+        if (alpha <= 0.0)
+            throw std::invalid_argument("KSpacePME: alpha must be > 0");
+        if (grid_spacing <= 0.0)
+            throw std::invalid_argument("KSpacePME: grid_spacing must be > 0");
+        if (order < 3)
+            throw std::invalid_argument("KSpacePME: order must be >= 3");
+    }
+    std::string name() const override { return "pme"; }
+    bool provides_force() const override { return true; }
+    void set_stress_required(bool v) override { stress_ = v; }
+    void build(const BuildContext& c) override {
+        check(atvh_pme_configure(c.runtime.sys(), alpha_, spacing_, order_), c.runtime.ctx(), "KSpacePME::build");
+    }
+    void launch(const LaunchContext& c) override {
+        if (!standalone_)
+            return;
+        const int ef = c.runtime.energy_this_step ? 1 : 0;
+        const int vf = (stress_ || c.runtime.stress_required) && ef ? 1 : 0;
+        check(atvh_pme_compute(c.runtime.sys(), static_cast<int>(c.runtime.precision), ef, vf), c.runtime.ctx(),
+              "KSpacePME::launch");
+    }
+    void set_standalone(bool v) { standalone_ = v; }
+    double alpha() const { return alpha_; }
+    int order() const { return order_; }
+
+  private:
+    double alpha_, spacing_;
+    int order_;
+    bool stress_ = false;
+    bool standalone_ = false;
+};
+
 namespace md {
 
 /// VelocityVerlet(dt): initial_integrate and final_integrate at order 0.
@@ -390,14 +503,30 @@ class MD {
         }
         if (nh && vv)
             vv->couple_thermostat(nh->T_target(), nh->tau());
+        // k-space runs inside a pair style's force call when one is present
+        bool has_pair = false;
+        for (auto& op : ops_)
+            has_pair |= dynamic_cast<PairLJCut*>(op.comp) != nullptr ||
+                        dynamic_cast<PairLJCutCoulLong*>(op.comp) != nullptr;
+        for (auto& op : ops_)
+            if (auto* k = dynamic_cast<KSpacePME*>(op.comp))
+                k->set_standalone(!has_pair);
         for (auto& op : ops_)
             op.comp->set_stress_required(rt_->stress_required);
         fused_ = fused_allowed_;
         for (auto& op : ops_) {
             const bool known = dynamic_cast<VelocityVerlet*>(op.comp) || dynamic_cast<NoseHoover*>(op.comp) ||
                                dynamic_cast<NeighborList*>(op.comp) || dynamic_cast<PairLJCut*>(op.comp) ||
-                               dynamic_cast<KineticEnergy*>(op.comp);
+                               dynamic_cast<PairLJCutCoulLong*>(op.comp) || dynamic_cast<KineticEnergy*>(op.comp) ||
+                               dynamic_cast<KSpacePME*>(op.comp);
             fused_ = fused_ && known;
+        }
+        if (fused_) { // the fused loop evaluates forces through a pair style
+            bool pair = false;
+            for (auto& op : ops_)
+                pair |= dynamic_cast<PairLJCut*>(op.comp) != nullptr ||
+                        dynamic_cast<PairLJCutCoulLong*>(op.comp) != nullptr;
+            fused_ = pair;
         }
         built_ = true;
     }

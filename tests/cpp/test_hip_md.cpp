@@ -44,6 +44,44 @@ static void run(hip::HostFrame& f, bool fused, int steps, double& e_first, doubl
     e_last = f.energy + f.kinetic;
 }
 
+// Charged binary melt (rock salt 10^3, rho* = 0.9, T* = 1.2, +-1 reduced
+// charges, sigma_A 1.0 / sigma_B 0.8) with PairLJCutCoulLong + KSpacePME under
+// Nose-Hoover, configured as the reference's MD would be.
+static hip::HostFrame make_melt() {
+    const double a = std::cbrt(1.0 / 0.9);
+    const int nc = 10;
+    hip::HostFrame f;
+    const int64_t n = atvh_util_rocksalt_lattice(nc, nc, nc, a, nullptr, nullptr, nullptr, nullptr, nullptr, 0.0);
+    f.resize(static_cast<size_t>(n));
+    f.charge.assign(static_cast<size_t>(n), 0.0);
+    atvh_util_rocksalt_lattice(nc, nc, nc, a, f.x.data(), f.y.data(), f.z.data(), f.type.data(), f.charge.data(),
+                               1.0 / std::sqrt(14.399645));
+    f.box = {nc * a, nc * a, nc * a};
+    const double kB = 8.617333262e-5;
+    atvh_util_init_velocity(n, f.mass.data(), 1.2 / kB, 87289u, 3 * n - 3, f.vx.data(), f.vy.data(), f.vz.data());
+    return f;
+}
+
+static void run_melt(hip::HostFrame& f, bool fused, int steps) {
+    const double tau = std::sqrt(103.6427);
+    const double dt = 0.005 * tau;
+    const double alpha = (1.35 - 0.15 * std::log(1e-5)) / 3.0;
+    hip::md::MD sim;
+    sim.set_neighbor(hip::NeighborList{3.0, 0.3, 1, 0, true});
+    sim.set_evaluator(hip::PairLJCutCoulLong{{{0, 0, 1.0, 1.0}, {1, 1, 1.0, 0.8}}, alpha, 3.0});
+    sim.set_evaluator(hip::KSpacePME{alpha, 0.65, 4});
+    sim.set_integrator(hip::md::VelocityVerlet{dt});
+    sim.set_thermostat(hip::md::NoseHoover{1.2 / 8.617333262e-5, 0.5 * tau, dt});
+    sim.set_precision(hip::Precision::mixed);
+    sim.set_fused(fused);
+    sim.build();
+    if (sim.fused() != fused) {
+        std::fprintf(stderr, "unexpected fused mode (charged)\n");
+        std::exit(2);
+    }
+    sim.run(f, steps);
+}
+
 int main() {
     hip::HostFrame a = make_frame(), b = make_frame();
     double a0, a1, b0, b1;
@@ -60,6 +98,21 @@ int main() {
     std::printf("hip::md::MD: %zu atoms, E0/N %.6f, |dE|/N %.3e, per-phase == fused\n", n, a0 / n, drift);
     if (!(drift < 1e-2) || !std::isfinite(a1))
         return 1;
+    hip::HostFrame c = make_melt(), d = make_melt();
+    run_melt(c, true, 60);
+    run_melt(d, false, 60);
+    for (size_t i = 0; i < c.nlocal(); ++i) {
+        if (c.x[i] != d.x[i] || c.vz[i] != d.vz[i] || c.fy[i] != d.fy[i]) {
+            std::fprintf(stderr, "charged: fused and per-phase runs differ at atom %zu\n", i);
+            return 1;
+        }
+    }
+    if (!std::isfinite(c.energy) || !(c.energy < 0.0)) {
+        std::fprintf(stderr, "charged: bad energy %g\n", c.energy);
+        return 1;
+    }
+    std::printf("hip::md::MD charged melt: %zu ions, PE/N %.6f, per-phase == fused (PME)\n", c.nlocal(),
+                c.energy / c.nlocal());
     // a pair evaluator without a neighbor list is rejected, as cuda::md::MD does
     try {
         hip::md::MD bad;

@@ -18,3 +18,4 @@ def test_hip_md_cpp_consumer():
     print(out.stdout, out.stderr)
     assert out.returncode == 0, out.stderr
     assert "per-phase == fused" in out.stdout
+    assert "per-phase == fused (PME)" in out.stdout
