@@ -58,6 +58,10 @@ SIGNATURES = {
     "atvh_thermostat_state": (C.c_int, [_vp, _dp, _dp]),
     "atvh_md_run": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "atvh_md_thermo": (C.c_int, [_vp, _dp, _dp, C.c_int]),
+    "atvh_batch_create": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.POINTER(_vp)]),
+    "atvh_batch_destroy": (C.c_int, [_vp]),
+    "atvh_batch_set_graph": (C.c_int, [_vp, C.c_int]),
+    "atvh_batch_run": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "atvh_util_fcc_lattice": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int64, _dp, _dp, _dp]),
     "atvh_util_rocksalt_lattice": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _ip,
                                                _dp, C.c_double]),

@@ -55,6 +55,7 @@ struct DevCtl {
     int halo_overflow; // bricks whose halo exceeded the LDS capacity
     long long nbr_total; // full-list entries of the last build
     double e_kspace[3];  // last k-space energy parts: reciprocal, self, net charge
+    int rec_step;        // next thermo record index of a batched run
 };
 
 struct CoulState; // PairLJCutCoulLong tables (coul_pme.hip)
@@ -179,6 +180,26 @@ struct atvh_system {
     int thermo_n = 0;
 };
 
+// One stage of a batched launch: a device array of per-system argument
+// blocks and the prefix of their block counts (nsys + 1 entries).
+struct BatchStage {
+    void* args = nullptr;
+    int* base = nullptr;
+    int nblocks = 0;
+};
+
+struct atvh_batch {
+    atvh_ctx* ctx = nullptr;
+    std::vector<atvh_system*> sys;
+    std::map<std::string, BatchStage> stages;
+    atvh::DevCtl** d_ctls = nullptr; // member control blocks
+    bool typed = false;
+    bool use_graph = false;
+    hipGraphExec_t graph[2] = {nullptr, nullptr}; // [energy step?]
+    int graph_prec[2] = {-1, -1};
+    int thermo_cap = 0;
+};
+
 namespace atvh {
 
 inline int grid_for(int64_t n, int block = kBlock) {
@@ -219,6 +240,7 @@ void dev_free(T*& p) {
 }
 
 // neighbor.hip
+constexpr int kNlStages = 10; // kernels of one brick-mode rebuild
 void nl_setup_grid(atvh_system* s);
 void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces);
 void nl_enqueue_decide(atvh_system* s);
@@ -239,9 +261,43 @@ const double* pme_partials(const atvh_system* s);
 void pme_energy_consts(const atvh_system* s, double& escale, double& e_self, double& e_net);
 void coul_free(atvh_system* s);
 
+// batched launches (batch.hip + the module builders)
+template <class Args>
+void batch_stage(atvh_batch* b, const std::string& name, const std::vector<Args>& args,
+                 const std::vector<int>& blocks);
+void batch_free_stages(atvh_batch* b);
+void integ_batch_build(atvh_batch* b);
+void integ_batch_launch(atvh_batch* b, const char* stage);
+void nl_batch_build(atvh_batch* b);
+void nl_batch_launch_rebuild(atvh_batch* b);
+void pair_batch_build(atvh_batch* b);
+void pair_batch_launch(atvh_batch* b, int precision, bool ef);
+
 // integrate.hip
 void integ_enqueue_initial(atvh_system* s);
 void integ_enqueue_final(atvh_system* s);
 void energy_enqueue_finalize(atvh_system* s, double* record_pe, int step, bool with_pair = true);
+
+} // namespace atvh
+
+namespace atvh {
+
+template <class Args>
+void batch_stage(atvh_batch* b, const std::string& name, const std::vector<Args>& args,
+                 const std::vector<int>& blocks) {
+    BatchStage& st = b->stages[name];
+    dev_free(st.args);
+    dev_free(st.base);
+    std::vector<int> base(blocks.size() + 1, 0);
+    for (size_t k = 0; k < blocks.size(); ++k)
+        base[k + 1] = base[k] + blocks[k];
+    Args* d = nullptr;
+    dev_alloc(&d, args.size());
+    st.args = d;
+    dev_alloc(&st.base, base.size());
+    ATVH_HIP(hipMemcpy(d, args.data(), args.size() * sizeof(Args), hipMemcpyHostToDevice));
+    ATVH_HIP(hipMemcpy(st.base, base.data(), base.size() * sizeof(int), hipMemcpyHostToDevice));
+    st.nblocks = base.back();
+}
 
 } // namespace atvh

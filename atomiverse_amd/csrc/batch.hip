@@ -1,0 +1,224 @@
+// batch.hip — many independent systems advanced by one launch per stage.
+//
+// The step of every member is the single-system step (integrate.hip's fused
+// loop) with each stage launched once over all members: a member owns a
+// contiguous range of the launch's blocks and runs exactly its own
+// single-system code on its own buffers (batch.hpp), so a member's trajectory
+// is bit-identical alone and in any batch. The per-step sequence — kick/drift,
+// NH + decide, the ten predicated rebuild stages, pair forces, energy totals,
+// final kick, NH + KE — is 16 or 17 launches whatever the number of members,
+// and can be captured once into a hipGraph and replayed (no memset nodes: the
+// step clears its counters with kernels).
+//
+// Reference counterpart: replicas advanced one after another
+// (Remd::propagate_, src/cpu/path/remd.cpp:98-103), each a cpu::md::MD.
+
+#include "atvh_internal.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+using namespace atvh;
+
+namespace {
+
+__global__ void k_batch_reset(DevCtl** ctls, int nsys) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nsys)
+        ctls[k]->rec_step = 0;
+}
+
+void free_graphs(atvh_batch* b) {
+    for (int k = 0; k < 2; ++k) {
+        if (b->graph[k])
+            (void)hipGraphExecDestroy(b->graph[k]);
+        b->graph[k] = nullptr;
+        b->graph_prec[k] = -1;
+    }
+}
+
+void enqueue_step(atvh_batch* b, int precision, bool ef) {
+    integ_batch_launch(b, "initial");
+    integ_batch_launch(b, "post_initial");
+    nl_batch_launch_rebuild(b);
+    pair_batch_launch(b, precision, ef);
+    if (ef)
+        integ_batch_launch(b, "finalize");
+    integ_batch_launch(b, "final");
+    integ_batch_launch(b, "post_final");
+}
+
+void build_stages(atvh_batch* b, int nsteps) {
+    // thermo buffers: one record per step per member, NaN where no energy was asked
+    for (atvh_system* s : b->sys) {
+        if (s->thermo_cap < nsteps) {
+            dev_free(s->d_thermo);
+            s->thermo_cap = nsteps;
+            dev_alloc(&s->d_thermo, size_t(2) * nsteps);
+        }
+        s->thermo_n = nsteps;
+        std::vector<double> nanv(size_t(2) * s->thermo_cap, std::nan(""));
+        ATVH_HIP(hipMemcpy(s->d_thermo, nanv.data(), nanv.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    batch_free_stages(b);
+    free_graphs(b);
+    integ_batch_build(b);
+    nl_batch_build(b);
+    pair_batch_build(b);
+}
+
+} // namespace
+
+namespace atvh {
+
+void batch_free_stages(atvh_batch* b) {
+    for (auto& kv : b->stages) {
+        dev_free(kv.second.args);
+        dev_free(kv.second.base);
+    }
+    b->stages.clear();
+}
+
+} // namespace atvh
+
+#define API_TRY try {
+#define API_CATCH(C)                                                                    \
+    }                                                                                   \
+    catch (const atvh::Error& e_) {                                                     \
+        if (C)                                                                          \
+            (C)->err = e_.what();                                                       \
+        return e_.code;                                                                 \
+    }                                                                                   \
+    catch (const std::exception& e_) {                                                  \
+        if (C)                                                                          \
+            (C)->err = e_.what();                                                       \
+        return ATVH_E_ARG;                                                              \
+    }
+
+extern "C" {
+
+int atvh_batch_create(atvh_ctx* ctx, atvh_system* const* systems, int nsys, atvh_batch** out) {
+    if (!ctx || !out || !systems || nsys < 1)
+        return ATVH_E_ARG;
+    API_TRY
+    ATVH_HIP(hipSetDevice(ctx->device));
+    auto* b = new atvh_batch();
+    b->ctx = ctx;
+    try {
+        std::vector<DevCtl*> ctls;
+        for (int k = 0; k < nsys; ++k) {
+            atvh_system* s = systems[k];
+            if (!s || s->ctx != ctx)
+                throw Error(ATVH_E_ARG, "atvh_batch_create: members must be systems of this context");
+            for (int j = 0; j < k; ++j)
+                if (systems[j] == s)
+                    throw Error(ATVH_E_ARG, "atvh_batch_create: a system appears twice");
+            b->sys.push_back(s);
+            ctls.push_back(s->ctl);
+        }
+        dev_alloc(&b->d_ctls, ctls.size());
+        ATVH_HIP(hipMemcpy(b->d_ctls, ctls.data(), ctls.size() * sizeof(DevCtl*), hipMemcpyHostToDevice));
+    } catch (...) {
+        dev_free(b->d_ctls);
+        delete b;
+        throw;
+    }
+    ctx->live_systems += 1; // a batch pins its context like a system does
+    *out = b;
+    return ATVH_OK;
+    API_CATCH(ctx)
+}
+
+int atvh_batch_destroy(atvh_batch* b) {
+    if (!b)
+        return ATVH_E_ARG;
+    atvh_ctx* c = b->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_graphs(b);
+    batch_free_stages(b);
+    dev_free(b->d_ctls);
+    c->live_systems -= 1;
+    delete b;
+    return ATVH_OK;
+}
+
+int atvh_batch_set_graph(atvh_batch* b, int enable) {
+    if (!b)
+        return ATVH_E_ARG;
+    b->use_graph = enable != 0;
+    free_graphs(b);
+    return ATVH_OK;
+}
+
+int atvh_batch_run(atvh_batch* b, int nsteps, int precision, int energy_every) {
+    if (!b)
+        return ATVH_E_ARG;
+    atvh_ctx* c = b->ctx;
+    API_TRY
+    if (nsteps < 0)
+        throw Error(ATVH_E_ARG, "atvh_batch_run: nsteps < 0");
+    if (precision != ATVH_PREC_FP64 && precision != ATVH_PREC_MIXED)
+        throw Error(ATVH_E_ARG, "atvh_batch_run: unknown precision");
+    ATVH_HIP(hipSetDevice(c->device));
+    for (atvh_system* s : b->sys) {
+        if (!s->integ_configured || !s->nl_configured || !s->pair_configured)
+            throw Error(ATVH_E_STATE, "atvh_batch_run: every member needs an integrator, a neighbor list and "
+                                      "a pair style");
+        if (!s->nl_built) {
+            const int rc = atvh_nlist_build(s);
+            if (rc != ATVH_OK)
+                throw Error(rc, std::string("atvh_batch_run: member list build failed: ") + c->err);
+        }
+    }
+    if (nsteps == 0)
+        return ATVH_OK;
+    hipStream_t st = c->stream;
+    build_stages(b, nsteps);
+    const int nsys = static_cast<int>(b->sys.size());
+    k_batch_reset<<<(nsys + kBlock - 1) / kBlock, kBlock, 0, st>>>(b->d_ctls, nsys);
+    ATVH_HIP(hipGetLastError());
+    const bool timing = c->timing;
+    for (int step = 0; step < nsteps; ++step) {
+        const bool ef = (step == nsteps - 1) || (energy_every > 0 && (step % energy_every) == 0);
+        if (b->use_graph && !timing) {
+            hipGraphExec_t& ge = b->graph[ef ? 1 : 0];
+            if (!ge || b->graph_prec[ef ? 1 : 0] != precision) {
+                if (ge)
+                    ATVH_HIP(hipGraphExecDestroy(ge));
+                ge = nullptr;
+                hipGraph_t g = nullptr;
+                ATVH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                try {
+                    enqueue_step(b, precision, ef);
+                } catch (...) {
+                    (void)hipStreamEndCapture(st, &g);
+                    if (g)
+                        (void)hipGraphDestroy(g);
+                    throw;
+                }
+                ATVH_HIP(hipStreamEndCapture(st, &g));
+                const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                ATVH_HIP(e);
+                b->graph_prec[ef ? 1 : 0] = precision;
+            }
+            ATVH_HIP(hipGraphLaunch(ge, st));
+        } else {
+            ScopedTimer tm(c, "batch_step");
+            enqueue_step(b, precision, ef);
+        }
+    }
+    // capacity and halo overflows of any member surface here
+    ATVH_HIP(hipStreamSynchronize(st));
+    for (atvh_system* s : b->sys) {
+        DevCtl h{};
+        ATVH_HIP(hipMemcpy(&h, s->ctl, sizeof(h), hipMemcpyDeviceToHost));
+        nl_check_ctl(h);
+    }
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+} // extern "C"

@@ -1,0 +1,38 @@
+// batch.hpp — one launch over many independent systems.
+//
+// A kernel body is written once as `Op::run(args, bid, nbk)`, where bid/nbk
+// stand for blockIdx.x/gridDim.x of a launch over one system. The
+// single-system kernel calls it with the hardware indices; the batched
+// kernel gives every system a contiguous range of blocks (prefix table
+// `base`, nsys+1 entries), finds its system by binary search and calls the
+// same body with the system's own argument block. Every system therefore
+// runs exactly the code (and the reduction order) it runs alone.
+#pragma once
+
+#include "atvh_internal.hpp"
+
+namespace atvh {
+
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_single(typename Op::Args a) {
+    Op::run(a, blockIdx.x, gridDim.x);
+}
+
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_batched(const typename Op::Args* __restrict__ args,
+                                                    const int* __restrict__ base, int nsys) {
+    const int blk = blockIdx.x;
+    int lo = 0, hi = nsys - 1; // largest s with base[s] <= blk
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= blk)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    // by reference: the argument block stays in global memory (a private copy
+    // would be indexed dynamically for per-type tables and land in scratch)
+    Op::run(args[lo], blk - base[lo], base[lo + 1] - base[lo]);
+}
+
+} // namespace atvh

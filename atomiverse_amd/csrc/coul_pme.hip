@@ -128,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     if (nh > kHaloCap) { // reported by the build; nothing valid to compute
         if (EF || VF)
-            block_partials(acc, a.partials);
+            block_partials(acc, a.partials, blockIdx.x);
         return;
     }
     const int* __restrict__ rows = a.halo_rows + (int64_t)b * kHaloCap;
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
 #pragma unroll
         for (int k = 0; k < 7; ++k)
             acc[k] *= 0.5; // full list: every pair seen from both ends
-        block_partials(acc, a.partials);
+        block_partials(acc, a.partials, blockIdx.x);
     }
 }
 
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_mixed(LjcArgs a) {
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     if (nh > kHaloCap) {
         if (EF || VF)
-            block_partials(acc, a.partials);
+            block_partials(acc, a.partials, blockIdx.x);
         return;
     }
     const int2 br = a.brick_range[b];
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_mixed(LjcArgs a) {
 #pragma unroll
         for (int k = 0; k < 7; ++k)
             acc[k] *= 0.5;
-        block_partials(acc, a.partials);
+        block_partials(acc, a.partials, blockIdx.x);
     }
 }
 
@@ -386,7 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_ljc_global(LjcArgs a) {
 #pragma unroll
         for (int k = 0; k < 7; ++k)
             acc[k] *= 0.5;
-        block_partials(acc, a.partials);
+        block_partials(acc, a.partials, blockIdx.x);
     }
 }
 
@@ -553,7 +553,7 @@ __global__ __launch_bounds__(kBlock) void k_pme_influence(PmeArgs a, T2* __restr
         cplx[idx] = c;
     }
     if (EV)
-        block_partials(acc, a.partials);
+        block_partials(acc, a.partials, blockIdx.x);
 }
 
 template <class T, int N>

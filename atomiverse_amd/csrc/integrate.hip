@@ -21,6 +21,7 @@
 // (cell_list.cpp:372-391): 4*|min_image(x - x_save)|^2 > skin^2 per atom.
 
 #include "atvh_internal.hpp"
+#include "batch.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -33,6 +34,7 @@ namespace {
 constexpr int kIntGrid = 1024; // integrator grid cap: few kinetic partials to reduce
 
 struct IntArgs {
+    int64_t n;
     double *x, *y, *z, *vx, *vy, *vz;
     const double *fx, *fy, *fz;
     const int* type;
@@ -69,72 +71,76 @@ __device__ __forceinline__ void block_sum_to(double v, double* out) {
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_initial(int64_t n, IntArgs a) {
-    const double s = a.thermostat ? a.ctl->nh_scale : 1.0;
-    double mv2 = 0.0;
-    int moved = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int t = a.ntypes > 1 ? a.type[i] : 0;
-        const double dtfm = a.dtfm[t];
-        double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
-        if (a.thermostat) {
-            vx *= s;
-            vy *= s;
-            vz *= s;
+struct OpInitial {
+    using Args = IntArgs;
+    static __device__ void run(const IntArgs& a, int bid, int nbk) {
+        const double s = a.thermostat ? a.ctl->nh_scale : 1.0;
+        double mv2 = 0.0;
+        int moved = 0;
+        for (int64_t i = bid * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)nbk * blockDim.x) {
+            const int t = a.ntypes > 1 ? a.type[i] : 0;
+            const double dtfm = a.dtfm[t];
+            double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
+            if (a.thermostat) {
+                vx *= s;
+                vy *= s;
+                vz *= s;
+            }
+            vx += dtfm * a.fx[i];
+            vy += dtfm * a.fy[i];
+            vz += dtfm * a.fz[i];
+            // drift; the periodic wrap of wrap_orthorhombic is deferred to the next
+            // neighbor rebuild (and to downloads), which keeps the image of every
+            // listed pair fixed between rebuilds
+            const double x = a.x[i] + a.dt * vx;
+            const double y = a.y[i] + a.dt * vy;
+            const double z = a.z[i] + a.dt * vz;
+            a.x[i] = x;
+            a.y[i] = y;
+            a.z[i] = z;
+            a.vx[i] = vx;
+            a.vy[i] = vy;
+            a.vz[i] = vz;
+            mv2 += a.mass[t] * (vx * vx + vy * vy + vz * vz);
+            double dx = x - a.xs[i], dy = y - a.ys[i], dz = z - a.zs[i];
+            dx -= a.lx * nearbyint(dx / a.lx);
+            dy -= a.ly * nearbyint(dy / a.ly);
+            dz -= a.lz * nearbyint(dz / a.lz);
+            const double dsq = dx * dx + dy * dy + dz * dz;
+            if (dsq * 4.0 > a.skin_sq)
+                moved = 1;
         }
-        vx += dtfm * a.fx[i];
-        vy += dtfm * a.fy[i];
-        vz += dtfm * a.fz[i];
-        // drift; the periodic wrap of wrap_orthorhombic is deferred to the next
-        // neighbor rebuild (and to downloads), which keeps the image of every
-        // listed pair fixed between rebuilds
-        const double x = a.x[i] + a.dt * vx;
-        const double y = a.y[i] + a.dt * vy;
-        const double z = a.z[i] + a.dt * vz;
-        a.x[i] = x;
-        a.y[i] = y;
-        a.z[i] = z;
-        a.vx[i] = vx;
-        a.vy[i] = vy;
-        a.vz[i] = vz;
-        mv2 += a.mass[t] * (vx * vx + vy * vy + vz * vz);
-        double dx = x - a.xs[i], dy = y - a.ys[i], dz = z - a.zs[i];
-        dx -= a.lx * nearbyint(dx / a.lx);
-        dy -= a.ly * nearbyint(dy / a.ly);
-        dz -= a.lz * nearbyint(dz / a.lz);
-        const double dsq = dx * dx + dy * dy + dz * dz;
-        if (dsq * 4.0 > a.skin_sq)
-            moved = 1;
+        if (__any(moved) && (threadIdx.x & 63) == 0)
+            a.ctl->disp_flag = 1;
+        block_sum_to(mv2, &a.ke_partials[bid]);
     }
-    if (__any(moved) && (threadIdx.x & 63) == 0)
-        a.ctl->disp_flag = 1;
-    block_sum_to(mv2, &a.ke_partials[blockIdx.x]);
-}
+};
 
-__global__ __launch_bounds__(kBlock) void k_final(int64_t n, IntArgs a) {
-    const double s = a.thermostat ? a.ctl->nh_scale : 1.0;
-    double mv2 = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int t = a.ntypes > 1 ? a.type[i] : 0;
-        const double dtfm = a.dtfm[t];
-        double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
-        if (a.thermostat) {
-            vx *= s;
-            vy *= s;
-            vz *= s;
+struct OpFinal {
+    using Args = IntArgs;
+    static __device__ void run(const IntArgs& a, int bid, int nbk) {
+        const double s = a.thermostat ? a.ctl->nh_scale : 1.0;
+        double mv2 = 0.0;
+        for (int64_t i = bid * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)nbk * blockDim.x) {
+            const int t = a.ntypes > 1 ? a.type[i] : 0;
+            const double dtfm = a.dtfm[t];
+            double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
+            if (a.thermostat) {
+                vx *= s;
+                vy *= s;
+                vz *= s;
+            }
+            vx += dtfm * a.fx[i];
+            vy += dtfm * a.fy[i];
+            vz += dtfm * a.fz[i];
+            a.vx[i] = vx;
+            a.vy[i] = vy;
+            a.vz[i] = vz;
+            mv2 += a.mass[t] * (vx * vx + vy * vy + vz * vz);
         }
-        vx += dtfm * a.fx[i];
-        vy += dtfm * a.fy[i];
-        vz += dtfm * a.fz[i];
-        a.vx[i] = vx;
-        a.vy[i] = vy;
-        a.vz[i] = vz;
-        mv2 += a.mass[t] * (vx * vx + vy * vy + vz * vz);
+        block_sum_to(mv2, &a.ke_partials[bid]);
     }
-    block_sum_to(mv2, &a.ke_partials[blockIdx.x]);
-}
+};
 
 // Fixed-order sum of nb partials (stride `stride`) over one 256-thread block.
 __device__ double block_fixed_sum(const double* p, int nb, int stride) {
@@ -163,45 +169,51 @@ struct PostArgs {
     // decide (initial phase only)
     int decide, every, delay, check;
     double* record_ke; // may be null
-    int step;
+    int step;          // record index; < 0: the device counter ctl->rec_step (advanced here)
     int final_phase;
 };
 
 // NoseHoover::launch scalar part + NeighborList::decide_reneighbor.
-__global__ __launch_bounds__(kBlock) void k_post(PostArgs a) {
-    const double mv2 = block_fixed_sum(a.ke_partials, a.nb, 1);
-    if (threadIdx.x != 0)
-        return;
-    DevCtl* c = a.ctl;
-    const double ke = 0.5 * mass_v2_to_energy * mv2;
-    double scale = 1.0;
-    if (a.thermostat) {
-        c->xi += (a.dt * 0.5) * (2.0 * ke - a.ndof_kT) / a.Q;
-        scale = exp(-a.dt * 0.5 * c->xi);
-        c->nh_scale = scale;
-    }
-    if (a.final_phase) {
-        c->ke = ke * scale * scale; // KineticEnergy at end_of_step (scaled v)
-        if (a.record_ke)
-            a.record_ke[a.step] = c->ke;
-    } else {
-        c->ke_half = ke;
-    }
-    if (a.decide) {
-        c->ago += 1;
-        const bool delay_ok = c->ago >= a.delay;
-        const bool every_ok = (c->ago % a.every) == 0;
-        const bool rb = delay_ok && every_ok && (!a.check || c->disp_flag != 0);
-        c->disp_flag = 0;
-        c->rebuild = rb ? 1 : 0;
-        if (rb) {
-            c->ago = 0;
-            c->nrebuilds += 1;
-            c->nbr_total = 0;
-            c->max_nbr = 0;
+struct OpPost {
+    using Args = PostArgs;
+    static __device__ void run(const PostArgs& a, int, int) {
+        const double mv2 = block_fixed_sum(a.ke_partials, a.nb, 1);
+        if (threadIdx.x != 0)
+            return;
+        DevCtl* c = a.ctl;
+        const double ke = 0.5 * mass_v2_to_energy * mv2;
+        double scale = 1.0;
+        if (a.thermostat) {
+            c->xi += (a.dt * 0.5) * (2.0 * ke - a.ndof_kT) / a.Q;
+            scale = exp(-a.dt * 0.5 * c->xi);
+            c->nh_scale = scale;
+        }
+        if (a.final_phase) {
+            c->ke = ke * scale * scale; // KineticEnergy at end_of_step (scaled v)
+            const int step = a.step >= 0 ? a.step : c->rec_step;
+            if (a.record_ke)
+                a.record_ke[step] = c->ke;
+            if (a.step < 0)
+                c->rec_step = step + 1;
+        } else {
+            c->ke_half = ke;
+        }
+        if (a.decide) {
+            c->ago += 1;
+            const bool delay_ok = c->ago >= a.delay;
+            const bool every_ok = (c->ago % a.every) == 0;
+            const bool rb = delay_ok && every_ok && (!a.check || c->disp_flag != 0);
+            c->disp_flag = 0;
+            c->rebuild = rb ? 1 : 0;
+            if (rb) {
+                c->ago = 0;
+                c->nrebuilds += 1;
+                c->nbr_total = 0;
+                c->max_nbr = 0;
+            }
         }
     }
-}
+};
 
 struct FinalizeArgs {
     DevCtl* ctl;
@@ -211,40 +223,44 @@ struct FinalizeArgs {
     int knb;
     double escale, e_self, e_net;
     double* record_pe;
-    int step;
+    int step; // record index; < 0: the device counter ctl->rec_step
 };
 
 // Fixed-order totals of the force phase: pair energy and virial, then the
 // k-space terms in the reference's order (self first, then reciprocal + net
 // charge; pme.cpp:222-246).
-__global__ __launch_bounds__(kBlock) void k_energy_finalize(FinalizeArgs a) {
-    double v[7], k[7];
-    for (int c = 0; c < 7; ++c)
-        v[c] = a.nb > 0 ? block_fixed_sum(a.partials + c, a.nb, kPartialStride) : 0.0;
-    for (int c = 0; c < 7; ++c)
-        k[c] = a.knb > 0 ? block_fixed_sum(a.kpartials + c, a.knb, kPartialStride) : 0.0;
-    if (threadIdx.x == 0) {
-        double pe = v[0];
-        if (a.knb > 0) {
-            const double e_recip = k[0] * a.escale;
-            pe += a.e_self;
-            pe += e_recip + a.e_net;
+struct OpFinalize {
+    using Args = FinalizeArgs;
+    static __device__ void run(const FinalizeArgs& a, int, int) {
+        double v[7], k[7];
+        for (int c = 0; c < 7; ++c)
+            v[c] = a.nb > 0 ? block_fixed_sum(a.partials + c, a.nb, kPartialStride) : 0.0;
+        for (int c = 0; c < 7; ++c)
+            k[c] = a.knb > 0 ? block_fixed_sum(a.kpartials + c, a.knb, kPartialStride) : 0.0;
+        if (threadIdx.x == 0) {
+            double pe = v[0];
+            if (a.knb > 0) {
+                const double e_recip = k[0] * a.escale;
+                pe += a.e_self;
+                pe += e_recip + a.e_net;
+                for (int c = 0; c < 6; ++c)
+                    v[c + 1] += k[c + 1] * a.escale;
+                a.ctl->e_kspace[0] = e_recip;
+                a.ctl->e_kspace[1] = a.e_self;
+                a.ctl->e_kspace[2] = a.e_net;
+            }
+            a.ctl->pe = pe;
             for (int c = 0; c < 6; ++c)
-                v[c + 1] += k[c + 1] * a.escale;
-            a.ctl->e_kspace[0] = e_recip;
-            a.ctl->e_kspace[1] = a.e_self;
-            a.ctl->e_kspace[2] = a.e_net;
+                a.ctl->vir[c] = v[c + 1];
+            if (a.record_pe)
+                a.record_pe[a.step >= 0 ? a.step : a.ctl->rec_step] = pe;
         }
-        a.ctl->pe = pe;
-        for (int c = 0; c < 6; ++c)
-            a.ctl->vir[c] = v[c + 1];
-        if (a.record_pe)
-            a.record_pe[a.step] = pe;
     }
-}
+};
 
 IntArgs make_int_args(atvh_system* s) {
     IntArgs a;
+    a.n = s->n;
     a.x = s->x;
     a.y = s->y;
     a.z = s->z;
@@ -310,14 +326,69 @@ int integ_grid(const atvh_system* s) { return std::min(grid_for(s->n), kIntGrid)
 void integ_enqueue_initial(atvh_system* s) {
     ScopedTimer tm(s->ctx, "integrate");
     IntArgs a = make_int_args(s);
-    k_initial<<<integ_grid(s), kBlock, 0, s->ctx->stream>>>(s->n, a);
+    k_single<OpInitial><<<integ_grid(s), kBlock, 0, s->ctx->stream>>>(a);
     ATVH_HIP(hipGetLastError());
 }
 
 void integ_enqueue_final(atvh_system* s) {
     ScopedTimer tm(s->ctx, "integrate");
     IntArgs a = make_int_args(s);
-    k_final<<<integ_grid(s), kBlock, 0, s->ctx->stream>>>(s->n, a);
+    k_single<OpFinal><<<integ_grid(s), kBlock, 0, s->ctx->stream>>>(a);
+    ATVH_HIP(hipGetLastError());
+}
+
+// Batched stages: kick/drift (+ displacement flag), NH + decide, final kick,
+// NH + end-of-step KE, and the force-phase energy totals. Records go to each
+// member's thermo buffer at the member's device step counter.
+void integ_batch_build(atvh_batch* b) {
+    std::vector<IntArgs> ia;
+    std::vector<PostArgs> pi, pf;
+    std::vector<FinalizeArgs> fa;
+    std::vector<int> gi, g1;
+    for (atvh_system* s : b->sys) {
+        ia.push_back(make_int_args(s));
+        gi.push_back(integ_grid(s));
+        PostArgs p0 = make_post_args(s, false, true);
+        PostArgs p1 = make_post_args(s, true, false);
+        p1.record_ke = s->d_thermo + s->thermo_cap;
+        p1.step = -1;
+        pi.push_back(p0);
+        pf.push_back(p1);
+        FinalizeArgs f;
+        f.ctl = s->ctl;
+        f.partials = s->partials;
+        f.nb = pair_grid(s);
+        f.kpartials = nullptr;
+        f.knb = 0;
+        f.escale = f.e_self = f.e_net = 0.0;
+        f.record_pe = s->d_thermo;
+        f.step = -1;
+        fa.push_back(f);
+        g1.push_back(1);
+    }
+    batch_stage(b, "initial", ia, gi);
+    batch_stage(b, "final", ia, gi);
+    batch_stage(b, "post_initial", pi, g1);
+    batch_stage(b, "post_final", pf, g1);
+    batch_stage(b, "finalize", fa, g1);
+}
+
+void integ_batch_launch(atvh_batch* b, const char* stage) {
+    const BatchStage& st = b->stages.at(stage);
+    hipStream_t s = b->ctx->stream;
+    const int nsys = static_cast<int>(b->sys.size());
+    const std::string name(stage);
+    if (name == "initial")
+        k_batched<OpInitial><<<st.nblocks, kBlock, 0, s>>>(static_cast<const IntArgs*>(st.args), st.base, nsys);
+    else if (name == "final")
+        k_batched<OpFinal><<<st.nblocks, kBlock, 0, s>>>(static_cast<const IntArgs*>(st.args), st.base, nsys);
+    else if (name == "post_initial" || name == "post_final")
+        k_batched<OpPost><<<st.nblocks, kBlock, 0, s>>>(static_cast<const PostArgs*>(st.args), st.base, nsys);
+    else if (name == "finalize")
+        k_batched<OpFinalize><<<st.nblocks, kBlock, 0, s>>>(static_cast<const FinalizeArgs*>(st.args), st.base,
+                                                             nsys);
+    else
+        throw Error(ATVH_E_ARG, "unknown batch stage");
     ATVH_HIP(hipGetLastError());
 }
 
@@ -333,7 +404,7 @@ void energy_enqueue_finalize(atvh_system* s, double* record_pe, int step, bool w
         pme_energy_consts(s, a.escale, a.e_self, a.e_net);
     a.record_pe = record_pe;
     a.step = step;
-    k_energy_finalize<<<1, kBlock, 0, s->ctx->stream>>>(a);
+    k_single<OpFinalize><<<1, kBlock, 0, s->ctx->stream>>>(a);
     ATVH_HIP(hipGetLastError());
 }
 
@@ -388,11 +459,11 @@ int atvh_step_vvnh(atvh_system* s, int phase) {
     if (phase == ATVH_PHASE_INITIAL) {
         integ_enqueue_initial(s);
         PostArgs p = make_post_args(s, false, false);
-        k_post<<<1, kBlock, 0, c->stream>>>(p);
+        k_single<OpPost><<<1, kBlock, 0, c->stream>>>(p);
     } else if (phase == ATVH_PHASE_FINAL) {
         integ_enqueue_final(s);
         PostArgs p = make_post_args(s, true, false);
-        k_post<<<1, kBlock, 0, c->stream>>>(p);
+        k_single<OpPost><<<1, kBlock, 0, c->stream>>>(p);
     } else {
         throw Error(ATVH_E_ARG, "atvh_step_vvnh: unknown phase");
     }
@@ -445,9 +516,9 @@ int atvh_md_run(atvh_system* s, int nsteps, int precision, int energy_every) {
     for (int step = 0; step < nsteps; ++step) {
         {
             ScopedTimer tm(c, "integrate");
-            k_initial<<<g, kBlock, 0, st>>>(s->n, ia);
+            k_single<OpInitial><<<g, kBlock, 0, st>>>(ia);
         }
-        k_post<<<1, kBlock, 0, st>>>(p_init);
+        k_single<OpPost><<<1, kBlock, 0, st>>>(p_init);
         nl_enqueue_rebuild(s, &s->ctl->rebuild, false);
         const bool ef = (step == nsteps - 1) || (energy_every > 0 && (step % energy_every) == 0);
         pair_enqueue(s, precision, ef, ef);
@@ -455,10 +526,10 @@ int atvh_md_run(atvh_system* s, int nsteps, int precision, int energy_every) {
             energy_enqueue_finalize(s, rec_pe, step);
         {
             ScopedTimer tm(c, "integrate");
-            k_final<<<g, kBlock, 0, st>>>(s->n, ia);
+            k_single<OpFinal><<<g, kBlock, 0, st>>>(ia);
         }
         p_fin.step = step;
-        k_post<<<1, kBlock, 0, st>>>(p_fin);
+        k_single<OpPost><<<1, kBlock, 0, st>>>(p_fin);
         ATVH_HIP(hipGetLastError());
     }
     return ATVH_OK;

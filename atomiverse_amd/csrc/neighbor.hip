@@ -26,6 +26,7 @@
 // for the decision. All expressions here are compiled with -ffp-contract=off.
 
 #include "atvh_internal.hpp"
+#include "batch.hpp"
 #include "brick.hpp"
 
 #include <algorithm>
@@ -39,41 +40,58 @@ namespace {
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kBlock * kScanItems; // elements per scan block
 
+// Arguments of the counting-sort kernels (one system).
+struct SortArgs {
+    const int* pred;
+    int64_t n;
+    int ncells;
+    double *x, *y, *z;
+    double lx, ly, lz, csx, csy, csz;
+    int ncx, ncy, ncz;
+    const int* cell_rank;
+    int* cell_of;
+    int* count;
+    int* start;
+    int* tile_sums;
+    int ntiles;
+    int* fill;
+    int* perm;
+    const int* tag;
+};
+
 // CellList::bin_atoms: cell of every slot (current order) + histogram.
-__global__ void k_nl_bin(const int* __restrict__ pred, int64_t n, double* __restrict__ x,
-                         double* __restrict__ y, double* __restrict__ z, double lx,
-                         double ly, double lz, double csx, double csy, double csz, int ncx, int ncy,
-                         int ncz, const int* __restrict__ cell_rank, int* __restrict__ cell_of,
-                         int* __restrict__ count) {
-    if (!*pred)
-        return;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        // wrap_orthorhombic (pbc.h:37-51) is applied here, at the rebuild,
-        // rather than every step: between rebuilds positions stay unwrapped so
-        // the periodic image of every listed pair is the one fixed at the build.
-        double xi = x[i], yi = y[i], zi = z[i];
-        xi -= lx * floor(xi / lx);
-        yi -= ly * floor(yi / ly);
-        zi -= lz * floor(zi / lz);
-        x[i] = xi;
-        y[i] = yi;
-        z[i] = zi;
-        // bin_atoms wraps its own copy once more (a no-op except at x == L)
-        xi -= lx * floor(xi / lx);
-        yi -= ly * floor(yi / ly);
-        zi -= lz * floor(zi / lz);
-        int cx = static_cast<int>(floor((xi - 0.0) / csx));
-        int cy = static_cast<int>(floor((yi - 0.0) / csy));
-        int cz = static_cast<int>(floor((zi - 0.0) / csz));
-        cx = max(0, min(cx, ncx - 1));
-        cy = max(0, min(cy, ncy - 1));
-        cz = max(0, min(cz, ncz - 1));
-        const int c = cx * ncy * ncz + cy * ncz + cz;
-        cell_of[i] = c;
-        atomicAdd(&count[cell_rank[c]], 1);
+struct OpBin {
+    using Args = SortArgs;
+    static __device__ void run(const SortArgs& a, int bid, int nbk) {
+        if (!*a.pred)
+            return;
+        for (int64_t i = bid * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)nbk * blockDim.x) {
+            // wrap_orthorhombic (pbc.h:37-51) is applied here, at the rebuild,
+            // rather than every step: between rebuilds positions stay unwrapped so
+            // the periodic image of every listed pair is the one fixed at the build.
+            double xi = a.x[i], yi = a.y[i], zi = a.z[i];
+            xi -= a.lx * floor(xi / a.lx);
+            yi -= a.ly * floor(yi / a.ly);
+            zi -= a.lz * floor(zi / a.lz);
+            a.x[i] = xi;
+            a.y[i] = yi;
+            a.z[i] = zi;
+            // bin_atoms wraps its own copy once more (a no-op except at x == L)
+            xi -= a.lx * floor(xi / a.lx);
+            yi -= a.ly * floor(yi / a.ly);
+            zi -= a.lz * floor(zi / a.lz);
+            int cx = static_cast<int>(floor((xi - 0.0) / a.csx));
+            int cy = static_cast<int>(floor((yi - 0.0) / a.csy));
+            int cz = static_cast<int>(floor((zi - 0.0) / a.csz));
+            cx = max(0, min(cx, a.ncx - 1));
+            cy = max(0, min(cy, a.ncy - 1));
+            cz = max(0, min(cz, a.ncz - 1));
+            const int c = cx * a.ncy * a.ncz + cy * a.ncz + cz;
+            a.cell_of[i] = c;
+            atomicAdd(&a.count[a.cell_rank[c]], 1);
+        }
     }
-}
+};
 
 // Block-wide exclusive scan of one int per thread (LDS Hillis-Steele).
 __device__ int block_exclusive_scan(int v, int* sh, int& total) {
@@ -93,106 +111,110 @@ __device__ int block_exclusive_scan(int v, int* sh, int& total) {
 }
 
 // Scan phase 1: per-tile exclusive scan (tile = kScanTile cells) + tile total.
-__global__ __launch_bounds__(kBlock) void k_nl_scan_tiles(const int* __restrict__ pred, int ncells,
-                                                          const int* __restrict__ count,
-                                                          int* __restrict__ start,
-                                                          int* __restrict__ tile_sums) {
-    if (!*pred)
-        return;
-    __shared__ int sh[kBlock];
-    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    int v[kScanItems];
-    int sum = 0;
-    for (int k = 0; k < kScanItems; ++k) {
-        v[k] = (base + k < ncells) ? count[base + k] : 0;
-        sum += v[k];
+struct OpScanTiles {
+    using Args = SortArgs;
+    static __device__ void run(const SortArgs& a, int bid, int) {
+        if (!*a.pred)
+            return;
+        __shared__ int sh[kBlock];
+        const int base = bid * kScanTile + threadIdx.x * kScanItems;
+        int v[kScanItems];
+        int sum = 0;
+        for (int k = 0; k < kScanItems; ++k) {
+            v[k] = (base + k < a.ncells) ? a.count[base + k] : 0;
+            sum += v[k];
+        }
+        int total;
+        int ex = block_exclusive_scan(sum, sh, total);
+        for (int k = 0; k < kScanItems; ++k) {
+            if (base + k < a.ncells)
+                a.start[base + k] = ex;
+            ex += v[k];
+        }
+        if (threadIdx.x == 0)
+            a.tile_sums[bid] = total;
     }
-    int total;
-    int ex = block_exclusive_scan(sum, sh, total);
-    for (int k = 0; k < kScanItems; ++k) {
-        if (base + k < ncells)
-            start[base + k] = ex;
-        ex += v[k];
-    }
-    if (threadIdx.x == 0)
-        tile_sums[blockIdx.x] = total;
-}
+};
 
 // Scan phase 2: one block scans the tile totals (any count, chunked).
-__global__ __launch_bounds__(kBlock) void k_nl_scan_sums(const int* __restrict__ pred, int ntiles,
-                                                         int* __restrict__ tile_sums,
-                                                         int* __restrict__ start, int ncells) {
-    if (!*pred)
-        return;
-    __shared__ int sh[kBlock];
-    int carry = 0;
-    for (int b0 = 0; b0 < ntiles; b0 += kBlock) {
-        const int idx = b0 + threadIdx.x;
-        const int v = idx < ntiles ? tile_sums[idx] : 0;
-        int total;
-        const int ex = block_exclusive_scan(v, sh, total);
-        if (idx < ntiles)
-            tile_sums[idx] = ex + carry;
-        carry += total;
+struct OpScanSums {
+    using Args = SortArgs;
+    static __device__ void run(const SortArgs& a, int, int) {
+        if (!*a.pred)
+            return;
+        __shared__ int sh[kBlock];
+        int carry = 0;
+        for (int b0 = 0; b0 < a.ntiles; b0 += kBlock) {
+            const int idx = b0 + threadIdx.x;
+            const int v = idx < a.ntiles ? a.tile_sums[idx] : 0;
+            int total;
+            const int ex = block_exclusive_scan(v, sh, total);
+            if (idx < a.ntiles)
+                a.tile_sums[idx] = ex + carry;
+            carry += total;
+        }
+        if (threadIdx.x == 0)
+            a.start[a.ncells] = carry;
     }
-    if (threadIdx.x == 0)
-        start[ncells] = carry;
-}
+};
 
 // Scan phase 3: add the tile offsets.
-__global__ __launch_bounds__(kBlock) void k_nl_scan_add(const int* __restrict__ pred, int ncells,
-                                                        const int* __restrict__ tile_sums,
-                                                        int* __restrict__ start) {
-    if (!*pred)
-        return;
-    const int off = tile_sums[blockIdx.x];
-    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    for (int k = 0; k < kScanItems; ++k)
-        if (base + k < ncells)
-            start[base + k] += off;
-}
+struct OpScanAdd {
+    using Args = SortArgs;
+    static __device__ void run(const SortArgs& a, int bid, int) {
+        if (!*a.pred)
+            return;
+        const int off = a.tile_sums[bid];
+        const int base = bid * kScanTile + threadIdx.x * kScanItems;
+        for (int k = 0; k < kScanItems; ++k)
+            if (base + k < a.ncells)
+                a.start[base + k] += off;
+    }
+};
 
 // Counting-sort scatter: slot of every atom inside its cell (order fixed
-// afterwards by k_nl_cell_order). Also clears the histogram for next time.
-__global__ void k_nl_scatter(const int* __restrict__ pred, int64_t n,
-                             const int* __restrict__ cell_of, const int* __restrict__ cell_rank,
-                             const int* __restrict__ start, int* __restrict__ fill,
-                             int* __restrict__ count, int* __restrict__ perm) {
-    if (!*pred)
-        return;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int c = cell_rank[cell_of[i]];
-        const int slot = start[c] + atomicAdd(&fill[c], 1);
-        perm[slot] = static_cast<int>(i);
-        count[c] = 0;
+// afterwards by the in-cell order pass). Also clears the histogram.
+struct OpScatter {
+    using Args = SortArgs;
+    static __device__ void run(const SortArgs& a, int bid, int nbk) {
+        if (!*a.pred)
+            return;
+        for (int64_t i = bid * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)nbk * blockDim.x) {
+            const int c = a.cell_rank[a.cell_of[i]];
+            const int slot = a.start[c] + atomicAdd(&a.fill[c], 1);
+            a.perm[slot] = static_cast<int>(i);
+            a.count[c] = 0;
+        }
     }
-}
+};
 
 // Deterministic order inside a cell: ascending tag (insertion sort; cells
 // hold a handful of atoms). Also clears the fill counters.
-__global__ void k_nl_cell_order(const int* __restrict__ pred, int ncells,
-                                const int* __restrict__ start, const int* __restrict__ tag,
-                                int* __restrict__ perm, int* __restrict__ fill) {
-    if (!*pred)
-        return;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x) {
-        const int b = start[c], e = start[c + 1];
-        for (int k = b + 1; k < e; ++k) {
-            const int p = perm[k];
-            const int tp = tag[p];
-            int m = k - 1;
-            while (m >= b && tag[perm[m]] > tp) {
-                perm[m + 1] = perm[m];
-                --m;
+struct OpCellOrder {
+    using Args = SortArgs;
+    static __device__ void run(const SortArgs& a, int bid, int nbk) {
+        if (!*a.pred)
+            return;
+        for (int c = bid * blockDim.x + threadIdx.x; c < a.ncells; c += nbk * blockDim.x) {
+            const int b = a.start[c], e = a.start[c + 1];
+            for (int k = b + 1; k < e; ++k) {
+                const int p = a.perm[k];
+                const int tp = a.tag[p];
+                int m = k - 1;
+                while (m >= b && a.tag[a.perm[m]] > tp) {
+                    a.perm[m + 1] = a.perm[m];
+                    --m;
+                }
+                a.perm[m + 1] = p;
             }
-            perm[m + 1] = p;
+            a.fill[c] = 0;
         }
-        fill[c] = 0;
     }
-}
+};
 
 struct PermArgs {
+    const int* pred;
+    int64_t n;
     const int* perm;
     const double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *q;
     const int *type, *tag, *cell_of;
@@ -201,11 +223,12 @@ struct PermArgs {
     int with_forces;
 };
 
-__global__ void k_nl_permute(const int* __restrict__ pred, int64_t n, PermArgs a) {
-    if (!*pred)
+struct OpPermute {
+    using Args = PermArgs;
+    static __device__ void run(const PermArgs& a, int bid, int nbk) {
+    if (!*a.pred)
         return;
-    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
-         s += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t s = bid * (int64_t)blockDim.x + threadIdx.x; s < a.n; s += (int64_t)nbk * blockDim.x) {
         const int p = a.perm[s];
         a.ax[s] = a.x[p];
         a.ay[s] = a.y[p];
@@ -223,9 +246,12 @@ __global__ void k_nl_permute(const int* __restrict__ pred, int64_t n, PermArgs a
             a.afz[s] = a.fz[p];
         }
     }
-}
+    }
+};
 
 struct CopyArgs {
+    const int* pred;
+    int64_t n;
     const double *ax, *ay, *az, *avx, *avy, *avz, *afx, *afy, *afz, *aq;
     const int *atype, *atag, *acell_of;
     double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *q, *xs, *ys, *zs;
@@ -234,11 +260,12 @@ struct CopyArgs {
 };
 
 // Copy the permuted arrays back and snapshot x_save (CellList::save_positions).
-__global__ void k_nl_copyback(const int* __restrict__ pred, int64_t n, CopyArgs a) {
-    if (!*pred)
+struct OpCopyback {
+    using Args = CopyArgs;
+    static __device__ void run(const CopyArgs& a, int bid, int nbk) {
+    if (!*a.pred)
         return;
-    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
-         s += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t s = bid * (int64_t)blockDim.x + threadIdx.x; s < a.n; s += (int64_t)nbk * blockDim.x) {
         const double xx = a.ax[s], yy = a.ay[s], zz = a.az[s];
         a.x[s] = xx;
         a.y[s] = yy;
@@ -259,7 +286,8 @@ __global__ void k_nl_copyback(const int* __restrict__ pred, int64_t n, CopyArgs 
             a.fz[s] = a.afz[s];
         }
     }
-}
+    }
+};
 
 struct BuildArgs {
     const double *x, *y, *z;
@@ -399,6 +427,7 @@ __global__ __launch_bounds__(kBlock) void k_nl_build(const int* __restrict__ pre
 }
 
 struct BrickBuildArgs {
+    const int* pred;
     BrickGeom g;
     const int* halo_cells; // static per-brick halo cell table (rank | code << 26)
     int* halo_off;         // [nbricks*(kHaloCellCap+1)] halo row offset of every halo cell
@@ -427,14 +456,19 @@ constexpr int kMaxBrickAtoms = 320;
 // brick's halo table from the static halo-cell table and the new cell_start —
 // halo rows (global slot | image code), per-halo-cell row offsets, the halo
 // size and the brick's slot range.
-__global__ __launch_bounds__(kBlock) void k_nl_halo(const int* __restrict__ pred, BrickBuildArgs a) {
-    if (!*pred)
+struct OpHalo {
+    using Args = BrickBuildArgs;
+    static __device__ void run(const BrickBuildArgs& a, int bid, int nbk);
+};
+
+__device__ void OpHalo::run(const BrickBuildArgs& a, int bid, int nbk) {
+    if (!*a.pred)
         return;
     __shared__ int h_start[kHaloCellCap + 1];
     __shared__ int h_off[kHaloCellCap + 1];
     __shared__ int h_shift[kHaloCellCap];
     __shared__ int scan_tmp[kBlock];
-    for (int b = blockIdx.x; b < a.nbricks; b += gridDim.x) {
+    for (int b = bid; b < a.nbricks; b += nbk) {
         HaloTable ht;
         build_halo_table(a.g, a.halo_cells, a.start, b, ht, h_start, h_off, h_shift, scan_tmp);
         const int a0 = a.start[a.g.brick_cell_base[b]];
@@ -495,8 +529,13 @@ __device__ __noinline__ bool exact_pair(const double* __restrict__ x, const doub
     return r2 < rc_sq;
 }
 
-__global__ __launch_bounds__(kBlock) void k_nl_pairs(const int* __restrict__ pred, BrickBuildArgs a) {
-    if (!*pred)
+struct OpPairs {
+    using Args = BrickBuildArgs;
+    static __device__ void run(const BrickBuildArgs& a, int bid, int nbk);
+};
+
+__device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
+    if (!*a.pred)
         return;
     __shared__ float4 sp[kHaloCap]; // image x, y, z relative to the brick centre; w = halo z index
     __shared__ int h_off[kHaloCellCap + 1];
@@ -518,7 +557,7 @@ __global__ __launch_bounds__(kBlock) void k_nl_pairs(const int* __restrict__ pre
     const float rc2f = static_cast<float>(a.rc_sq);
     const float lo_acc = rc2f - kBand, hi_rej = rc2f + kBand;
     const int ncols = a.nrows;
-    for (int b = blockIdx.x; b < a.nbricks; b += gridDim.x) {
+    for (int b = bid; b < a.nbricks; b += nbk) {
         const int nh = a.halo_n[b];
         const int2 br = a.brick_range[b];
         const int a0 = br.x, nat = br.y;
@@ -1053,66 +1092,230 @@ void nl_free(atvh_system* s) {
     dev_free(s->brick_range);
 }
 
+SortArgs nl_sort_args(atvh_system* s, const int* pred) {
+    SortArgs a;
+    a.pred = pred;
+    a.n = s->n;
+    a.ncells = s->ncells;
+    a.x = s->x;
+    a.y = s->y;
+    a.z = s->z;
+    a.lx = s->box[0];
+    a.ly = s->box[1];
+    a.lz = s->box[2];
+    a.csx = s->cs[0];
+    a.csy = s->cs[1];
+    a.csz = s->cs[2];
+    a.ncx = s->nc[0];
+    a.ncy = s->nc[1];
+    a.ncz = s->nc[2];
+    a.cell_rank = s->cell_rank;
+    a.cell_of = s->cell_of;
+    a.count = s->cell_count;
+    a.start = s->cell_start;
+    a.tile_sums = s->scan_sums;
+    a.ntiles = s->nscan_blocks;
+    a.fill = s->cell_fill;
+    a.perm = s->perm;
+    a.tag = s->tag;
+    return a;
+}
+
+PermArgs nl_perm_args(atvh_system* s, const int* pred, bool permute_forces) {
+    PermArgs a;
+    a.pred = pred;
+    a.n = s->n;
+    a.perm = s->perm;
+    a.x = s->x;
+    a.y = s->y;
+    a.z = s->z;
+    a.vx = s->vx;
+    a.vy = s->vy;
+    a.vz = s->vz;
+    a.fx = s->fx;
+    a.fy = s->fy;
+    a.fz = s->fz;
+    a.q = s->q;
+    a.type = s->type;
+    a.tag = s->tag;
+    a.cell_of = s->cell_of;
+    a.ax = s->ax;
+    a.ay = s->ay;
+    a.az = s->az;
+    a.avx = s->avx;
+    a.avy = s->avy;
+    a.avz = s->avz;
+    a.afx = s->afx;
+    a.afy = s->afy;
+    a.afz = s->afz;
+    a.aq = s->aq;
+    a.atype = s->atype;
+    a.atag = s->atag;
+    a.acell_of = s->acell_of;
+    a.with_forces = permute_forces ? 1 : 0;
+    return a;
+}
+
+CopyArgs nl_copy_args(atvh_system* s, const int* pred, bool permute_forces) {
+    CopyArgs a;
+    a.pred = pred;
+    a.n = s->n;
+    a.ax = s->ax;
+    a.ay = s->ay;
+    a.az = s->az;
+    a.avx = s->avx;
+    a.avy = s->avy;
+    a.avz = s->avz;
+    a.afx = s->afx;
+    a.afy = s->afy;
+    a.afz = s->afz;
+    a.aq = s->aq;
+    a.atype = s->atype;
+    a.atag = s->atag;
+    a.acell_of = s->acell_of;
+    a.x = s->x;
+    a.y = s->y;
+    a.z = s->z;
+    a.vx = s->vx;
+    a.vy = s->vy;
+    a.vz = s->vz;
+    a.fx = s->fx;
+    a.fy = s->fy;
+    a.fz = s->fz;
+    a.q = s->q;
+    a.xs = s->xs;
+    a.ys = s->ys;
+    a.zs = s->zs;
+    a.type = s->type;
+    a.tag = s->tag;
+    a.cell_of = s->cell_of;
+    a.with_forces = permute_forces ? 1 : 0;
+    return a;
+}
+
+BrickBuildArgs nl_brick_args(atvh_system* s, const int* pred) {
+    BrickBuildArgs ba;
+    ba.pred = pred;
+    ba.g = brick_geom(s);
+    ba.halo_cells = s->halo_cells;
+    ba.nbricks = s->nbricks;
+    ba.halo_rows = s->halo_rows;
+    ba.halo_n = s->halo_n;
+    ba.brick_range = s->brick_range;
+    ba.x = s->x;
+    ba.y = s->y;
+    ba.z = s->z;
+    ba.cell_of = s->cell_of;
+    ba.start = s->cell_start;
+    ba.rows = reinterpret_cast<const int4*>(s->d_cols);
+    ba.nrows = s->ncols;
+    ba.rc_sq = s->cutoff_skin * s->cutoff_skin;
+    ba.rcs = s->cutoff_skin;
+    ba.nbr16 = s->nbr16;
+    ba.self16 = s->self16;
+    ba.numneigh = s->numneigh;
+    ba.cap = s->cap;
+    ba.ctl = s->ctl;
+    ba.halo_off = s->halo_off;
+    return ba;
+}
+
+// Launch grids of the rebuild stages for one system (the batched rebuild
+// gives every system the same block counts).
+void nl_stage_grids(const atvh_system* s, int g[kNlStages]) {
+    const int ga = grid_capped(s->n);
+    g[0] = ga;                            // bin
+    g[1] = s->nscan_blocks;               // scan tiles
+    g[2] = 1;                             // scan sums
+    g[3] = s->nscan_blocks;               // scan add
+    g[4] = ga;                            // scatter
+    g[5] = grid_capped(s->ncells);        // in-cell order
+    g[6] = ga;                            // permute
+    g[7] = ga;                            // copy back
+    g[8] = std::min(s->nbricks, 4096);    // halo rows
+    g[9] = std::min(s->nbricks, 2048);    // pair build
+}
+
 void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
     ScopedTimer tm(s->ctx, "nlist");
     hipStream_t st = s->ctx->stream;
-    const int64_t n = s->n;
-    const int g = grid_capped(n);
-    k_nl_bin<<<g, kBlock, 0, st>>>(pred, n, s->x, s->y, s->z, s->box[0], s->box[1], s->box[2],
-                                   s->cs[0], s->cs[1], s->cs[2], s->nc[0], s->nc[1], s->nc[2],
-                                   s->cell_rank, s->cell_of, s->cell_count);
-    k_nl_scan_tiles<<<s->nscan_blocks, kBlock, 0, st>>>(pred, s->ncells, s->cell_count,
-                                                        s->cell_start, s->scan_sums);
-    k_nl_scan_sums<<<1, kBlock, 0, st>>>(pred, s->nscan_blocks, s->scan_sums, s->cell_start,
-                                         s->ncells);
-    k_nl_scan_add<<<s->nscan_blocks, kBlock, 0, st>>>(pred, s->ncells, s->scan_sums,
-                                                      s->cell_start);
-    k_nl_scatter<<<g, kBlock, 0, st>>>(pred, n, s->cell_of, s->cell_rank, s->cell_start,
-                                       s->cell_fill, s->cell_count, s->perm);
-    k_nl_cell_order<<<grid_capped(s->ncells), kBlock, 0, st>>>(pred, s->ncells, s->cell_start,
-                                                               s->tag, s->perm, s->cell_fill);
-    PermArgs pa{s->perm, s->x,  s->y,  s->z,  s->vx,  s->vy,  s->vz,  s->fx,  s->fy,  s->fz,
-                s->q,    s->type, s->tag, s->cell_of, s->ax, s->ay, s->az, s->avx, s->avy,
-                s->avz,  s->afx, s->afy, s->afz, s->aq, s->atype, s->atag, s->acell_of,
-                permute_forces ? 1 : 0};
-    k_nl_permute<<<g, kBlock, 0, st>>>(pred, n, pa);
-    CopyArgs ca{s->ax, s->ay, s->az, s->avx, s->avy, s->avz, s->afx, s->afy, s->afz, s->aq,
-                s->atype, s->atag, s->acell_of, s->x, s->y, s->z, s->vx, s->vy, s->vz,
-                s->fx, s->fy, s->fz, s->q, s->xs, s->ys, s->zs, s->type, s->tag, s->cell_of,
-                permute_forces ? 1 : 0};
-    k_nl_copyback<<<g, kBlock, 0, st>>>(pred, n, ca);
+    int g[kNlStages];
+    nl_stage_grids(s, g);
+    const SortArgs sa = nl_sort_args(s, pred);
+    k_single<OpBin><<<g[0], kBlock, 0, st>>>(sa);
+    k_single<OpScanTiles><<<g[1], kBlock, 0, st>>>(sa);
+    k_single<OpScanSums><<<g[2], kBlock, 0, st>>>(sa);
+    k_single<OpScanAdd><<<g[3], kBlock, 0, st>>>(sa);
+    k_single<OpScatter><<<g[4], kBlock, 0, st>>>(sa);
+    k_single<OpCellOrder><<<g[5], kBlock, 0, st>>>(sa);
+    k_single<OpPermute><<<g[6], kBlock, 0, st>>>(nl_perm_args(s, pred, permute_forces));
+    k_single<OpCopyback><<<g[7], kBlock, 0, st>>>(nl_copy_args(s, pred, permute_forces));
     if (s->brick_mode) {
-        BrickBuildArgs ba;
-        ba.g = brick_geom(s);
-        ba.halo_cells = s->halo_cells;
-        ba.nbricks = s->nbricks;
-        ba.halo_rows = s->halo_rows;
-        ba.halo_n = s->halo_n;
-        ba.brick_range = s->brick_range;
-        ba.x = s->x;
-        ba.y = s->y;
-        ba.z = s->z;
-        ba.cell_of = s->cell_of;
-        ba.start = s->cell_start;
-        ba.rows = reinterpret_cast<const int4*>(s->d_cols);
-        ba.nrows = s->ncols;
-        ba.rc_sq = s->cutoff_skin * s->cutoff_skin;
-        ba.rcs = s->cutoff_skin;
-        ba.nbr16 = s->nbr16;
-        ba.self16 = s->self16;
-        ba.numneigh = s->numneigh;
-        ba.cap = s->cap;
-        ba.ctl = s->ctl;
-        ba.halo_off = s->halo_off;
-        k_nl_halo<<<std::min(s->nbricks, 4096), kBlock, 0, st>>>(pred, ba);
-        k_nl_pairs<<<std::min(s->nbricks, 2048), kBlock, 0, st>>>(pred, ba);
+        const BrickBuildArgs ba = nl_brick_args(s, pred);
+        k_single<OpHalo><<<g[8], kBlock, 0, st>>>(ba);
+        k_single<OpPairs><<<g[9], kBlock, 0, st>>>(ba);
     } else {
         BuildArgs ba{s->x, s->y, s->z, s->cell_of, s->cell_rank, s->cell_start,
                      reinterpret_cast<const int4*>(s->d_stencil), s->nstencil, s->nc[0], s->nc[1],
                      s->nc[2], s->box[0], s->box[1], s->box[2], s->cutoff_skin * s->cutoff_skin,
                      0, s->nbr, s->numneigh, s->cap, s->npad, s->ctl};
-        k_nl_build<false><<<grid_for(n), kBlock, 0, st>>>(pred, n, ba);
+        k_nl_build<false><<<grid_for(s->n), kBlock, 0, st>>>(pred, s->n, ba);
     }
+    ATVH_HIP(hipGetLastError());
+}
+
+// Batched rebuild: the ten stages of a brick-mode rebuild, one launch each
+// over every member; a member whose rebuild flag is clear leaves each stage
+// at its first instruction.
+void nl_batch_build(atvh_batch* b) {
+    std::vector<SortArgs> sa;
+    std::vector<PermArgs> pa;
+    std::vector<CopyArgs> ca;
+    std::vector<BrickBuildArgs> ba;
+    std::vector<int> g[kNlStages];
+    for (atvh_system* s : b->sys) {
+        if (!s->brick_mode)
+            throw Error(ATVH_E_ARG, "batch: every member needs the brick layout (>= 2*reach+1 cells per axis)");
+        const int* pred = &s->ctl->rebuild;
+        sa.push_back(nl_sort_args(s, pred));
+        pa.push_back(nl_perm_args(s, pred, false));
+        ca.push_back(nl_copy_args(s, pred, false));
+        ba.push_back(nl_brick_args(s, pred));
+        int gs[kNlStages];
+        nl_stage_grids(s, gs);
+        for (int k = 0; k < kNlStages; ++k)
+            g[k].push_back(gs[k]);
+    }
+    batch_stage(b, "nl_bin", sa, g[0]);
+    batch_stage(b, "nl_scan_tiles", sa, g[1]);
+    batch_stage(b, "nl_scan_sums", sa, g[2]);
+    batch_stage(b, "nl_scan_add", sa, g[3]);
+    batch_stage(b, "nl_scatter", sa, g[4]);
+    batch_stage(b, "nl_cell_order", sa, g[5]);
+    batch_stage(b, "nl_permute", pa, g[6]);
+    batch_stage(b, "nl_copyback", ca, g[7]);
+    batch_stage(b, "nl_halo", ba, g[8]);
+    batch_stage(b, "nl_pairs", ba, g[9]);
+}
+
+template <class Op>
+static void nl_batch_one(atvh_batch* b, const char* name) {
+    const BatchStage& st = b->stages.at(name);
+    k_batched<Op><<<st.nblocks, kBlock, 0, b->ctx->stream>>>(static_cast<const typename Op::Args*>(st.args),
+                                                              st.base, static_cast<int>(b->sys.size()));
+}
+
+void nl_batch_launch_rebuild(atvh_batch* b) {
+    nl_batch_one<OpBin>(b, "nl_bin");
+    nl_batch_one<OpScanTiles>(b, "nl_scan_tiles");
+    nl_batch_one<OpScanSums>(b, "nl_scan_sums");
+    nl_batch_one<OpScanAdd>(b, "nl_scan_add");
+    nl_batch_one<OpScatter>(b, "nl_scatter");
+    nl_batch_one<OpCellOrder>(b, "nl_cell_order");
+    nl_batch_one<OpPermute>(b, "nl_permute");
+    nl_batch_one<OpCopyback>(b, "nl_copyback");
+    nl_batch_one<OpHalo>(b, "nl_halo");
+    nl_batch_one<OpPairs>(b, "nl_pairs");
     ATVH_HIP(hipGetLastError());
 }
 

@@ -15,9 +15,10 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
-// Fixed-order block reduction of 7 doubles into partials[block*8 + k]
-// (256-thread blocks: 4 waves, summed wave 0..3 by thread k).
-__device__ __forceinline__ void block_partials(double (&acc)[7], double* partials) {
+// Fixed-order block reduction of 7 doubles into partials[slot*8 + k]
+// (256-thread blocks: 4 waves, summed wave 0..3 by thread k); slot is the
+// block's index within its system's launch.
+__device__ __forceinline__ void block_partials(double (&acc)[7], double* partials, int slot) {
     __shared__ double sh[kBlock / 64][7];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -32,7 +33,7 @@ __device__ __forceinline__ void block_partials(double (&acc)[7], double* partial
         double t = 0.0;
         for (int w = 0; w < kBlock / 64; ++w)
             t += sh[w][threadIdx.x];
-        partials[blockIdx.x * kPartialStride + threadIdx.x] = t;
+        partials[slot * kPartialStride + threadIdx.x] = t;
     }
 }
 

@@ -25,6 +25,7 @@
 #include "atvh_internal.hpp"
 #include "brick.hpp"
 #include "pair_common.hpp"
+#include "batch.hpp"
 
 #include <cmath>
 
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_lj_fp64(int64_t n, LJArgs a) {
 #pragma unroll
         for (int k = 0; k < 7; ++k)
             acc[k] *= 0.5; // full list: every pair seen from both ends
-        block_partials(acc, a.partials);
+        block_partials(acc, a.partials, blockIdx.x);
     }
 }
 
@@ -171,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void k_lj_mixed(int64_t n, LJArgs a) {
 #pragma unroll
         for (int k = 0; k < 7; ++k)
             acc[k] *= 0.5;
-        block_partials(acc, a.partials);
+        block_partials(acc, a.partials, blockIdx.x);
     }
 }
 
@@ -197,15 +198,21 @@ struct BrickLJArgs {
 };
 
 template <bool EF, bool VF, bool TYPED>
-__global__ __launch_bounds__(kBlock) void k_lj_brick_fp64(BrickLJArgs a) {
+struct OpLjBrickFp64 {
+    using Args = BrickLJArgs;
+    static __device__ void run(const BrickLJArgs& a, int bid, int nbk);
+};
+
+template <bool EF, bool VF, bool TYPED>
+__device__ void OpLjBrickFp64<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid, int) {
     __shared__ double sx[kHaloCap], sy[kHaloCap], sz[kHaloCap];
     __shared__ int stype[TYPED ? kHaloCap : 1];
-    const int b = blockIdx.x;
+    const int b = bid;
     const int nh = a.halo_n[b];
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     if (nh > kHaloCap) { // reported by the build; nothing valid to compute
         if (EF || VF)
-            block_partials(acc, a.partials);
+            block_partials(acc, a.partials, bid);
         return;
     }
     const int* __restrict__ rows = a.halo_rows + (int64_t)b * kHaloCap;
@@ -297,19 +304,25 @@ __global__ __launch_bounds__(kBlock) void k_lj_brick_fp64(BrickLJArgs a) {
 #pragma unroll
         for (int k = 0; k < 7; ++k)
             acc[k] *= 0.5;
-        block_partials(acc, a.partials);
+        block_partials(acc, a.partials, bid);
     }
 }
 
 template <bool EF, bool VF, bool TYPED>
-__global__ __launch_bounds__(kBlock) void k_lj_brick_mixed(BrickLJArgs a) {
+struct OpLjBrickMixed {
+    using Args = BrickLJArgs;
+    static __device__ void run(const BrickLJArgs& a, int bid, int nbk);
+};
+
+template <bool EF, bool VF, bool TYPED>
+__device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid, int) {
     __shared__ float4 sp[kHaloCap]; // x, y, z relative to the brick centre; w = type bits
-    const int b = blockIdx.x;
+    const int b = bid;
     const int nh = a.halo_n[b];
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     if (nh > kHaloCap) {
         if (EF || VF)
-            block_partials(acc, a.partials);
+            block_partials(acc, a.partials, bid);
         return;
     }
     const int2 br = a.brick_range[b];
@@ -406,27 +419,27 @@ __global__ __launch_bounds__(kBlock) void k_lj_brick_mixed(BrickLJArgs a) {
 #pragma unroll
         for (int k = 0; k < 7; ++k)
             acc[k] *= 0.5;
-        block_partials(acc, a.partials);
+        block_partials(acc, a.partials, bid);
     }
 }
 
-#define LJ_BRICK_DISPATCH(KERNEL, EF, VF, TY, GRID, ST, ARGS)                                  \
+#define LJ_BRICK_DISPATCH(OP, EF, VF, TY, GRID, ST, ARGS)                                      \
     do {                                                                                      \
         if (EF) {                                                                             \
             if (VF) {                                                                         \
-                if (TY) KERNEL<true, true, true><<<GRID, kBlock, 0, ST>>>(ARGS);              \
-                else KERNEL<true, true, false><<<GRID, kBlock, 0, ST>>>(ARGS);                \
+                if (TY) k_single<OP<true, true, true>><<<GRID, kBlock, 0, ST>>>(ARGS);        \
+                else k_single<OP<true, true, false>><<<GRID, kBlock, 0, ST>>>(ARGS);          \
             } else {                                                                          \
-                if (TY) KERNEL<true, false, true><<<GRID, kBlock, 0, ST>>>(ARGS);             \
-                else KERNEL<true, false, false><<<GRID, kBlock, 0, ST>>>(ARGS);               \
+                if (TY) k_single<OP<true, false, true>><<<GRID, kBlock, 0, ST>>>(ARGS);       \
+                else k_single<OP<true, false, false>><<<GRID, kBlock, 0, ST>>>(ARGS);         \
             }                                                                                 \
         } else {                                                                              \
             if (VF) {                                                                         \
-                if (TY) KERNEL<false, true, true><<<GRID, kBlock, 0, ST>>>(ARGS);             \
-                else KERNEL<false, true, false><<<GRID, kBlock, 0, ST>>>(ARGS);               \
+                if (TY) k_single<OP<false, true, true>><<<GRID, kBlock, 0, ST>>>(ARGS);       \
+                else k_single<OP<false, true, false>><<<GRID, kBlock, 0, ST>>>(ARGS);         \
             } else {                                                                          \
-                if (TY) KERNEL<false, false, true><<<GRID, kBlock, 0, ST>>>(ARGS);            \
-                else KERNEL<false, false, false><<<GRID, kBlock, 0, ST>>>(ARGS);              \
+                if (TY) k_single<OP<false, false, true>><<<GRID, kBlock, 0, ST>>>(ARGS);      \
+                else k_single<OP<false, false, false>><<<GRID, kBlock, 0, ST>>>(ARGS);        \
             }                                                                                 \
         }                                                                                     \
     } while (0)
@@ -481,9 +494,7 @@ void pair_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
         pme_enqueue(s, precision, eflag || vflag);
 }
 
-static void lj_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
-    if (s->lj_ntypes > 0 && s->ntypes > s->lj_ntypes)
-        throw Error(ATVH_E_ARG, "PairLJCut: the frame has atom types beyond the coefficient table");
+static LJArgs lj_args(atvh_system* s) {
     LJArgs a;
     a.x = s->x;
     a.y = s->y;
@@ -513,44 +524,56 @@ static void lj_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
     a.fy = s->fy;
     a.fz = s->fz;
     a.partials = s->partials;
+    return a;
+}
+
+static BrickLJArgs brick_lj_args(atvh_system* s, const LJArgs& a) {
+    BrickLJArgs b;
+    b.x = s->x;
+    b.y = s->y;
+    b.z = s->z;
+    b.type = s->type;
+    b.halo_rows = s->halo_rows;
+    b.halo_n = s->halo_n;
+    b.brick_range = s->brick_range;
+    b.nbr16 = s->nbr16;
+    b.self16 = s->self16;
+    b.numneigh = s->numneigh;
+    b.cap = s->cap;
+    for (int d = 0; d < 3; ++d) {
+        b.L[d] = s->box[d];
+        b.cs[d] = s->cs[d];
+        b.nc[d] = s->nc[d];
+        b.nb[d] = s->nb[d];
+        b.bdim[d] = s->bdim[d];
+    }
+    b.rc_sq = a.rc_sq;
+    b.eps24 = a.eps24;
+    b.eps4 = a.eps4;
+    b.sig6 = a.sig6;
+    b.vshift = a.vshift;
+    b.tab = a.tab;
+    b.ntypes = a.ntypes;
+    b.fx = s->fx;
+    b.fy = s->fy;
+    b.fz = s->fz;
+    b.partials = s->partials;
+    return b;
+}
+
+static void lj_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
+    if (s->lj_ntypes > 0 && s->ntypes > s->lj_ntypes)
+        throw Error(ATVH_E_ARG, "PairLJCut: the frame has atom types beyond the coefficient table");
+    const LJArgs a = lj_args(s);
     const bool typed = s->lj_ntypes > 0;
     const int g = pair_grid(s);
     hipStream_t st = s->ctx->stream;
     if (s->brick_mode) {
-        BrickLJArgs b;
-        b.x = s->x;
-        b.y = s->y;
-        b.z = s->z;
-        b.type = s->type;
-        b.halo_rows = s->halo_rows;
-        b.halo_n = s->halo_n;
-        b.brick_range = s->brick_range;
-        b.nbr16 = s->nbr16;
-        b.self16 = s->self16;
-        b.numneigh = s->numneigh;
-        b.cap = s->cap;
-        for (int d = 0; d < 3; ++d) {
-            b.L[d] = s->box[d];
-            b.cs[d] = s->cs[d];
-            b.nc[d] = s->nc[d];
-            b.nb[d] = s->nb[d];
-            b.bdim[d] = s->bdim[d];
-        }
-        b.rc_sq = a.rc_sq;
-        b.eps24 = a.eps24;
-        b.eps4 = a.eps4;
-        b.sig6 = a.sig6;
-        b.vshift = a.vshift;
-        b.tab = a.tab;
-        b.ntypes = a.ntypes;
-        b.fx = s->fx;
-        b.fy = s->fy;
-        b.fz = s->fz;
-        b.partials = s->partials;
+        const BrickLJArgs b = brick_lj_args(s, a);
         if (precision == ATVH_PREC_MIXED)
-            LJ_BRICK_DISPATCH(k_lj_brick_mixed, eflag, vflag, typed, g, st, b);
+            LJ_BRICK_DISPATCH(OpLjBrickMixed, eflag, vflag, typed, g, st, b);
         else
-            LJ_BRICK_DISPATCH(k_lj_brick_fp64, eflag, vflag, typed, g, st, b);
+            LJ_BRICK_DISPATCH(OpLjBrickFp64, eflag, vflag, typed, g, st, b);
         ATVH_HIP(hipGetLastError());
         return;
     }
@@ -558,6 +581,55 @@ static void lj_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
         LJ_DISPATCH(k_lj_mixed, eflag, vflag, typed, g, st, s->n, a);
     else
         LJ_DISPATCH(k_lj_fp64, eflag, vflag, typed, g, st, s->n, a);
+    ATVH_HIP(hipGetLastError());
+}
+
+// Batched pair stage: one workgroup per brick of every member (PairLJCut).
+void pair_batch_build(atvh_batch* b) {
+    std::vector<BrickLJArgs> args;
+    std::vector<int> blocks;
+    bool typed_any = false, typed_all = true;
+    for (atvh_system* s : b->sys) {
+        if (!s->pair_configured || s->pair_style != 0)
+            throw Error(ATVH_E_ARG, "batch: every member needs PairLJCut");
+        if (s->pme)
+            throw Error(ATVH_E_ARG, "batch: k-space members are not batched");
+        const bool t = s->lj_ntypes > 0;
+        typed_any |= t;
+        typed_all &= t;
+        args.push_back(brick_lj_args(s, lj_args(s)));
+        blocks.push_back(s->nbricks);
+    }
+    if (typed_any && !typed_all)
+        throw Error(ATVH_E_ARG, "batch: members must all be single-type or all per-type");
+    b->typed = typed_any;
+    batch_stage(b, "pair", args, blocks);
+}
+
+template <class Op>
+static void pair_batch_one(atvh_batch* b) {
+    const BatchStage& st = b->stages.at("pair");
+    k_batched<Op><<<st.nblocks, kBlock, 0, b->ctx->stream>>>(static_cast<const BrickLJArgs*>(st.args), st.base,
+                                                              static_cast<int>(b->sys.size()));
+}
+
+void pair_batch_launch(atvh_batch* b, int precision, bool ef) {
+    const bool mixed = precision == ATVH_PREC_MIXED;
+    if (mixed) {
+        if (ef)
+            b->typed ? pair_batch_one<OpLjBrickMixed<true, true, true>>(b)
+                     : pair_batch_one<OpLjBrickMixed<true, true, false>>(b);
+        else
+            b->typed ? pair_batch_one<OpLjBrickMixed<false, false, true>>(b)
+                     : pair_batch_one<OpLjBrickMixed<false, false, false>>(b);
+    } else {
+        if (ef)
+            b->typed ? pair_batch_one<OpLjBrickFp64<true, true, true>>(b)
+                     : pair_batch_one<OpLjBrickFp64<true, true, false>>(b);
+        else
+            b->typed ? pair_batch_one<OpLjBrickFp64<false, false, true>>(b)
+                     : pair_batch_one<OpLjBrickFp64<false, false, false>>(b);
+    }
     ATVH_HIP(hipGetLastError());
 }
 

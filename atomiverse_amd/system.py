@@ -14,14 +14,17 @@ import numpy as np
 from . import _abi
 from ._abi import dptr, iptr, lptr
 
-# Live handles, released in dependency order (systems, then contexts) before
+# Live handles, released in dependency order (batches, systems, then contexts) before
 # the interpreter and the HIP runtime shut down.
+_live_batches = weakref.WeakSet()
 _live_systems = weakref.WeakSet()
 _live_contexts = weakref.WeakSet()
 
 
 @atexit.register
 def _release_all():
+    for b in list(_live_batches):
+        b.close()
     for s in list(_live_systems):
         s.close()
     for c in list(_live_contexts):
@@ -222,3 +225,34 @@ class System:
 
     def synchronize(self):
         self.ctx.synchronize()
+
+
+class Batch:
+    """atvh_batch: member systems advanced together, one launch per stage."""
+
+    def __init__(self, ctx: Context, systems):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.systems = list(systems)
+        arr = (C.c_void_p * len(self.systems))(*[s.h.value for s in self.systems])
+        h = C.c_void_p()
+        _check(self.lib.atvh_batch_create(ctx.h, arr, len(self.systems), C.byref(h)), ctx.h)
+        self.h = h
+        _live_batches.add(self)
+
+    def close(self):
+        if self.h:
+            self.lib.atvh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_graph(self, enable=True):
+        _check(self.lib.atvh_batch_set_graph(self.h, 1 if enable else 0), self.ctx.h)
+
+    def run(self, nsteps, precision=_abi.PREC_FP64, energy_every=0):
+        _check(self.lib.atvh_batch_run(self.h, int(nsteps), precision, int(energy_every)), self.ctx.h)

@@ -147,6 +147,33 @@ def lj_c2():
     return charged_melt(80, name="charged_melt_512000_nvt")
 
 
+def lj_trimmed(natoms, density=0.8442, T_star=0.72, seed=87287, thermostat=1, name=None):
+    """FCC lattice trimmed to `natoms` sites in lattice order inside the box of
+    that density: L = (N/rho)^(1/3), ceil((N/4)^(1/3)) cells per axis of edge
+    L/cells (the first sites fill the box bottom-up; the rest is void)."""
+    L = (natoms / density) ** (1.0 / 3.0)
+    nc = math.ceil((natoms / 4.0) ** (1.0 / 3.0) - 1e-12)
+    x, y, z = fcc_lattice(nc, nc, nc, L / nc, natoms)
+    mass = np.ones(natoms)
+    T = T_star / KB
+    vx, vy, vz = init_velocity(mass, T, seed)
+    return LJWorkload(name or f"lj_trimmed_{natoms}", x, y, z, vx, vy, vz, mass, np.array([L, L, L]), T=T,
+                      thermostat=thermostat, meta=dict(density=density, T_star=T_star, seed=seed, cells=nc))
+
+
+def lj_ladder(nsys=256, nmin=500, nmax=20000, seed=1234, t_lo=0.7, t_hi=1.5):
+    """Batched REMD-style ensemble: nsys independent LJ fluids, N log-uniform in
+    [nmin, nmax] (seeded), temperatures on the geometric ladder
+    T*_k = t_lo (t_hi/t_lo)^(k/(nsys-1)), velocities seeded 87287 + k, NVT."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(math.log(nmin), math.log(nmax), nsys)).astype(np.int64)
+    out = []
+    for k, n in enumerate(sizes):
+        t = t_lo * (t_hi / t_lo) ** (k / max(nsys - 1, 1))
+        out.append(lj_trimmed(int(n), T_star=t, seed=87287 + k, thermostat=1, name=f"ladder_{k}_{int(n)}"))
+    return out
+
+
 def upload(sysobj, w: LJWorkload):
     sysobj.upload(w.x, w.y, w.z, w.mass, w.vx, w.vy, w.vz, type_=w.type_, charge=w.charge)
 

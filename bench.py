@@ -51,6 +51,55 @@ def run_gpu(av, W, ctx, precision, steps, warmup):
     return w, s, t1 - t0, kt, st0, st1
 
 
+def run_charged(av, W, ctx, precision, steps, warmup):
+    """The 512,000-ion charged melt (LJ + real-space Coulomb + PME 128^3, order 4, NVT)."""
+    w = W.lj_c2()
+    s = av.System(ctx, w.n, w.box)
+    W.upload(s, w)
+    W.configure(s, w)
+    s.nlist_build()
+    s.run(warmup, precision, energy_every=0)
+    ctx.synchronize()
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    s.run(steps, precision, energy_every=0)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ctx.enable_timing(False)
+    kt = {k: ctx.kernel_time(k) for k in ("pair", "pme", "integrate", "nlist")}
+    return w, s, t1 - t0, kt
+
+
+def charged_section(av, W, ctx, steps, warmup):
+    out = {}
+    sys_keep = {}
+    for name, prec in (("fp64", av.PREC_FP64), ("mixed", av.PREC_MIXED)):
+        log(f"bench: charged melt {name}, {warmup} warmup + {steps} timed steps")
+        w, s, dt, kt = run_charged(av, W, ctx, prec, steps, warmup)
+        out[f"{name}_atom_steps_per_s"] = w.n * steps / dt
+        out[f"{name}_ms_per_step"] = dt * 1e3 / steps
+        out[f"{name}_kernel_avg_us"] = {k: v[0] * 1e3 / max(v[1], 1) for k, v in kt.items()}
+        log(f"  {name}: {dt * 1e3 / steps:.3f} ms/step, " + ", ".join(
+            f"{k} {v[0] / max(v[1], 1) * 1e3:.1f}us x{v[1]}" for k, v in kt.items()))
+        sys_keep[name] = s
+    s = sys_keep["mixed"]
+    s.compute(av.PREC_FP64, False, False)
+    F = s.download()
+    s.compute(av.PREC_MIXED, False, False)
+    M = s.download()
+    F = np.stack([F["fx"], F["fy"], F["fz"]])
+    M = np.stack([M["fx"], M["fy"], M["fz"]])
+    out["force_rms_rel_err_mixed_vs_fp64"] = float(np.sqrt(np.sum((M - F) ** 2) / np.sum(F ** 2)))
+    out["mixed_over_fp64"] = out["mixed_atom_steps_per_s"] / out["fp64_atom_steps_per_s"]
+    out["workload"] = "charged_melt_512000_nvt (pme 128^3 order 4, rc 3.0, alpha 1.025646)"
+    out["steps"] = steps
+    for s in sys_keep.values():
+        s.close()
+    return out
+
+
 def cpu_baseline(W, nsteps):
     """The CPU oracle (FP64 restatement of the reference CPU path) on the host
     cores: a bounded sample of the same workload."""
@@ -83,6 +132,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--charged-steps", type=int, default=100,
+                    help="timed steps of the charged-melt (PME) side measurement; 0 skips it")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one MI355X; multi-GPU runs are not part of it "
@@ -144,7 +195,7 @@ def main():
         "fp64_ms_per_step": res["fp64"]["dt"] * 1e3 / steps,
         "mixed_over_fp64": value / fp64_value,
         "force_rms_rel_err_mixed_vs_fp64": rms,
-        "roofline": {"bound": "hbm", "kernel": "k_lj_mixed", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": "k_lj_brick_mixed", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_alg_per_launch": bytes_alg, "avg_launch_us": avg_s * 1e6},
         "kernel_avg_us": {name: {k: v[0] * 1e3 / max(v[1], 1) for k, v in r["kt"].items()}
@@ -152,6 +203,8 @@ def main():
     }
     for r in res.values():
         r["s"].close()
+    if args.charged_steps > 0:
+        out["charged_melt"] = charged_section(av, W, ctx, args.charged_steps, 10)
     if not args.no_cpu:
         log(f"bench: cpu baseline ({args.cpu_steps} oracle steps)")
         out["cpu_baseline"] = cpu_baseline(W, args.cpu_steps)

@@ -37,6 +37,7 @@ extern "C" {
 
 typedef struct atvh_ctx atvh_ctx;
 typedef struct atvh_system atvh_system;
+typedef struct atvh_batch atvh_batch;
 
 /* ---- context --------------------------------------------------------------
  * Replaces cuda::Backend::init's stream set-up (src/cuda/driver/driver.cu:25-32)
@@ -190,6 +191,27 @@ int64_t atvh_util_rocksalt_lattice(int nx, int ny, int nz, double spacing, doubl
                                    double* y, double* z, int* type, double* charge, double q0);
 int atvh_util_init_velocity(int64_t n, const double* mass, double T, uint32_t seed,
                             int64_t ndof, double* vx, double* vy, double* vz);
+
+/* ---- batches of independent systems ------------------------------------------------
+ * Replaces the replica loop of Remd::propagate_ (src/cpu/path/remd.cpp:98-103:
+ * one cpu::md::MD per replica, run one after another). Members are ordinary
+ * systems of one context, each uploaded and configured through the calls
+ * above (NeighborList in the brick layout, PairLJCut, VV +/- NH; atom counts,
+ * boxes and temperatures may all differ). atvh_batch_run advances every
+ * member by the same number of steps with one launch per stage over all of
+ * them; each member's trajectory is bit-identical to atvh_md_run on it alone.
+ * Per-member results: atvh_md_thermo, atvh_frame_download, atvh_energy.
+ * The batch references its members (destroy it before them).
+ */
+int atvh_batch_create(atvh_ctx* ctx, atvh_system* const* systems, int nsys,
+                      atvh_batch** out);
+int atvh_batch_destroy(atvh_batch* batch);
+/* Capture the batched step once into a hipGraph and replay it (default off). */
+int atvh_batch_set_graph(atvh_batch* batch, int enable);
+/* energy_every as in atvh_md_run. Synchronises at the end and reports any
+ * member's capacity overflow as ATVH_E_CAPACITY. */
+int atvh_batch_run(atvh_batch* batch, int nsteps, int precision,
+                   int energy_every);
 
 #ifdef __cplusplus
 }
