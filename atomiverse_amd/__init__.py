@@ -7,6 +7,6 @@ CPU fallback and raises if the HIP library is not built.
 """
 from . import _abi  # noqa: F401
 from ._abi import PHASE_FINAL, PHASE_INITIAL, PREC_FP64, PREC_MIXED, HipError, load  # noqa: F401
-from .system import Batch, Context, System  # noqa: F401
+from .system import Batch, Context, Server, System, segment_bytes  # noqa: F401
 
-__all__ = ["Batch", "Context", "System", "HipError", "load", "PREC_FP64", "PREC_MIXED", "PHASE_INITIAL", "PHASE_FINAL"]
+__all__ = ["Batch", "Context", "Server", "System", "segment_bytes", "HipError", "load", "PREC_FP64", "PREC_MIXED", "PHASE_INITIAL", "PHASE_FINAL"]

@@ -35,6 +35,7 @@ SIGNATURES = {
     "atvh_ctx_enable_timing": (C.c_int, [_vp, C.c_int]),
     "atvh_ctx_kernel_time": (C.c_int, [_vp, C.c_char_p, _dp, _lp]),
     "atvh_ctx_reset_timing": (C.c_int, [_vp]),
+    "atvh_ctx_mem_info": (C.c_int, [_vp, _lp, _lp]),
     "atvh_system_create": (C.c_int, [_vp, C.c_int64, _dp, C.POINTER(_vp)]),
     "atvh_system_destroy": (C.c_int, [_vp]),
     "atvh_system_natoms": (C.c_int64, [_vp]),
@@ -62,6 +63,14 @@ SIGNATURES = {
     "atvh_batch_destroy": (C.c_int, [_vp]),
     "atvh_batch_set_graph": (C.c_int, [_vp, C.c_int]),
     "atvh_batch_run": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "atvh_segment_bytes": (C.c_int64, [_vp]),
+    "atvh_server_create": (C.c_int, [_vp, C.c_int64, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "atvh_server_destroy": (C.c_int, [_vp]),
+    "atvh_server_submit": (C.c_int, [_vp, _vp, _lp]),
+    "atvh_server_advance": (C.c_int, [_vp, _ip, _ip]),
+    "atvh_server_poll": (C.c_int, [_vp, _lp, C.c_int, _ip]),
+    "atvh_server_fetch": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "atvh_server_usage": (C.c_int, [_vp, _lp, _lp]),
     "atvh_util_fcc_lattice": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int64, _dp, _dp, _dp]),
     "atvh_util_rocksalt_lattice": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _ip,
                                                _dp, C.c_double]),

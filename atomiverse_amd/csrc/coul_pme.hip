@@ -118,9 +118,11 @@ struct LjcArgs {
     int64_t n;
 };
 
+// Force arithmetic is written with explicit fma and no automatic
+// contraction, so it rounds identically in every EF/VF instantiation: a
+// trajectory does not depend on which steps asked for energies.
 template <bool EF, bool VF>
 __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
-#pragma clang fp contract(fast)
     __shared__ double sx[kHaloCap], sy[kHaloCap], sz[kHaloCap], sq[kHaloCap];
     __shared__ int stype[kHaloCap];
     const int b = blockIdx.x;
@@ -179,7 +181,8 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
                     const double tt = xb - static_cast<double>(bi);
                     const double4 ev = etab[bi];
                     const double qq = qi_scale * sq[q];
-                    const double fpair = r2inv * (t[0] * r6inv * r6inv - t[1] * r6inv) + qq * (ev.x + tt * ev.y);
+                    const double flj = r2inv * fma(t[0] * r6inv, r6inv, -(t[1] * r6inv));
+                    const double fpair = fma(qq, fma(tt, ev.y, ev.x), flj);
                     fxi = fma(fpair, dx, fxi);
                     fyi = fma(fpair, dy, fyi);
                     fzi = fma(fpair, dz, fzi);
@@ -314,7 +317,6 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_mixed(LjcArgs a) {
 // reference's per-pair minimum image d -= L nearbyint(d / L) (1/L hoisted).
 template <bool EF, bool VF, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_ljc_global(LjcArgs a) {
-#pragma clang fp contract(fast)
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     const int ntypes = a.ntypes, nbin1 = a.nbin - 1;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n;

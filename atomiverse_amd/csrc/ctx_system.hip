@@ -220,6 +220,21 @@ int atvh_ctx_reset_timing(atvh_ctx* c) {
     API_CATCH(c)
 }
 
+int atvh_ctx_mem_info(atvh_ctx* c, int64_t* free_bytes, int64_t* total_bytes) {
+    if (!c)
+        return ATVH_E_ARG;
+    API_TRY
+    ATVH_HIP(hipSetDevice(c->device));
+    size_t f = 0, t = 0;
+    ATVH_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes)
+        *free_bytes = static_cast<int64_t>(f);
+    if (total_bytes)
+        *total_bytes = static_cast<int64_t>(t);
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
 int atvh_system_create(atvh_ctx* c, int64_t n, const double box[3], atvh_system** out) {
     if (!c || !out || !box || n <= 0)
         return ATVH_E_ARG;

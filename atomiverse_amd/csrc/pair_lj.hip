@@ -52,7 +52,6 @@ struct LJArgs {
 
 template <bool EF, bool VF, bool TYPED>
 __global__ __launch_bounds__(kBlock) void k_lj_fp64(int64_t n, LJArgs a) {
-#pragma clang fp contract(fast)
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -81,7 +80,7 @@ __global__ __launch_bounds__(kBlock) void k_lj_fp64(int64_t n, LJArgs a) {
                 }
                 const double r2inv = 1.0 / r2;
                 const double r6inv = r2inv * r2inv * r2inv * sig6;
-                const double fpair = eps24 * r2inv * (2.0 * r6inv * r6inv - r6inv);
+                const double fpair = eps24 * r2inv * fma(2.0 * r6inv, r6inv, -r6inv);
                 const double fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
                 fxi += fxp;
                 fyi += fyp;
@@ -112,7 +111,6 @@ __global__ __launch_bounds__(kBlock) void k_lj_fp64(int64_t n, LJArgs a) {
 
 template <bool EF, bool VF, bool TYPED>
 __global__ __launch_bounds__(kBlock) void k_lj_mixed(int64_t n, LJArgs a) {
-#pragma clang fp contract(fast)
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     const float eps24s = static_cast<float>(a.eps24), eps4s = static_cast<float>(a.eps4);
     const float sig6s = static_cast<float>(a.sig6), shifts = static_cast<float>(a.vshift);
@@ -147,7 +145,7 @@ __global__ __launch_bounds__(kBlock) void k_lj_mixed(int64_t n, LJArgs a) {
                 }
                 const float r2inv = __builtin_amdgcn_rcpf(r2);
                 const float r6inv = r2inv * r2inv * r2inv * sig6;
-                const float fpair = eps24 * r2inv * (2.0f * r6inv * r6inv - r6inv);
+                const float fpair = eps24 * r2inv * __builtin_fmaf(2.0f * r6inv, r6inv, -r6inv);
                 const float fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
                 fxi += fxp;
                 fyi += fyp;

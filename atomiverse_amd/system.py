@@ -12,7 +12,7 @@ import weakref
 import numpy as np
 
 from . import _abi
-from ._abi import dptr, iptr, lptr
+from ._abi import dptr, iptr, lptr  # noqa: F401
 
 # Live handles, released in dependency order (batches, systems, then contexts) before
 # the interpreter and the HIP runtime shut down.
@@ -69,6 +69,13 @@ class Context:
 
     def reset_timing(self):
         _check(self.lib.atvh_ctx_reset_timing(self.h), self.h)
+
+    def mem_info(self):
+        """(free, total) device bytes."""
+        f = C.c_int64()
+        t = C.c_int64()
+        _check(self.lib.atvh_ctx_mem_info(self.h, C.byref(f), C.byref(t)), self.h)
+        return f.value, t.value
 
     def kernel_time(self, name):
         ms = C.c_double()
@@ -256,3 +263,122 @@ class Batch:
 
     def run(self, nsteps, precision=_abi.PREC_FP64, energy_every=0):
         _check(self.lib.atvh_batch_run(self.h, int(nsteps), precision, int(energy_every)), self.ctx.h)
+
+
+class Segment(C.Structure):
+    """atvh_segment (include/atvh.h)."""
+    _fields_ = [
+        ("n", C.c_int64), ("box", C.c_double * 3),
+        ("x", _abi._dp), ("y", _abi._dp), ("z", _abi._dp), ("vx", _abi._dp), ("vy", _abi._dp), ("vz", _abi._dp),
+        ("mass", _abi._dp), ("charge", _abi._dp), ("type", _abi._ip),
+        ("cutoff", C.c_double), ("skin", C.c_double),
+        ("pair_style", C.c_int), ("eps", C.c_double), ("sigma", C.c_double),
+        ("ntypes", C.c_int), ("eps_tab", _abi._dp), ("sigma_tab", _abi._dp), ("set_tab", _abi._ip),
+        ("alpha", C.c_double), ("pme_spacing", C.c_double), ("pme_order", C.c_int),
+        ("dt", C.c_double), ("thermostat", C.c_int), ("T", C.c_double), ("tau", C.c_double),
+        ("nsteps", C.c_int64),
+    ]
+
+
+def segment_of(w, nsteps):
+    """Segment spec of a workload (keeps its arrays alive on the returned object)."""
+    g = Segment()
+    g.n = w.n
+    for d in range(3):
+        g.box[d] = float(w.box[d])
+    keep = []
+
+    def d_(a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return dptr(a)
+
+    def i_(a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        keep.append(a)
+        return iptr(a)
+
+    g.x, g.y, g.z = d_(w.x), d_(w.y), d_(w.z)
+    g.vx, g.vy, g.vz = d_(w.vx), d_(w.vy), d_(w.vz)
+    g.mass, g.charge, g.type = d_(w.mass), d_(w.charge), i_(w.type_)
+    g.cutoff, g.skin = w.cutoff, w.skin
+    if w.pair == "ljcoul":
+        g.pair_style = 1
+        g.ntypes = int(round(np.sqrt(np.asarray(w.eps_tab).size)))
+        g.eps_tab, g.sigma_tab, g.set_tab = d_(w.eps_tab), d_(w.sigma_tab), i_(w.set_tab)
+        g.alpha, g.pme_spacing, g.pme_order = w.alpha, w.pme_spacing, w.pme_order
+    else:
+        g.pair_style = 0
+        g.eps, g.sigma = w.eps, w.sigma
+    g.dt, g.thermostat, g.T, g.tau = w.dt, w.thermostat, w.T, w.tau_T
+    g.nsteps = int(nsteps)
+    g._keep = keep
+    return g
+
+
+class Server:
+    """atvh_server: HBM-budget admission and continuous batching of MD segments."""
+
+    def __init__(self, ctx: Context, budget_bytes, chunk_steps=100, precision=_abi.PREC_MIXED):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = C.c_void_p()
+        _check(self.lib.atvh_server_create(ctx.h, int(budget_bytes), int(chunk_steps), precision, C.byref(h)),
+               ctx.h)
+        self.h = h
+        self.n = {}
+        _live_batches.add(self)
+
+    def close(self):
+        if self.h:
+            self.lib.atvh_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, w, nsteps):
+        g = segment_of(w, nsteps)
+        sid = C.c_int64()
+        _check(self.lib.atvh_server_submit(self.h, C.byref(g), C.byref(sid)), self.ctx.h)
+        self.n[sid.value] = w.n
+        return sid.value
+
+    def advance(self):
+        a = C.c_int()
+        q = C.c_int()
+        _check(self.lib.atvh_server_advance(self.h, C.byref(a), C.byref(q)), self.ctx.h)
+        return a.value, q.value
+
+    def poll(self):
+        ids = np.zeros(1024, np.int64)
+        k = C.c_int()
+        _check(self.lib.atvh_server_poll(self.h, lptr(ids), ids.size, C.byref(k)), self.ctx.h)
+        return [int(v) for v in ids[:k.value]]
+
+    def fetch(self, sid):
+        n = self.n.pop(sid)
+        out = {k: np.empty(n) for k in ("x", "y", "z", "vx", "vy", "vz")}
+        pe = C.c_double()
+        ke = C.c_double()
+        _check(self.lib.atvh_server_fetch(self.h, sid, *[dptr(out[k]) for k in ("x", "y", "z", "vx", "vy", "vz")],
+                                          C.byref(pe), C.byref(ke)), self.ctx.h)
+        out["pe"], out["ke"] = pe.value, ke.value
+        return out
+
+    def usage(self):
+        u = C.c_int64()
+        b = C.c_int64()
+        _check(self.lib.atvh_server_usage(self.h, C.byref(u), C.byref(b)), self.ctx.h)
+        return u.value, b.value
+
+
+def segment_bytes(w, nsteps=0):
+    return _abi.load().atvh_segment_bytes(C.byref(segment_of(w, nsteps)))

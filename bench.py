@@ -100,6 +100,57 @@ def charged_section(av, W, ctx, steps, warmup):
     return out
 
 
+def ladder_section(av, W, ctx, steps, loop_steps):
+    """Batched REMD-style ensemble: 256 LJ fluids (N log-uniform in [500, 20000],
+    T* ladder 0.7..1.5, NVT, mixed), one batched step per launch stage (hipGraph
+    replay) against a loop of plain per-system launches over the same members."""
+    ws = W.lj_ladder(256)
+    total = sum(w.n for w in ws)
+
+    def make():
+        out = []
+        for w in ws:
+            s = av.System(ctx, w.n, w.box)
+            W.upload(s, w)
+            W.configure(s, w)
+            s.nlist_build()
+            out.append(s)
+        return out
+
+    log(f"bench: ladder batched ({len(ws)} systems, {total} atoms), {steps} steps")
+    ms = make()
+    b = av.Batch(ctx, ms)
+    b.set_graph(True)
+    b.run(20, av.PREC_MIXED)  # warm (graph capture, list growth)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    b.run(steps, av.PREC_MIXED)
+    ctx.synchronize()
+    dt_b = time.perf_counter() - t0
+    b.close()
+    for s in ms:
+        s.close()
+    log(f"  batched: {dt_b * 1e3 / steps:.3f} ms/step, {total * steps / dt_b:.4e} atom-steps/s")
+    log(f"bench: ladder per-system loop, {loop_steps} steps")
+    ms = make()
+    for s in ms:
+        s.run(20, av.PREC_MIXED)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for s in ms:
+        s.run(loop_steps, av.PREC_MIXED)
+    ctx.synchronize()
+    dt_l = time.perf_counter() - t0
+    for s in ms:
+        s.close()
+    log(f"  loop: {dt_l * 1e3 / loop_steps:.3f} ms/step, {total * loop_steps / dt_l:.4e} atom-steps/s")
+    return {"workload": "lj_ladder_256 (N log-uniform 500..20000, T* 0.7..1.5, NVT, mixed)",
+            "atoms": total, "systems": len(ws),
+            "batched_atom_steps_per_s": total * steps / dt_b, "batched_steps": steps,
+            "loop_atom_steps_per_s": total * loop_steps / dt_l, "loop_steps": loop_steps,
+            "batched_over_loop": (total * steps / dt_b) / (total * loop_steps / dt_l)}
+
+
 def cpu_baseline(W, nsteps):
     """The CPU oracle (FP64 restatement of the reference CPU path) on the host
     cores: a bounded sample of the same workload."""
@@ -132,6 +183,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ladder-steps", type=int, default=200,
+                    help="timed steps of the batched 256-system ladder; 0 skips it")
     ap.add_argument("--charged-steps", type=int, default=100,
                     help="timed steps of the charged-melt (PME) side measurement; 0 skips it")
     args = ap.parse_args()
@@ -205,6 +258,8 @@ def main():
         r["s"].close()
     if args.charged_steps > 0:
         out["charged_melt"] = charged_section(av, W, ctx, args.charged_steps, 10)
+    if args.ladder_steps > 0:
+        out["batched_ladder"] = ladder_section(av, W, ctx, args.ladder_steps, max(20, args.ladder_steps // 5))
     if not args.no_cpu:
         log(f"bench: cpu baseline ({args.cpu_steps} oracle steps)")
         out["cpu_baseline"] = cpu_baseline(W, args.cpu_steps)

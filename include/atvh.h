@@ -38,6 +38,7 @@ extern "C" {
 typedef struct atvh_ctx atvh_ctx;
 typedef struct atvh_system atvh_system;
 typedef struct atvh_batch atvh_batch;
+typedef struct atvh_server atvh_server;
 
 /* ---- context --------------------------------------------------------------
  * Replaces cuda::Backend::init's stream set-up (src/cuda/driver/driver.cu:25-32)
@@ -59,6 +60,8 @@ int atvh_ctx_enable_timing(atvh_ctx* ctx, int enable);
 int atvh_ctx_kernel_time(atvh_ctx* ctx, const char* name, double* total_ms,
                          int64_t* launches);
 int atvh_ctx_reset_timing(atvh_ctx* ctx);
+/* Free and total device memory of the context's device (hipMemGetInfo). */
+int atvh_ctx_mem_info(atvh_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
 
 /* ---- frame (device state) ---------------------------------------------------
  * Replaces cuda::Frame + cpu::to_cuda / cuda::to_cpu
@@ -212,6 +215,65 @@ int atvh_batch_set_graph(atvh_batch* batch, int enable);
  * member's capacity overflow as ATVH_E_CAPACITY. */
 int atvh_batch_run(atvh_batch* batch, int nsteps, int precision,
                    int energy_every);
+
+/* ---- serving: MD segments with continuous batching ---------------------------------
+ * No counterpart in the reference (its run() is continuable and frames move
+ * through cpu::to_cuda / cuda::to_cpu, include/atomiverse/cuda/dynamics/
+ * md.cuh:130-178, src/cuda/frame/transfer.cu:145-475); this is the serving
+ * layer over that boundary. A segment is one system and a step count.
+ * Submitted segments are admitted in FIFO order while their estimated device
+ * footprint fits the HBM budget, stay resident until done, and advance
+ * together one chunk per atvh_server_advance (PairLJCut segments through one
+ * batched step, k-space segments through their own fused loop). A segment's
+ * result does not depend on what it ran beside.
+ */
+#define ATVH_PAIR_LJ 0
+#define ATVH_PAIR_LJCOUL 1
+
+typedef struct atvh_segment {
+    int64_t n;
+    double box[3];
+    /* frame-order host arrays, copied at submission; v*, charge, type may be NULL */
+    const double *x, *y, *z, *vx, *vy, *vz, *mass, *charge;
+    const int* type;
+    double cutoff, skin;            /* NeighborList(cutoff, skin, 1, 0, yes) */
+    int pair_style;                 /* ATVH_PAIR_LJ or ATVH_PAIR_LJCOUL */
+    double eps, sigma;              /* PairLJCut */
+    int ntypes;                     /* PairLJCutCoulLong coefficient table */
+    const double *eps_tab, *sigma_tab;
+    const int* set_tab;
+    double alpha;                   /* Ewald splitting (LJCOUL) */
+    double pme_spacing;             /* KSpacePME grid spacing; pme_order 0 = none */
+    int pme_order;
+    double dt;                      /* VelocityVerlet */
+    int thermostat;                 /* 0 NVE, 1 Nose-Hoover */
+    double T, tau;
+    int64_t nsteps;
+} atvh_segment;
+
+/* Admission estimate of a segment's device bytes. */
+int64_t atvh_segment_bytes(const atvh_segment* seg);
+int atvh_server_create(atvh_ctx* ctx, int64_t hbm_budget_bytes,
+                       int chunk_steps, int precision, atvh_server** out);
+int atvh_server_destroy(atvh_server* server);
+/* Queue a segment; *id identifies it. ATVH_E_CAPACITY if it alone exceeds
+ * the budget. */
+int atvh_server_submit(atvh_server* server, const atvh_segment* seg,
+                       int64_t* id);
+/* Admit what fits, advance every resident segment by one chunk (never past
+ * its own end), retire the finished ones (device memory released).
+ * Synchronises. */
+int atvh_server_advance(atvh_server* server, int* active, int* queued);
+/* Ids finished since the last poll (up to cap). */
+int atvh_server_poll(atvh_server* server, int64_t* ids, int cap, int* n);
+/* Final positions (wrapped) and velocities of a finished segment in frame
+ * order (any pointer may be NULL) and its last potential and kinetic energy;
+ * forgets the segment. */
+int atvh_server_fetch(atvh_server* server, int64_t id, double* x, double* y,
+                      double* z, double* vx, double* vy, double* vz,
+                      double* pe, double* ke);
+int atvh_server_usage(atvh_server* server, int64_t* bytes_in_use,
+                      int64_t* budget);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,106 @@
+"""Serving MD segments (atvh_server_*) through the C ABI: continuous
+batching under an HBM budget. The bar: every segment's final state equals,
+bit for bit, the same segment run alone with atvh_md_run."""
+import numpy as np
+import pytest
+
+import atomiverse_amd as av
+from atomiverse_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def requests():
+    """(workload, steps): LJ and charged (PME) segments of different sizes."""
+    return [
+        (W.lj_trimmed(1200, seed=11, T_star=0.8), 230),
+        (W.charged_melt(10, pme_spacing=0.83), 120),
+        (W.lj_trimmed(3100, seed=12), 310),
+        (W.lj_trimmed(1800, seed=13, T_star=1.1), 90),
+        (W.charged_melt(12, pme_spacing=0.83, seed=5), 170),
+        (W.lj_trimmed(2500, seed=14), 260),
+    ]
+
+
+def run_alone(ctx, w, nsteps, prec):
+    s = av.System(ctx, w.n, w.box)
+    W.upload(s, w)
+    W.configure(s, w)
+    s.nlist_build()
+    s.run(nsteps, prec)
+    d = s.download()
+    pe, _, ke = s.energy()
+    s.close()
+    return d, pe, ke
+
+
+def drain(srv, ids, max_rounds=10000):
+    out = {}
+    rounds = 0
+    while len(out) < len(ids):
+        srv.advance()
+        for sid in srv.poll():
+            out[sid] = srv.fetch(sid)
+        rounds += 1
+        assert rounds < max_rounds
+    return out
+
+
+@pytest.mark.parametrize("prec", [av.PREC_MIXED, av.PREC_FP64])
+def test_served_segments_equal_alone(hip_ctx, prec):
+    reqs = requests()
+    srv = av.Server(hip_ctx, budget_bytes=8 << 30, chunk_steps=50, precision=prec)
+    ids = [srv.submit(w, n) for w, n in reqs]
+    got = drain(srv, ids)
+    srv.close()
+    for sid, (w, n) in zip(ids, reqs):
+        d, pe, ke = run_alone(hip_ctx, w, n, prec)
+        for k in ("x", "y", "z", "vx", "vy", "vz"):
+            np.testing.assert_array_equal(got[sid][k], d[k], err_msg=f"{w.name} {k}")
+        assert got[sid]["pe"] == pe and got[sid]["ke"] == ke
+
+
+def test_admission_respects_the_budget(hip_ctx):
+    reqs = requests()
+    sizes = [av.segment_bytes(w) for w, _ in reqs]
+    budget = int(max(sizes) * 2.2)
+    srv = av.Server(hip_ctx, budget_bytes=budget, chunk_steps=40, precision=av.PREC_MIXED)
+    ids = [srv.submit(w, n) for w, n in reqs]
+    done = set()
+    peak = 0
+    while len(done) < len(ids):
+        active, queued = srv.advance()
+        used, b = srv.usage()
+        assert used <= b == budget
+        peak = max(peak, active)
+        for sid in srv.poll():
+            srv.fetch(sid)
+            done.add(sid)
+    assert 1 <= peak < len(reqs)
+    srv.close()
+
+
+def test_oversized_segment_is_refused(hip_ctx):
+    w = W.lj_trimmed(4000)
+    srv = av.Server(hip_ctx, budget_bytes=av.segment_bytes(w) // 2, chunk_steps=10)
+    with pytest.raises(av.HipError) as e:
+        srv.submit(w, 10)
+    assert e.value.code == av._abi.E_CAPACITY
+    srv.close()
+
+
+def test_footprint_estimate_bounds_the_allocation(hip_ctx):
+    """The admission estimate is at least what a built segment allocates (up
+    to 32 MiB of allocator granularity)."""
+    for w in (W.lj_trimmed(60000), W.charged_melt(30, pme_spacing=0.83)):
+        free0, _ = hip_ctx.mem_info()
+        s = av.System(hip_ctx, w.n, w.box)
+        W.upload(s, w)
+        W.configure(s, w)
+        s.nlist_build()
+        s.compute(av.PREC_MIXED, True, True)
+        s.compute(av.PREC_FP64, True, True)
+        free1, _ = hip_ctx.mem_info()
+        used = free0 - free1
+        assert used <= av.segment_bytes(w) + (32 << 20), (w.name, used, av.segment_bytes(w))
+        s.close()
