@@ -151,6 +151,80 @@ def ladder_section(av, W, ctx, steps, loop_steps):
             "batched_over_loop": (total * steps / dt_b) / (total * loop_steps / dt_l)}
 
 
+def serving_requests(W, count, seed=4242):
+    """Request set: 70% LJ / 30% charged melt with PME (grid ~1.2 points per
+    sigma, order 4), N log-uniform in [1000, 50000], steps uniform in [1000, 10000]."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for k in range(count):
+        n = int(np.exp(rng.uniform(np.log(1000), np.log(50000))))
+        steps = int(rng.integers(1000, 10001))
+        if rng.uniform() < 0.3:
+            side = max(10, 2 * int(round(n ** (1.0 / 3.0) / 2)))
+            w = W.charged_melt(side, pme_spacing=1.0 / 1.2, seed=9000 + k, name=f"req{k}_charged_{side ** 3}")
+        else:
+            w = W.lj_trimmed(n, seed=7000 + k, name=f"req{k}_lj_{n}")
+        reqs.append((w, steps))
+    return reqs
+
+
+def serving_section(av, W, ctx, count, chunk):
+    """Offline batched run of the request set (all submitted at once) against
+    the same set arriving as a seeded Poisson stream at the offline service
+    rate; throughput in atom-steps/s and completion latency percentiles."""
+    reqs = serving_requests(W, count)
+    work = sum(w.n * st for w, st in reqs)
+    budget = 64 << 30
+    log(f"bench: serving, {count} requests, {work:.3e} atom-steps, chunk {chunk}")
+
+    srv = av.Server(ctx, budget, chunk, av.PREC_MIXED)
+    t0 = time.perf_counter()
+    ids = [srv.submit(w, st) for w, st in reqs]
+    left = set(ids)
+    while left:
+        srv.advance()
+        for sid in srv.poll():
+            srv.fetch(sid)
+            left.discard(sid)
+    t_off = time.perf_counter() - t0
+    srv.close()
+    thr_off = work / t_off
+    log(f"  offline: {t_off:.2f} s, {thr_off:.4e} atom-steps/s")
+
+    lam = count / t_off  # arrivals per second = offline service rate
+    rng = np.random.default_rng(4243)
+    arrive = np.cumsum(rng.exponential(1.0 / lam, count))
+    srv = av.Server(ctx, budget, chunk, av.PREC_MIXED)
+    start = time.perf_counter()
+    k = 0
+    arrival_of = {}
+    latency = []
+    while len(latency) < count:
+        now = time.perf_counter() - start
+        while k < count and arrive[k] <= now:
+            sid = srv.submit(*reqs[k])
+            arrival_of[sid] = arrive[k]
+            k += 1
+        active, queued = srv.advance()
+        t = time.perf_counter() - start
+        for sid in srv.poll():
+            srv.fetch(sid)
+            latency.append(t - arrival_of[sid])
+        if active == 0 and queued == 0 and k < count:
+            time.sleep(max(0.0, arrive[k] - (time.perf_counter() - start)))
+    span = (time.perf_counter() - start) - arrive[0]
+    srv.close()
+    thr_srv = work / span
+    lat = np.array(latency)
+    log(f"  served: {span:.2f} s, {thr_srv:.4e} atom-steps/s, p50 {np.percentile(lat, 50):.3f} s, "
+        f"p99 {np.percentile(lat, 99):.3f} s")
+    return {"workload": f"{count} segments (70% LJ / 30% charged+PME, N 1e3..5e4, 1e3..1e4 steps, mixed)",
+            "atom_steps": work, "chunk_steps": chunk, "hbm_budget_bytes": budget,
+            "offline_atom_steps_per_s": thr_off, "served_atom_steps_per_s": thr_srv,
+            "served_over_offline": thr_srv / thr_off, "arrival_rate_per_s": lam,
+            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99))}
+
+
 def cpu_baseline(W, nsteps):
     """The CPU oracle (FP64 restatement of the reference CPU path) on the host
     cores: a bounded sample of the same workload."""
@@ -183,6 +257,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="requests of the serving measurement; 0 skips it")
     ap.add_argument("--ladder-steps", type=int, default=200,
                     help="timed steps of the batched 256-system ladder; 0 skips it")
     ap.add_argument("--charged-steps", type=int, default=100,
@@ -260,6 +336,8 @@ def main():
         out["charged_melt"] = charged_section(av, W, ctx, args.charged_steps, 10)
     if args.ladder_steps > 0:
         out["batched_ladder"] = ladder_section(av, W, ctx, args.ladder_steps, max(20, args.ladder_steps // 5))
+    if args.serve_requests > 0:
+        out["serving"] = serving_section(av, W, ctx, args.serve_requests, 250)
     if not args.no_cpu:
         log(f"bench: cpu baseline ({args.cpu_steps} oracle steps)")
         out["cpu_baseline"] = cpu_baseline(W, args.cpu_steps)
