@@ -28,9 +28,10 @@ def snapshot(nside, jitter=0.06, seed=3):
     like configuration: the perfect lattice has near-zero forces)."""
     w = W.charged_melt(nside)
     rng = np.random.default_rng(seed)
-    w.x = w.x + rng.normal(0, jitter, w.n)
-    w.y = w.y + rng.normal(0, jitter, w.n)
-    w.z = w.z + rng.normal(0, jitter, w.n)
+    # wrapped into [0, L): the reference builds its list from wrapped positions
+    w.x = np.mod(w.x + rng.normal(0, jitter, w.n), w.box[0])
+    w.y = np.mod(w.y + rng.normal(0, jitter, w.n), w.box[1])
+    w.z = np.mod(w.z + rng.normal(0, jitter, w.n), w.box[2])
     return w
 
 

@@ -366,7 +366,8 @@ def _melt_snapshot(nside=6, seed=9):
     rng = np.random.default_rng(seed)
     a = 1.03574
     i, j, k = np.meshgrid(np.arange(nside), np.arange(nside), np.arange(nside), indexing="ij")
-    x, y, z = (c.reshape(-1) * a + rng.normal(0, 0.05, nside ** 3) for c in (i, j, k))
+    L = nside * a
+    x, y, z = (np.mod(c.reshape(-1) * a + rng.normal(0, 0.05, nside ** 3), L) for c in (i, j, k))
     t = ((i + j + k).reshape(-1) % 2).astype(np.int32)
     q = np.where(t == 0, 1.0, -1.0) / math.sqrt(COUL)
     return x, y, z, q, t, np.full(3, nside * a)
