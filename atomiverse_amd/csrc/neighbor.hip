@@ -1413,6 +1413,125 @@ int atvh_nlist_build(atvh_system* s) {
     API_CATCH(c)
 }
 
+int atvh_nlist_brick_shape(atvh_system* s, int* candidate, int* bdim) {
+    if (!s)
+        return ATVH_E_ARG;
+    if (candidate)
+        *candidate = s->brick_mode ? s->brick_pick : -1;
+    if (bdim)
+        for (int a = 0; a < 3; ++a)
+            bdim[a] = s->bdim[a];
+    return ATVH_OK;
+}
+
+int atvh_nlist_set_brick_shape(atvh_system* s, int candidate) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    if (!s->nl_configured)
+        throw Error(ATVH_E_STATE, "atvh_nlist_set_brick_shape: neighbor list not configured");
+    if (candidate < 0 || candidate > 10)
+        throw Error(ATVH_E_ARG, "atvh_nlist_set_brick_shape: candidate out of range [0, 10]");
+    ATVH_HIP(hipSetDevice(c->device));
+    ATVH_HIP(hipStreamSynchronize(c->stream));
+    if (s->brick_mode) {
+        s->brick_pick_min = candidate;
+        nl_setup_grid(s);
+        s->nl_built = false;
+    }
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+// Brick-shape autotuner: for the current shape and the next two smaller
+// valid ones, build the list, time one rebuild and `trials` pair launches
+// with HIP events, and keep the shape of least (pair + rebuild / 8) time
+// (a rebuild every ~8 steps is typical at skin 0.3). The forces of the frame
+// are restored after every timing; the list is rebuilt with the winner and
+// the rebuild counter restarts at 1.
+int atvh_nlist_autotune(atvh_system* s, int precision, int trials, int* chosen, double* best_us) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    if (!s->nl_configured || !s->pair_configured)
+        throw Error(ATVH_E_STATE, "atvh_nlist_autotune: configure the neighbor list and the pair style first");
+    if (trials < 1)
+        throw Error(ATVH_E_ARG, "atvh_nlist_autotune: trials must be >= 1");
+    ATVH_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    if (!s->brick_mode) {
+        if (chosen)
+            *chosen = -1;
+        return ATVH_OK;
+    }
+    const int k0 = s->brick_pick;
+    double* save = nullptr;
+    dev_alloc(&save, size_t(3) * s->n);
+    hipEvent_t ev[4];
+    for (auto& e : ev)
+        ATVH_HIP(hipEventCreate(&e));
+    int best = k0;
+    double best_t = 1e300;
+    std::vector<int> tried;
+    try {
+        for (int k = k0; k <= std::min(k0 + 2, 10); ++k) {
+            s->brick_pick_min = k;
+            nl_setup_grid(s);
+            if (std::find(tried.begin(), tried.end(), s->brick_pick) != tried.end())
+                continue;
+            tried.push_back(s->brick_pick);
+            s->nl_built = false;
+            if (atvh_nlist_build(s) != ATVH_OK)
+                continue;
+            ATVH_HIP(hipEventRecord(ev[0], st));
+            nl_enqueue_rebuild(s, &s->ctl->always, true);
+            ATVH_HIP(hipEventRecord(ev[1], st));
+            const size_t nb = size_t(s->n) * sizeof(double);
+            ATVH_HIP(hipMemcpyAsync(save, s->fx, nb, hipMemcpyDeviceToDevice, st));
+            ATVH_HIP(hipMemcpyAsync(save + s->n, s->fy, nb, hipMemcpyDeviceToDevice, st));
+            ATVH_HIP(hipMemcpyAsync(save + 2 * s->n, s->fz, nb, hipMemcpyDeviceToDevice, st));
+            ATVH_HIP(hipEventRecord(ev[2], st));
+            for (int t = 0; t < trials; ++t)
+                pair_enqueue(s, precision, false, false);
+            ATVH_HIP(hipEventRecord(ev[3], st));
+            ATVH_HIP(hipMemcpyAsync(s->fx, save, nb, hipMemcpyDeviceToDevice, st));
+            ATVH_HIP(hipMemcpyAsync(s->fy, save + s->n, nb, hipMemcpyDeviceToDevice, st));
+            ATVH_HIP(hipMemcpyAsync(s->fz, save + 2 * s->n, nb, hipMemcpyDeviceToDevice, st));
+            ATVH_HIP(hipStreamSynchronize(st));
+            float t_rb = 0.f, t_pair = 0.f;
+            ATVH_HIP(hipEventElapsedTime(&t_rb, ev[0], ev[1]));
+            ATVH_HIP(hipEventElapsedTime(&t_pair, ev[2], ev[3]));
+            const double cost = 1e3 * (double(t_pair) / trials + double(t_rb) / 8.0);
+            if (cost < best_t) {
+                best_t = cost;
+                best = s->brick_pick;
+            }
+        }
+    } catch (...) {
+        for (auto& e : ev)
+            (void)hipEventDestroy(e);
+        dev_free(save);
+        throw;
+    }
+    for (auto& e : ev)
+        (void)hipEventDestroy(e);
+    dev_free(save);
+    s->brick_pick_min = best;
+    nl_setup_grid(s);
+    s->nl_built = false;
+    const int rc = atvh_nlist_build(s);
+    if (rc != ATVH_OK)
+        return rc;
+    if (chosen)
+        *chosen = best;
+    if (best_us)
+        *best_us = best_t;
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
 int atvh_nlist_decide_and_rebuild(atvh_system* s) {
     if (!s)
         return ATVH_E_ARG;

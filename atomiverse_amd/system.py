@@ -146,6 +146,22 @@ class System:
     def nlist_decide_and_rebuild(self):
         self._ck(self.lib.atvh_nlist_decide_and_rebuild(self.h))
 
+    def brick_shape(self):
+        k = C.c_int()
+        b = np.zeros(3, np.int32)
+        self._ck(self.lib.atvh_nlist_brick_shape(self.h, C.byref(k), iptr(b)))
+        return k.value, tuple(int(v) for v in b)
+
+    def set_brick_shape(self, candidate):
+        self._ck(self.lib.atvh_nlist_set_brick_shape(self.h, int(candidate)))
+
+    def autotune(self, precision=_abi.PREC_MIXED, trials=3):
+        """Pick the fastest brick shape on this system; returns (candidate, microseconds)."""
+        k = C.c_int()
+        us = C.c_double()
+        self._ck(self.lib.atvh_nlist_autotune(self.h, precision, int(trials), C.byref(k), C.byref(us)))
+        return k.value, us.value
+
     def nlist_stats(self):
         nr = C.c_int64()
         npairs = C.c_int64()

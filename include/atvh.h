@@ -103,6 +103,20 @@ int atvh_nlist_build(atvh_system* sys);
 /* LAMMPS Neighbor::decide on the device (++ago, displacement check) followed
  * by the rebuild it decides; no host synchronisation. */
 int atvh_nlist_decide_and_rebuild(atvh_system* sys);
+/* Brick layout of the list (this backend's tiling of the reference cell
+ * grid): the candidate index in use (0 = largest brick, 10 = one cell; -1
+ * for the small-grid layout) and its size in cells per axis. */
+int atvh_nlist_brick_shape(atvh_system* sys, int* candidate, int* bdim);
+/* Use candidate k or the next smaller valid one; the list must be built
+ * again (atvh_nlist_build). */
+int atvh_nlist_set_brick_shape(atvh_system* sys, int candidate);
+/* Autotuner: time the current brick shape and the next two smaller ones on
+ * this system (one rebuild + `trials` pair launches each, HIP events), keep
+ * the fastest and rebuild the list with it. The frame's forces are kept;
+ * the rebuild counter restarts. *chosen = candidate (-1: nothing to tune),
+ * *best_us = its pair + rebuild/8 time in microseconds. */
+int atvh_nlist_autotune(atvh_system* sys, int precision, int trials,
+                        int* chosen, double* best_us);
 /* Host read of the rebuild count (synchronises the stream). */
 int atvh_nlist_stats(atvh_system* sys, int64_t* nrebuilds, int64_t* npairs_half,
                      int* max_neighbors, int* capacity);
