@@ -66,6 +66,8 @@ SIGNATURES = {
     "atvh_batch_destroy": (C.c_int, [_vp]),
     "atvh_batch_set_graph": (C.c_int, [_vp, C.c_int]),
     "atvh_batch_run": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "atvh_batch_exchange": (C.c_int, [_vp, C.c_int64, C.c_uint64, _ip]),
+    "atvh_batch_ladder": (C.c_int, [_vp, _ip, _dp]),
     "atvh_segment_bytes": (C.c_int64, [_vp]),
     "atvh_server_create": (C.c_int, [_vp, C.c_int64, C.c_int, C.c_int, C.POINTER(_vp)]),
     "atvh_server_destroy": (C.c_int, [_vp]),

@@ -56,6 +56,9 @@ struct DevCtl {
     long long nbr_total; // full-list entries of the last build
     double e_kspace[3];  // last k-space energy parts: reciprocal, self, net charge
     int rec_step;        // next thermo record index of a batched run
+    double nh_Q;         // Nose-Hoover mass Q = Ndof kB T tau^2
+    double nh_ndof_kT;   // Ndof kB T
+    double T_target;     // thermostat temperature (moves with replica exchanges)
 };
 
 struct CoulState; // PairLJCutCoulLong tables (coul_pme.hip)
@@ -198,6 +201,8 @@ struct atvh_batch {
     hipGraphExec_t graph[2] = {nullptr, nullptr}; // [energy step?]
     int graph_prec[2] = {-1, -1};
     int thermo_cap = 0;
+    int* d_slot_member = nullptr; // replica exchange: member holding ladder slot k
+    int* d_accept = nullptr;      // accepted exchanges of the last round
 };
 
 namespace atvh {

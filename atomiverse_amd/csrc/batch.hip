@@ -29,6 +29,57 @@ __global__ void k_batch_reset(DevCtl** ctls, int nsys) {
         ctls[k]->rec_step = 0;
 }
 
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// One exchange round (Remd::exchange_round_, src/cpu/path/remd.cpp:105-151)
+// between the members holding adjacent ladder slots (k, k+1), k = parity,
+// parity + 2, ...: accept with min(1, exp((beta_k - beta_k+1)(U_k - U_k+1)))
+// against a counter-based uniform u(seed, round, k). The reference swaps
+// configurations and rescales velocities so each slot keeps its temperature;
+// here the two members swap their thermostat (T, Q, Ndof kB T, xi) instead —
+// the same exchange with configurations staying put — and each member's
+// velocities are rescaled by sqrt(T_new / T_old) through its pending NH scale
+// (the scale is applied in the next kick; the kinetic energy is rescaled now).
+__global__ void k_batch_exchange(DevCtl** ctls, int* slot_member, int nslots, int parity,
+                                 unsigned long long seed, unsigned long long round, int* accepted) {
+    if (blockIdx.x != 0 || threadIdx.x != 0)
+        return;
+    int acc = 0;
+    for (int k = parity; k + 1 < nslots; k += 2) {
+        DevCtl* ci = ctls[slot_member[k]];
+        DevCtl* cj = ctls[slot_member[k + 1]];
+        const double bi = 1.0 / (kB_ev_per_k * ci->T_target);
+        const double bj = 1.0 / (kB_ev_per_k * cj->T_target);
+        const double arg = (bi - bj) * (ci->pe - cj->pe);
+        const double p = arg >= 0.0 ? 1.0 : exp(arg);
+        const unsigned long long r = splitmix64(seed ^ splitmix64((round << 32) ^ static_cast<unsigned long long>(k)));
+        const double u = static_cast<double>(r >> 11) * (1.0 / 9007199254740992.0);
+        if (u < p) {
+            ++acc;
+            const double Ti = ci->T_target, Tj = cj->T_target;
+            const double si = sqrt(Tj / Ti), sj = sqrt(Ti / Tj);
+            ci->nh_scale *= si;
+            ci->ke *= si * si;
+            cj->nh_scale *= sj;
+            cj->ke *= sj * sj;
+            double t;
+            t = ci->T_target; ci->T_target = cj->T_target; cj->T_target = t;
+            t = ci->nh_Q; ci->nh_Q = cj->nh_Q; cj->nh_Q = t;
+            t = ci->nh_ndof_kT; ci->nh_ndof_kT = cj->nh_ndof_kT; cj->nh_ndof_kT = t;
+            t = ci->xi; ci->xi = cj->xi; cj->xi = t;
+            const int m = slot_member[k];
+            slot_member[k] = slot_member[k + 1];
+            slot_member[k + 1] = m;
+        }
+    }
+    *accepted = acc;
+}
+
 void free_graphs(atvh_batch* b) {
     for (int k = 0; k < 2; ++k) {
         if (b->graph[k])
@@ -119,8 +170,16 @@ int atvh_batch_create(atvh_ctx* ctx, atvh_system* const* systems, int nsys, atvh
         }
         dev_alloc(&b->d_ctls, ctls.size());
         ATVH_HIP(hipMemcpy(b->d_ctls, ctls.data(), ctls.size() * sizeof(DevCtl*), hipMemcpyHostToDevice));
+        std::vector<int> slots(nsys);
+        for (int k = 0; k < nsys; ++k)
+            slots[k] = k;
+        dev_alloc(&b->d_slot_member, slots.size());
+        ATVH_HIP(hipMemcpy(b->d_slot_member, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice));
+        dev_alloc(&b->d_accept, 1);
     } catch (...) {
         dev_free(b->d_ctls);
+        dev_free(b->d_slot_member);
+        dev_free(b->d_accept);
         delete b;
         throw;
     }
@@ -139,6 +198,8 @@ int atvh_batch_destroy(atvh_batch* b) {
     free_graphs(b);
     batch_free_stages(b);
     dev_free(b->d_ctls);
+    dev_free(b->d_slot_member);
+    dev_free(b->d_accept);
     c->live_systems -= 1;
     delete b;
     return ATVH_OK;
@@ -217,6 +278,53 @@ int atvh_batch_run(atvh_batch* b, int nsteps, int precision, int energy_every) {
         ATVH_HIP(hipMemcpy(&h, s->ctl, sizeof(h), hipMemcpyDeviceToHost));
         nl_check_ctl(h);
     }
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+int atvh_batch_exchange(atvh_batch* b, int64_t round, uint64_t seed, int* accepted) {
+    if (!b)
+        return ATVH_E_ARG;
+    atvh_ctx* c = b->ctx;
+    API_TRY
+    if (round < 0)
+        throw Error(ATVH_E_ARG, "atvh_batch_exchange: round < 0");
+    for (atvh_system* s : b->sys) {
+        if (s->n != b->sys[0]->n)
+            throw Error(ATVH_E_ARG, "atvh_batch_exchange: replicas must hold the same atom count (an exchange "
+                                    "swaps one configuration for another)");
+        if (!s->integ_configured || s->thermostat != 1)
+            throw Error(ATVH_E_STATE, "atvh_batch_exchange: every replica needs the Nose-Hoover thermostat");
+    }
+    ATVH_HIP(hipSetDevice(c->device));
+    const int nsys = static_cast<int>(b->sys.size());
+    k_batch_exchange<<<1, 64, 0, c->stream>>>(b->d_ctls, b->d_slot_member, nsys, static_cast<int>(round & 1),
+                                              static_cast<unsigned long long>(seed),
+                                              static_cast<unsigned long long>(round), b->d_accept);
+    ATVH_HIP(hipGetLastError());
+    if (accepted) {
+        ATVH_HIP(hipMemcpyAsync(accepted, b->d_accept, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        ATVH_HIP(hipStreamSynchronize(c->stream));
+    }
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+int atvh_batch_ladder(atvh_batch* b, int* member_of_slot, double* T_of_member) {
+    if (!b)
+        return ATVH_E_ARG;
+    atvh_ctx* c = b->ctx;
+    API_TRY
+    const int nsys = static_cast<int>(b->sys.size());
+    ATVH_HIP(hipStreamSynchronize(c->stream));
+    if (member_of_slot)
+        ATVH_HIP(hipMemcpy(member_of_slot, b->d_slot_member, nsys * sizeof(int), hipMemcpyDeviceToHost));
+    if (T_of_member)
+        for (int k = 0; k < nsys; ++k) {
+            DevCtl h{};
+            ATVH_HIP(hipMemcpy(&h, b->sys[k]->ctl, sizeof(h), hipMemcpyDeviceToHost));
+            T_of_member[k] = h.T_target;
+        }
     return ATVH_OK;
     API_CATCH(c)
 }

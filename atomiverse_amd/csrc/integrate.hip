@@ -165,7 +165,7 @@ struct PostArgs {
     const double* ke_partials;
     int nb;
     int thermostat;
-    double dt, Q, ndof_kT; // ndof * kB * T
+    double dt;
     // decide (initial phase only)
     int decide, every, delay, check;
     double* record_ke; // may be null
@@ -184,7 +184,8 @@ struct OpPost {
         const double ke = 0.5 * mass_v2_to_energy * mv2;
         double scale = 1.0;
         if (a.thermostat) {
-            c->xi += (a.dt * 0.5) * (2.0 * ke - a.ndof_kT) / a.Q;
+            // Q and Ndof kB T live in the control block (a replica exchange moves them)
+            c->xi += (a.dt * 0.5) * (2.0 * ke - c->nh_ndof_kT) / c->nh_Q;
             scale = exp(-a.dt * 0.5 * c->xi);
             c->nh_scale = scale;
         }
@@ -299,9 +300,6 @@ PostArgs make_post_args(atvh_system* s, bool final_phase, bool decide) {
     p.nb = std::min(grid_for(s->n), kIntGrid);
     p.thermostat = s->thermostat;
     p.dt = s->dt;
-    // NoseHoover::build / launch: Q = Ndof kB T tau^2
-    p.Q = static_cast<double>(s->ndof) * kB_ev_per_k * s->T_target * s->tau_T * s->tau_T;
-    p.ndof_kT = static_cast<double>(s->ndof) * kB_ev_per_k * s->T_target;
     p.decide = decide ? 1 : 0;
     p.every = s->every;
     p.delay = s->delay;
@@ -444,6 +442,15 @@ int atvh_integrator_configure(atvh_system* s, double dt, int thermostat, double 
     s->tau_T = tau_T;
     // publish_global_dof (src/cpu/driver/dof.cpp:49): Ndof = 3N - 3
     s->ndof = ndof > 0 ? ndof : 3 * s->n - 3;
+    ATVH_HIP(hipSetDevice(c->device));
+    ATVH_HIP(hipStreamSynchronize(c->stream));
+    DevCtl h{};
+    ATVH_HIP(hipMemcpy(&h, s->ctl, sizeof(h), hipMemcpyDeviceToHost));
+    // NoseHoover::build: Q = Ndof kB T tau^2
+    h.nh_Q = static_cast<double>(s->ndof) * kB_ev_per_k * T_target * tau_T * tau_T;
+    h.nh_ndof_kT = static_cast<double>(s->ndof) * kB_ev_per_k * T_target;
+    h.T_target = T_target;
+    ATVH_HIP(hipMemcpy(s->ctl, &h, sizeof(h), hipMemcpyHostToDevice));
     s->integ_configured = true;
     return ATVH_OK;
     API_CATCH(c)

@@ -280,6 +280,20 @@ class Batch:
     def run(self, nsteps, precision=_abi.PREC_FP64, energy_every=0):
         _check(self.lib.atvh_batch_run(self.h, int(nsteps), precision, int(energy_every)), self.ctx.h)
 
+    def exchange(self, round_index, seed):
+        """One replica-exchange round; returns the number of accepted swaps."""
+        acc = C.c_int()
+        _check(self.lib.atvh_batch_exchange(self.h, int(round_index), int(seed), C.byref(acc)), self.ctx.h)
+        return acc.value
+
+    def ladder(self):
+        """(member holding each ladder slot, thermostat T of each member)."""
+        n = len(self.systems)
+        slots = np.zeros(n, np.int32)
+        T = np.zeros(n)
+        _check(self.lib.atvh_batch_ladder(self.h, iptr(slots), dptr(T)), self.ctx.h)
+        return slots, T
+
 
 class Segment(C.Structure):
     """atvh_segment (include/atvh.h)."""

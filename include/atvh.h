@@ -229,6 +229,22 @@ int atvh_batch_set_graph(atvh_batch* batch, int enable);
  * member's capacity overflow as ATVH_E_CAPACITY. */
 int atvh_batch_run(atvh_batch* batch, int nsteps, int precision,
                    int energy_every);
+/* One replica-exchange round on the device (Remd::exchange_round_,
+ * src/cpu/path/remd.cpp:105-151): members are copies of one system (same
+ * atom count) with Nose-Hoover thermostats, created in increasing ladder
+ * temperature; the members holding adjacent ladder slots (k, k+1), k even
+ * for even rounds and odd for odd rounds, exchange with probability
+ * min(1, exp((1/kT_k - 1/kT_k+1)(U_k - U_k+1))) using the potential energies
+ * of the last batched step and a counter-based uniform u(seed, round, k).
+ * An accepted exchange moves the thermostats (T, Q, xi) between the two
+ * members and rescales each one's velocities by sqrt(T_new/T_old); the
+ * configurations stay in their systems. *accepted = accepted swaps. */
+int atvh_batch_exchange(atvh_batch* batch, int64_t round, uint64_t seed,
+                        int* accepted);
+/* The member holding each ladder slot, and each member's current
+ * thermostat temperature (either pointer may be NULL; synchronises). */
+int atvh_batch_ladder(atvh_batch* batch, int* member_of_slot,
+                      double* T_of_member);
 
 /* ---- serving: MD segments with continuous batching ---------------------------------
  * No counterpart in the reference (its run() is continuable and frames move

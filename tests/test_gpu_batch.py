@@ -94,3 +94,65 @@ def test_batch_rejects_unsupported_members(hip_ctx):
     b.close()
     with pytest.raises(av.HipError):
         av.Batch(hip_ctx, [good, good])
+
+
+M64 = (1 << 64) - 1
+
+
+def _splitmix64(x):
+    x = (x + 0x9E3779B97F4A7C15) & M64
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & M64
+    return x ^ (x >> 31)
+
+
+def test_replica_exchange_on_device(hip_ctx):
+    """Remd::exchange_round_ (remd.cpp:105-151) on the device: decisions equal
+    the Metropolis rule evaluated on the host from the members' energies with
+    the same counter-based uniforms; temperatures stay a permutation of the
+    ladder and every member's kinetic energy is rescaled by T_new/T_old."""
+    kb = W.KB
+    base = W.lj_fcc(7)
+    ladder = [0.70, 0.72, 0.74, 0.76, 0.78, 0.80]
+    ms = []
+    for k, t in enumerate(ladder):
+        s = av.System(hip_ctx, base.n, base.box)
+        vx, vy, vz = W.init_velocity(base.mass, t / kb, 87287 + k)
+        s.upload(base.x, base.y, base.z, base.mass, vx, vy, vz)
+        s.nlist_configure(2.5, 0.3, 1, 0, True)
+        s.pair_lj(1.0, 1.0, 2.5)
+        s.integrator(base.dt, 1, t / kb, base.tau_T, 0)
+        ms.append(s)
+    b = av.Batch(hip_ctx, ms)
+    seed = 4242
+    total = 0
+    for rnd in range(6):
+        b.run(40, av.PREC_MIXED)
+        slots, T = b.ladder()
+        pe = np.array([s.energy()[0] for s in ms])
+        ke = np.array([s.energy()[2] for s in ms])
+        # host restatement of the round
+        want_slots = slots.copy()
+        want_T = T.copy()
+        acc = 0
+        for k in range(rnd & 1, len(ladder) - 1, 2):
+            i, j = want_slots[k], want_slots[k + 1]
+            arg = (1 / (kb * want_T[i]) - 1 / (kb * want_T[j])) * (pe[i] - pe[j])
+            p = 1.0 if arg >= 0 else np.exp(arg)
+            r = _splitmix64(seed ^ _splitmix64(((rnd << 32) ^ k) & M64))
+            u = (r >> 11) * (1.0 / 9007199254740992.0)
+            if u < p:
+                acc += 1
+                want_T[i], want_T[j] = want_T[j], want_T[i]
+                want_slots[k], want_slots[k + 1] = j, i
+        got = b.exchange(rnd, seed)
+        total += got
+        slots2, T2 = b.ladder()
+        assert got == acc
+        np.testing.assert_array_equal(slots2, want_slots)
+        np.testing.assert_allclose(T2, want_T, rtol=0, atol=0)
+        assert sorted(T2) == sorted(t / kb for t in ladder)
+        ke2 = np.array([s.energy()[2] for s in ms])
+        np.testing.assert_allclose(ke2, ke * T2 / T, rtol=1e-12)
+    assert total > 0
+    b.close()
