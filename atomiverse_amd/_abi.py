@@ -76,6 +76,7 @@ SIGNATURES = {
     "atvh_server_poll": (C.c_int, [_vp, _lp, C.c_int, _ip]),
     "atvh_server_fetch": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "atvh_server_usage": (C.c_int, [_vp, _lp, _lp]),
+    "atvh_util_stream_copy": (C.c_int, [_vp, C.c_int64, C.c_int, _dp]),
     "atvh_util_fcc_lattice": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int64, _dp, _dp, _dp]),
     "atvh_util_rocksalt_lattice": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _ip,
                                                _dp, C.c_double]),

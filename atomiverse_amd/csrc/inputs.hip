@@ -11,7 +11,55 @@
 #include <cmath>
 #include <random>
 
+namespace {
+
+// Plain streaming copy (16-byte vector loads and stores): the reference
+// kernel for calibrating HBM byte counters against a known traffic.
+__global__ __launch_bounds__(256) void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                     int64_t n16) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+} // namespace
+
 extern "C" {
+
+int atvh_util_stream_copy(atvh_ctx* c, int64_t bytes, int iters, double* avg_ms) {
+    if (!c || bytes < 16 || iters < 1)
+        return ATVH_E_ARG;
+    try {
+        ATVH_HIP(hipSetDevice(c->device));
+        const int64_t n16 = bytes / 16;
+        uint4 *src = nullptr, *dst = nullptr;
+        atvh::dev_alloc(&src, n16);
+        atvh::dev_alloc(&dst, n16);
+        hipStream_t st = c->stream;
+        ATVH_HIP(hipMemsetAsync(src, 1, n16 * 16, st));
+        hipEvent_t e0, e1;
+        ATVH_HIP(hipEventCreate(&e0));
+        ATVH_HIP(hipEventCreate(&e1));
+        const int grid = 256 * 16;
+        k_stream_copy<<<grid, 256, 0, st>>>(src, dst, n16); // warm
+        ATVH_HIP(hipEventRecord(e0, st));
+        for (int k = 0; k < iters; ++k)
+            k_stream_copy<<<grid, 256, 0, st>>>(src, dst, n16);
+        ATVH_HIP(hipEventRecord(e1, st));
+        ATVH_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        ATVH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        atvh::dev_free(src);
+        atvh::dev_free(dst);
+        if (avg_ms)
+            *avg_ms = double(ms) / iters;
+        return ATVH_OK;
+    } catch (const atvh::Error& e) {
+        c->err = e.what();
+        return e.code;
+    }
+}
 
 int64_t atvh_util_fcc_lattice(int nx, int ny, int nz, double a, int64_t nmax, double* x, double* y,
                               double* z) {

@@ -23,6 +23,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# HBM bytes per launch of the pair kernel at this workload, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE passes scaled by a plain copy's counters (the file
+# records the commands and numbers); absent -> traffic null.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_lj_brick_mixed_lj_fcc_2048000.txt")
+
+
+def measured_traffic():
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -263,6 +278,8 @@ def main():
                     help="timed steps of the batched 256-system ladder; 0 skips it")
     ap.add_argument("--charged-steps", type=int, default=100,
                     help="timed steps of the charged-melt (PME) side measurement; 0 skips it")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB device copy (the reference for byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one MI355X; multi-GPU runs are not part of it "
@@ -273,6 +290,13 @@ def main():
     from atomiverse_amd import workloads as W
 
     ctx = av.Context(0)
+    if args.calibrate_copy:
+        import ctypes as C
+        ms = C.c_double()
+        rc = av.load().atvh_util_stream_copy(ctx.h, 1 << 30, 3, C.byref(ms))
+        if rc != 0:
+            raise av.HipError(rc, "atvh_util_stream_copy")
+        log(f"bench: copy calibration, 1 GiB per launch, {ms.value:.3f} ms, {2 * (1 << 30) / ms.value / 1e6:.1f} GB/s")
     res = {}
     for name, prec in (("fp64", av.PREC_FP64), ("mixed", av.PREC_MIXED)):
         log(f"bench: {name} run, {args.warmup} warmup + {args.steps} timed steps")
@@ -325,7 +349,7 @@ def main():
         "mixed_over_fp64": value / fp64_value,
         "force_rms_rel_err_mixed_vs_fp64": rms,
         "roofline": {"bound": "hbm", "kernel": "k_lj_brick_mixed", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "bytes_alg_per_launch": bytes_alg, "avg_launch_us": avg_s * 1e6},
         "kernel_avg_us": {name: {k: v[0] * 1e3 / max(v[1], 1) for k, v in r["kt"].items()}
                           for name, r in res.items()},

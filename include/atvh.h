@@ -194,6 +194,12 @@ int atvh_md_run(atvh_system* sys, int nsteps, int precision, int energy_every);
  * (entries for steps without an energy request hold NaN). Synchronises. */
 int atvh_md_thermo(atvh_system* sys, double* pe, double* ke, int nsteps);
 
+/* Measurement aid: a plain 16-byte-vector copy of `bytes` between two
+ * device buffers, launched iters times on the context stream; *avg_ms =
+ * mean kernel time. Its byte counters calibrate the HBM traffic counters. */
+int atvh_util_stream_copy(atvh_ctx* ctx, int64_t bytes, int iters,
+                          double* avg_ms);
+
 /* ---- synthetic inputs (host only) ---------------------------------------------------
  * Lattice generators for the benchmark workloads and InitVelocity::build
  * (src/cpu/dynamics/init.cpp:107-215): mt19937 + normal(0,1)/sqrt(m), COM
