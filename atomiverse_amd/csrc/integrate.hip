@@ -539,6 +539,15 @@ int atvh_md_run(atvh_system* s, int nsteps, int precision, int energy_every) {
         k_single<OpPost><<<1, kBlock, 0, st>>>(p_fin);
         ATVH_HIP(hipGetLastError());
     }
+    // a neighbor capacity or halo overflow during the run means skipped work:
+    // report it (the list build that follows a failed call resizes)
+    ATVH_HIP(hipStreamSynchronize(st));
+    DevCtl h{};
+    ATVH_HIP(hipMemcpy(&h, s->ctl, sizeof(h), hipMemcpyDeviceToHost));
+    if (h.overflow || h.halo_overflow) {
+        s->nl_built = false;
+        nl_check_ctl(h);
+    }
     return ATVH_OK;
     API_CATCH(c)
 }

@@ -8,6 +8,7 @@
 #include <cstdlib>
 
 #include "atomiverse/hip/md.hpp"
+#include "atomiverse/hip/remd.hpp"
 
 using namespace ATV;
 
@@ -113,6 +114,29 @@ int main() {
     }
     std::printf("hip::md::MD charged melt: %zu ions, PE/N %.6f, per-phase == fused (PME)\n", c.nlocal(),
                 c.energy / c.nlocal());
+    // replica exchange over a 4-temperature ladder (cpu::path::Remd's interface)
+    {
+        const double a = std::cbrt(4.0 / 0.8442);
+        const int nc = 6;
+        hip::HostFrame f;
+        const int64_t n = atvh_util_fcc_lattice(nc, nc, nc, a, -1, nullptr, nullptr, nullptr);
+        f.resize(static_cast<size_t>(n));
+        atvh_util_fcc_lattice(nc, nc, nc, a, -1, f.x.data(), f.y.data(), f.z.data());
+        f.box = {nc * a, nc * a, nc * a};
+        const double kB = 8.617333262e-5, tau = std::sqrt(103.6427);
+        hip::path::Remd remd({0.70 / kB, 0.74 / kB, 0.78 / kB, 0.82 / kB}, 0.005 * tau, 25, 7);
+        remd.thermostat_tau = 0.5 * tau;
+        remd.set_neighbor(hip::NeighborList{2.5, 0.3, 1, 0, true});
+        remd.set_evaluator(hip::PairLJCut{1.0, 1.0, 2.5});
+        remd.set_precision(hip::Precision::mixed);
+        remd.run(f, 8);
+        if (remd.exchange_attempts() != 12 || !std::isfinite(f.energy) || !(f.energy < 0.0)) {
+            std::fprintf(stderr, "remd: attempts %d energy %g\n", remd.exchange_attempts(), f.energy);
+            return 1;
+        }
+        std::printf("hip::path::Remd: %d attempts, %d accepted, lowest-T replica PE/N %.6f\n",
+                    remd.exchange_attempts(), remd.exchange_accepts(), f.energy / f.nlocal());
+    }
     // a pair evaluator without a neighbor list is rejected, as cuda::md::MD does
     try {
         hip::md::MD bad;

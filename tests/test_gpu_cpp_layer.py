@@ -19,3 +19,4 @@ def test_hip_md_cpp_consumer():
     assert out.returncode == 0, out.stderr
     assert "per-phase == fused" in out.stdout
     assert "per-phase == fused (PME)" in out.stdout
+    assert "hip::path::Remd: 12 attempts" in out.stdout
