@@ -264,6 +264,8 @@ void pme_enqueue(atvh_system* s, int precision, bool ev);
 int pme_partial_blocks(const atvh_system* s);
 const double* pme_partials(const atvh_system* s);
 void pme_energy_consts(const atvh_system* s, double& escale, double& e_self, double& e_net);
+// device bytes one FFT plan pair took (largest seen; 64 MiB before the first)
+int64_t fft_pair_bytes();
 void coul_free(atvh_system* s);
 
 // batched launches (batch.hip + the module builders)

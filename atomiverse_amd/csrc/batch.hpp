@@ -11,15 +11,27 @@
 
 #include "atvh_internal.hpp"
 
+#include <type_traits>
+
 namespace atvh {
 
+// Workgroup size of an op: Op::kThreads where the op declares one, else kBlock.
+template <class Op, class = void>
+struct op_threads {
+    static constexpr int value = kBlock;
+};
 template <class Op>
-__global__ __launch_bounds__(kBlock) void k_single(typename Op::Args a) {
+struct op_threads<Op, std::void_t<decltype(Op::kThreads)>> {
+    static constexpr int value = Op::kThreads;
+};
+
+template <class Op>
+__global__ __launch_bounds__(op_threads<Op>::value) void k_single(typename Op::Args a) {
     Op::run(a, blockIdx.x, gridDim.x);
 }
 
 template <class Op>
-__global__ __launch_bounds__(kBlock) void k_batched(const typename Op::Args* __restrict__ args,
+__global__ __launch_bounds__(op_threads<Op>::value) void k_batched(const typename Op::Args* __restrict__ args,
                                                     const int* __restrict__ base, int nsys) {
     const int blk = blockIdx.x;
     int lo = 0, hi = nsys - 1; // largest s with base[s] <= blk

@@ -16,13 +16,16 @@
 //   the reference forms them per pair.
 //
 // KSpacePME (src/cpu/longrange/pme.cpp:147-253, pme_spline.h, pme_influence.h):
-//   1. spread   one thread per atom: fractional coordinate and base index in
+//   1. spread   one workgroup per brick of the neighbour list (or per chunk
+//               of consecutive slots): fractional coordinate and base index in
 //               FP64 (the reference's floor/wrap), B-spline weights (Essmann
 //               recursion) in the compute precision, n^3 contributions added to
-//               a 64-bit fixed-point grid with integer atomics — exact and
-//               order-independent, so the grid is bitwise reproducible. The
-//               scale is 2^(61 - ceil(log2 sum|q|)), which no grid point can
-//               overflow (a point holds at most sum|q_i|).
+//               a 64-bit fixed-point copy of the brick's box of grid points in
+//               LDS, then each touched point added once to the global
+//               fixed-point grid. Integer atomics make the grid exact and
+//               order-independent, so it is bitwise reproducible. The scale is
+//               2^(61 - ceil(log2 sum|q|)), which no grid point can overflow
+//               (a point holds at most sum|q_i|).
 //   2. unfix    fixed point -> real grid (FP64 or FP32), clearing the
 //               fixed-point grid for the next call.
 //   3. hipFFT   R2C (D2Z / R2C), unnormalised.
@@ -30,7 +33,8 @@
 //               energy and virial summed in FP64 over the half spectrum with
 //               the Hermitian partner of each interior mode folded in.
 //   5. hipFFT   C2R (Z2D / C2R).
-//   6. gather   one thread per atom: weights and derivatives again, n^3 reads,
+//   6. gather   one thread per atom: weights and derivatives again, n^3 reads
+//               of the (L2-resident) grid,
 //               F = -(k_e / (pi V)) q sum_d g_d K_d / L_d, added to the forces.
 //   Self and net-charge energies are host constants of the uploaded charges.
 
@@ -41,6 +45,8 @@
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
+#include <atomic>
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -152,7 +158,11 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
     const double* __restrict__ tab = a.tab;
     const double4* __restrict__ etab = a.erfc;
     const uint4* __restrict__ base = reinterpret_cast<const uint4*>(a.nbr16 + (int64_t)a0 * a.cap);
-    for (int li = threadIdx.x; li < nat; li += blockDim.x) {
+    // two threads per atom when the brick has at most half a block of atoms
+    const int tpa = (2 * nat <= static_cast<int>(blockDim.x)) ? 2 : 1;
+    for (int t = threadIdx.x; t < nat * tpa; t += blockDim.x) {
+        const int li = tpa == 2 ? (t >> 1) : t;
+        const int h = tpa == 2 ? (t & 1) : 0;
         const int i = a0 + li;
         const int qs = a.self16[i];
         const double xi = sx[qs], yi = sy[qs], zi = sz[qs];
@@ -162,25 +172,39 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
         const uint4* __restrict__ grp = base + li;
         const int ng = (nn + kGroup - 1) / kGroup;
         double fxi = 0.0, fyi = 0.0, fzi = 0.0;
-        uint4 cur = grp[0]; // in bounds even when nn == 0: the brick's rows always exist
-        for (int g = 0; g < ng; ++g) {
-            const uint4 nxt = grp[(int64_t)min(g + 1, ng - 1) * nat];
-            const int rem = min(nn - g * kGroup, kGroup);
-            for (int u = 0; u < rem; ++u) {
+        const int gl = max(ng - 1, 0); // group 0 exists even when nn == 0
+        uint4 cur = grp[(int64_t)min(h, gl) * nat];
+        uint4 nx1 = grp[(int64_t)min(h + tpa, gl) * nat];
+        for (int g = h; g < ng; g += tpa) {
+            const uint4 nx2 = grp[(int64_t)min(g + 2 * tpa, gl) * nat];
+            // the whole group's LDS reads issue before any arithmetic; padding
+            // rows are the atom itself (r^2 == 0, skipped like r >= r_c)
+            double jx[kGroup], jy[kGroup], jz[kGroup], jq[kGroup];
+            int jt[kGroup];
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) {
                 const int q = group_entry(cur, u);
-                const double dx = xi - sx[q];
-                const double dy = yi - sy[q];
-                const double dz = zi - sz[q];
+                jx[u] = sx[q];
+                jy[u] = sy[q];
+                jz[u] = sz[q];
+                jq[u] = sq[q];
+                jt[u] = stype[q];
+            }
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) {
+                const double dx = xi - jx[u];
+                const double dy = yi - jy[u];
+                const double dz = zi - jz[u];
                 const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
                 if (r2 < rc_sq && r2 != 0.0) {
-                    const double* __restrict__ t = trow + stype[q] * 8;
+                    const double* __restrict__ t = trow + jt[u] * 8;
                     const double r2inv = 1.0 / r2;
                     const double r6inv = r2inv * r2inv * r2inv;
                     const double xb = r2 * inv_dr2;
                     const int bi = min(static_cast<int>(xb), nbin1);
                     const double tt = xb - static_cast<double>(bi);
                     const double4 ev = etab[bi];
-                    const double qq = qi_scale * sq[q];
+                    const double qq = qi_scale * jq[u];
                     const double flj = r2inv * fma(t[0] * r6inv, r6inv, -(t[1] * r6inv));
                     const double fpair = fma(qq, fma(tt, ev.y, ev.x), flj);
                     fxi = fma(fpair, dx, fxi);
@@ -199,8 +223,16 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
                     }
                 }
             }
-            cur = nxt;
+            cur = nx1;
+            nx1 = nx2;
         }
+        if (tpa == 2) {
+            fxi += __shfl_xor(fxi, 1, 64);
+            fyi += __shfl_xor(fyi, 1, 64);
+            fzi += __shfl_xor(fzi, 1, 64);
+        }
+        if (h != 0)
+            continue;
         a.fx[i] = fxi;
         a.fy[i] = fyi;
         a.fz[i] = fzi;
@@ -248,7 +280,11 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_mixed(LjcArgs a) {
     const float* __restrict__ tab = a.tab32;
     const float4* __restrict__ etab = a.erfc32;
     const uint4* __restrict__ base = reinterpret_cast<const uint4*>(a.nbr16 + (int64_t)a0 * a.cap);
-    for (int li = threadIdx.x; li < nat; li += blockDim.x) {
+    // two threads per atom when the brick has at most half a block of atoms
+    const int tpa = (2 * nat <= static_cast<int>(blockDim.x)) ? 2 : 1;
+    for (int t = threadIdx.x; t < nat * tpa; t += blockDim.x) {
+        const int li = tpa == 2 ? (t >> 1) : t;
+        const int h = tpa == 2 ? (t & 1) : 0;
         const int i = a0 + li;
         const int qs = a.self16[i];
         const float4 pi = sp[qs];
@@ -258,19 +294,30 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_mixed(LjcArgs a) {
         const uint4* __restrict__ grp = base + li;
         const int ng = (nn + kGroup - 1) / kGroup;
         float fxi = 0.f, fyi = 0.f, fzi = 0.f;
-        uint4 cur = grp[0];
-        for (int g = 0; g < ng; ++g) {
-            const uint4 nxt = grp[(int64_t)min(g + 1, ng - 1) * nat];
-            const int rem = min(nn - g * kGroup, kGroup);
-            for (int u = 0; u < rem; ++u) {
+        const int gl = max(ng - 1, 0); // group 0 exists even when nn == 0
+        uint4 cur = grp[(int64_t)min(h, gl) * nat];
+        uint4 nx1 = grp[(int64_t)min(h + tpa, gl) * nat];
+        for (int g = h; g < ng; g += tpa) {
+            const uint4 nx2 = grp[(int64_t)min(g + 2 * tpa, gl) * nat];
+            // the whole group's LDS reads issue before any arithmetic; padding
+            // rows are the atom itself (r^2 == 0, skipped like r >= r_c)
+            float4 pjs[kGroup];
+            int jt[kGroup];
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) {
                 const int q = group_entry(cur, u);
-                const float4 pj = sp[q];
+                pjs[u] = sp[q];
+                jt[u] = stype[q];
+            }
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) {
+                const float4 pj = pjs[u];
                 const float dx = pi.x - pj.x;
                 const float dy = pi.y - pj.y;
                 const float dz = pi.z - pj.z;
                 const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                 if (r2 < rc_sq && r2 != 0.0f) {
-                    const float4 t = *reinterpret_cast<const float4*>(trow + stype[q] * 8);
+                    const float4 t = *reinterpret_cast<const float4*>(trow + jt[u] * 8);
                     const float r2inv = __builtin_amdgcn_rcpf(r2);
                     const float r6inv = r2inv * r2inv * r2inv;
                     const float xb = r2 * inv_dr2;
@@ -284,7 +331,7 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_mixed(LjcArgs a) {
                     fyi = __builtin_fmaf(fpair, dy, fyi);
                     fzi = __builtin_fmaf(fpair, dz, fzi);
                     if (EF) {
-                        const float sh = trow[stype[q] * 8 + 4];
+                        const float sh = trow[jt[u] * 8 + 4];
                         acc[0] += static_cast<double>(__builtin_fmaf(t.z * r6inv, r6inv, -t.w * r6inv) - sh) +
                                   static_cast<double>(qq * __builtin_fmaf(tt, ev.w, ev.z));
                     }
@@ -299,8 +346,16 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_mixed(LjcArgs a) {
                     }
                 }
             }
-            cur = nxt;
+            cur = nx1;
+            nx1 = nx2;
         }
+        if (tpa == 2) {
+            fxi += __shfl_xor(fxi, 1, 64);
+            fyi += __shfl_xor(fyi, 1, 64);
+            fzi += __shfl_xor(fzi, 1, 64);
+        }
+        if (h != 0)
+            continue;
         a.fx[i] = static_cast<double>(fxi);
         a.fy[i] = static_cast<double>(fyi);
         a.fz[i] = static_cast<double>(fzi);
@@ -446,52 +501,236 @@ __device__ __forceinline__ void bspline_t(T frac, T (&th)[N], T (&dth)[N]) {
 
 // pme_fill_atom_splines (pme_spline.h:131-146), one axis: u = (x/L) K wrapped
 // into [0, K) by u - floor(u/K) K, in FP64; the weights in precision T.
-template <class T, int N>
-__device__ __forceinline__ int axis_spline(double xv, double ic, int K, T (&th)[N], T (&dth)[N]) {
+__device__ __forceinline__ int axis_cell(double xv, double ic, int K, double& frac) {
     const double s = ic * xv;
     double u = s * K;
     u -= floor(u / K) * K;
     // u lies in [0, K]; the clamp only keeps a non-finite coordinate in bounds
     const int u0 = min(max(static_cast<int>(floor(u)), 0), K);
-    const double frac = u - u0;
+    frac = u - u0;
+    return u0;
+}
+
+template <class T, int N>
+__device__ __forceinline__ int axis_spline(double xv, double ic, int K, T (&th)[N], T (&dth)[N]) {
+    double frac;
+    const int u0 = axis_cell(xv, ic, K, frac);
     bspline_t<T, N>(static_cast<T>(frac), th, dth);
     return u0 - (N - 1);
 }
 
+// Fixed-point value of one weight: the same rounding on every path, so the
+// grid does not depend on which path (or order) added it.
+template <class T>
+__device__ __forceinline__ unsigned long long fix_of(T w, double scale) {
+    return static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(w) * scale)));
+}
+
+// All n^3 contributions of atom i added straight to the global grid.
 template <class T, int N>
-__global__ __launch_bounds__(kBlock) void k_pme_spread(PmeArgs a) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const double qi = a.q[i];
-        if (qi == 0.0)
+__device__ __forceinline__ void spread_atom_global(const PmeArgs& a, int64_t i) {
+    const double qi = a.q[i];
+    if (qi == 0.0)
+        return;
+    T tx[N], ty[N], tz[N], dx[N], dy[N], dz[N];
+    const int b0 = axis_spline<T, N>(a.x[i], a.ic0, a.K0, tx, dx);
+    const int b1 = axis_spline<T, N>(a.y[i], a.ic1, a.K1, ty, dy);
+    const int b2 = axis_spline<T, N>(a.z[i], a.ic2, a.K2, tz, dz);
+    const T qt = static_cast<T>(qi);
+    int g2s[N];
+#pragma unroll
+    for (int c = 0; c < N; ++c)
+        g2s[c] = pme_wrap(b2 + c, a.K2);
+#pragma unroll
+    for (int p = 0; p < N; ++p) {
+        const int g0 = pme_wrap(b0 + p, a.K0);
+        const T wa = qt * tx[p];
+#pragma unroll
+        for (int r = 0; r < N; ++r) {
+            const int g1 = pme_wrap(b1 + r, a.K1);
+            const T wab = wa * ty[r];
+            unsigned long long* __restrict__ row = a.fix + (int64_t)a.K2 * (g1 + (int64_t)a.K1 * g0);
+#pragma unroll
+            for (int c = 0; c < N; ++c)
+                atomicAdd(row + g2s[c], fix_of(wab * tz[c], a.fix_scale));
+        }
+    }
+}
+
+// Offset of grid index b from the reference index r along a periodic axis of
+// K points, folded into [-K/2, K/2] (|b - r| <= K + n on entry).
+__device__ __forceinline__ int grid_offset(int b, int r, int K) {
+    int d = b - r;
+    if (2 * d > K)
+        d -= K;
+    else if (2 * d < -K)
+        d += K;
+    return d;
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+    for (int off = 32; off > 0; off >>= 1)
+        v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max_i(int v) {
+    for (int off = 32; off > 0; off >>= 1)
+        v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+constexpr int kSpreadChunk = kBlock; // atoms per workgroup without brick ranges
+constexpr int kSpreadCap = 6144;     // LDS grid points per workgroup (48 KiB of int64)
+constexpr int kSpreadGridCap = 8192; // workgroups of one spread launch
+
+// The box of grid points a range of atoms touches. Base indices are taken as
+// periodic offsets from the range's first atom, so a range that drifted
+// across the box edge stays compact; box point (i0, i1, i2) is grid point
+// (o + i) mod K on each axis. All threads of the block call it (two barriers).
+struct RangeBox {
+    int64_t first;
+    int count;
+    int ref[3]; // base index of the first atom
+    int lo[3];  // smallest offset from ref
+    int o[3];   // ref + lo: grid index of box point 0 (unwrapped)
+    int e[3];   // box extent (points)
+    int64_t total;
+};
+
+__device__ __forceinline__ bool range_of(const PmeArgs& a, const int2* __restrict__ ranges, int r, RangeBox& rb) {
+    if (ranges) {
+        const int2 br = ranges[r];
+        rb.first = br.x;
+        rb.count = br.y;
+    } else {
+        rb.first = (int64_t)r * kSpreadChunk;
+        rb.count = static_cast<int>(min<int64_t>(kSpreadChunk, a.n - rb.first));
+    }
+    return rb.count > 0;
+}
+
+template <int N>
+__device__ __forceinline__ void range_box(const PmeArgs& a, RangeBox& rb, int (*red)[6]) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double fr;
+    rb.ref[0] = axis_cell(a.x[rb.first], a.ic0, a.K0, fr) - (N - 1);
+    rb.ref[1] = axis_cell(a.y[rb.first], a.ic1, a.K1, fr) - (N - 1);
+    rb.ref[2] = axis_cell(a.z[rb.first], a.ic2, a.K2, fr) - (N - 1);
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    for (int t = threadIdx.x; t < rb.count; t += blockDim.x) {
+        const int64_t i = rb.first + t;
+        const int d0 = grid_offset(axis_cell(a.x[i], a.ic0, a.K0, fr) - (N - 1), rb.ref[0], a.K0);
+        const int d1 = grid_offset(axis_cell(a.y[i], a.ic1, a.K1, fr) - (N - 1), rb.ref[1], a.K1);
+        const int d2 = grid_offset(axis_cell(a.z[i], a.ic2, a.K2, fr) - (N - 1), rb.ref[2], a.K2);
+        lo[0] = min(lo[0], d0);
+        lo[1] = min(lo[1], d1);
+        lo[2] = min(lo[2], d2);
+        hi[0] = max(hi[0], d0);
+        hi[1] = max(hi[1], d1);
+        hi[2] = max(hi[2], d2);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = wave_min_i(lo[k]);
+        hi[k] = wave_max_i(hi[k]);
+    }
+    __syncthreads(); // red[] may still be read for the previous range
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            red[wid][k] = lo[k];
+            red[wid][3 + k] = hi[k];
+        }
+    __syncthreads();
+    const int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        int l = red[0][k], h = red[0][3 + k];
+        for (int w = 1; w < nw; ++w) {
+            l = min(l, red[w][k]);
+            h = max(h, red[w][3 + k]);
+        }
+        rb.lo[k] = l;
+        rb.o[k] = rb.ref[k] + l;
+        rb.e[k] = h - l + N;
+    }
+    rb.total = (int64_t)rb.e[0] * rb.e[1] * rb.e[2];
+}
+
+__device__ __forceinline__ int wrap_any(int g, int K) {
+    g %= K;
+    return g < 0 ? g + K : g;
+}
+
+// Spread with the grid privatised per workgroup. Atoms are stored in cell
+// order, so a brick's atoms (or a chunk of consecutive slots) touch a small
+// box of grid points: the workgroup accumulates all its contributions to that
+// box in LDS with 64-bit integer atomics and adds every touched point to the
+// global grid once. Ranges whose box exceeds kSpreadCap points go straight
+// to the global grid. Integer addition makes both paths give the same grid
+// bit for bit. Work items are (atom, first-axis offset p) pairs so a range of
+// up to n * count items spreads evenly over the workgroup.
+template <class T, int N>
+__global__ __launch_bounds__(kBlock) void k_pme_spread_lds(PmeArgs a, const int2* __restrict__ ranges,
+                                                           int nranges) {
+    __shared__ unsigned long long acc[kSpreadCap];
+    __shared__ int red[kBlock / 64][6];
+    for (int r = blockIdx.x; r < nranges; r += gridDim.x) {
+        RangeBox rb;
+        if (!range_of(a, ranges, r, rb))
             continue;
-        T tx[N], ty[N], tz[N], dx[N], dy[N], dz[N];
-        const int b0 = axis_spline<T, N>(a.x[i], a.ic0, a.K0, tx, dx);
-        const int b1 = axis_spline<T, N>(a.y[i], a.ic1, a.K1, ty, dy);
-        const int b2 = axis_spline<T, N>(a.z[i], a.ic2, a.K2, tz, dz);
-        const T qt = static_cast<T>(qi);
-        int g2s[N];
+        range_box<N>(a, rb, red);
+        if (rb.total > kSpreadCap) {
+            for (int t = threadIdx.x; t < rb.count; t += blockDim.x)
+                spread_atom_global<T, N>(a, rb.first + t);
+            continue;
+        }
+        const int nbox = static_cast<int>(rb.total);
+        const int e1 = rb.e[1], e2 = rb.e[2];
+        for (int k = threadIdx.x; k < nbox; k += blockDim.x)
+            acc[k] = 0ull;
+        __syncthreads();
+        for (int item = threadIdx.x; item < rb.count * N; item += blockDim.x) {
+            const int t = item / N, p = item - t * N;
+            const int64_t i = rb.first + t;
+            const double qi = a.q[i];
+            if (qi == 0.0)
+                continue;
+            T tx[N], ty[N], tz[N], dx[N], dy[N], dz[N];
+            const int b0 = axis_spline<T, N>(a.x[i], a.ic0, a.K0, tx, dx);
+            const int b1 = axis_spline<T, N>(a.y[i], a.ic1, a.K1, ty, dy);
+            const int b2 = axis_spline<T, N>(a.z[i], a.ic2, a.K2, tz, dz);
+            T txp = tx[0];
 #pragma unroll
-        for (int c = 0; c < N; ++c)
-            g2s[c] = pme_wrap(b2 + c, a.K2);
+            for (int k = 1; k < N; ++k)
+                txp = (k == p) ? tx[k] : txp;
+            const int l0 = grid_offset(b0, rb.ref[0], a.K0) - rb.lo[0] + p;
+            const int l1 = grid_offset(b1, rb.ref[1], a.K1) - rb.lo[1];
+            const int l2 = grid_offset(b2, rb.ref[2], a.K2) - rb.lo[2];
+            const T wa = static_cast<T>(qi) * txp;
+            unsigned long long* plane = acc + (l0 * e1 + l1) * e2 + l2;
 #pragma unroll
-        for (int p = 0; p < N; ++p) {
-            const int g0 = pme_wrap(b0 + p, a.K0);
-            const T wa = qt * tx[p];
+            for (int rr = 0; rr < N; ++rr) {
+                const T wab = wa * ty[rr];
 #pragma unroll
-            for (int r = 0; r < N; ++r) {
-                const int g1 = pme_wrap(b1 + r, a.K1);
-                const T wab = wa * ty[r];
-                unsigned long long* __restrict__ row =
-                    a.fix + (int64_t)a.K2 * (g1 + (int64_t)a.K1 * g0);
-#pragma unroll
-                for (int c = 0; c < N; ++c) {
-                    const double v = static_cast<double>(wab * tz[c]);
-                    const long long fx = static_cast<long long>(rint(v * a.fix_scale));
-                    atomicAdd(row + g2s[c], static_cast<unsigned long long>(fx));
-                }
+                for (int c = 0; c < N; ++c)
+                    atomicAdd(plane + rr * e2 + c, fix_of(wab * tz[c], a.fix_scale));
             }
         }
+        __syncthreads();
+        for (int k = threadIdx.x; k < nbox; k += blockDim.x) {
+            const unsigned long long v = acc[k];
+            if (v == 0ull)
+                continue;
+            const int i2 = k % e2, rest = k / e2;
+            const int i1 = rest % e1, i0 = rest / e1;
+            const int g0 = wrap_any(rb.o[0] + i0, a.K0);
+            const int g1 = wrap_any(rb.o[1] + i1, a.K1);
+            const int g2 = wrap_any(rb.o[2] + i2, a.K2);
+            atomicAdd(a.fix + (int64_t)a.K2 * (g1 + (int64_t)a.K1 * g0) + g2, v);
+        }
+        __syncthreads(); // acc[] is cleared for the next range
     }
 }
 
@@ -558,46 +797,55 @@ __global__ __launch_bounds__(kBlock) void k_pme_influence(PmeArgs a, T2* __restr
         block_partials(acc, a.partials, blockIdx.x);
 }
 
+// Force on atom i from the potential grid, read through at(g0, g1, g2) with
+// the base indices b (grid index of weight 0 on each axis, before wrapping).
+template <class T, int N, class At>
+__device__ __forceinline__ void gather_atom(const PmeArgs& a, int64_t i, At at) {
+    const double qi = a.q[i];
+    if (qi == 0.0)
+        return;
+    T tx[N], ty[N], tz[N], dx[N], dy[N], dz[N];
+    const int b0 = axis_spline<T, N>(a.x[i], a.ic0, a.K0, tx, dx);
+    const int b1 = axis_spline<T, N>(a.y[i], a.ic1, a.K1, ty, dy);
+    const int b2 = axis_spline<T, N>(a.z[i], a.ic2, a.K2, tz, dz);
+    T gx = T(0), gy = T(0), gz = T(0);
+#pragma unroll
+    for (int p = 0; p < N; ++p) {
+#pragma unroll
+        for (int r = 0; r < N; ++r) {
+#pragma unroll
+            for (int c = 0; c < N; ++c) {
+                const T cval = at(b0, b1, b2, p, r, c);
+                gx += cval * dx[p] * ty[r] * tz[c];
+                gy += cval * tx[p] * dy[r] * tz[c];
+                gz += cval * tx[p] * ty[r] * dz[c];
+            }
+        }
+    }
+    const double fxi = static_cast<double>(gx) * a.K0 * a.ic0;
+    const double fyi = static_cast<double>(gy) * a.K1 * a.ic1;
+    const double fzi = static_cast<double>(gz) * a.K2 * a.ic2;
+    const double pre = -a.fscale * qi;
+    a.fx[i] += pre * fxi;
+    a.fy[i] += pre * fyi;
+    a.fz[i] += pre * fzi;
+}
+
+template <class T, int N>
+__device__ __forceinline__ void gather_atom_global(const PmeArgs& a, int64_t i, const T* __restrict__ real) {
+    gather_atom<T, N>(a, i, [&](int b0, int b1, int b2, int p, int r, int c) {
+        const int g0 = pme_wrap(b0 + p, a.K0), g1 = pme_wrap(b1 + r, a.K1), g2 = pme_wrap(b2 + c, a.K2);
+        return real[(int64_t)a.K2 * (g1 + (int64_t)a.K1 * g0) + g2];
+    });
+}
+
+// One thread per atom, grid values read from the global grid (L2-resident;
+// a box staged in LDS per brick measured slower).
 template <class T, int N>
 __global__ __launch_bounds__(kBlock) void k_pme_gather(PmeArgs a, const T* __restrict__ real) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const double qi = a.q[i];
-        if (qi == 0.0)
-            continue;
-        T tx[N], ty[N], tz[N], dx[N], dy[N], dz[N];
-        const int b0 = axis_spline<T, N>(a.x[i], a.ic0, a.K0, tx, dx);
-        const int b1 = axis_spline<T, N>(a.y[i], a.ic1, a.K1, ty, dy);
-        const int b2 = axis_spline<T, N>(a.z[i], a.ic2, a.K2, tz, dz);
-        int g2s[N];
-#pragma unroll
-        for (int c = 0; c < N; ++c)
-            g2s[c] = pme_wrap(b2 + c, a.K2);
-        T gx = T(0), gy = T(0), gz = T(0);
-#pragma unroll
-        for (int p = 0; p < N; ++p) {
-            const int g0 = pme_wrap(b0 + p, a.K0);
-#pragma unroll
-            for (int r = 0; r < N; ++r) {
-                const int g1 = pme_wrap(b1 + r, a.K1);
-                const T* __restrict__ row = real + (int64_t)a.K2 * (g1 + (int64_t)a.K1 * g0);
-#pragma unroll
-                for (int c = 0; c < N; ++c) {
-                    const T cval = row[g2s[c]];
-                    gx += cval * dx[p] * ty[r] * tz[c];
-                    gy += cval * tx[p] * dy[r] * tz[c];
-                    gz += cval * tx[p] * ty[r] * dz[c];
-                }
-            }
-        }
-        const double fxi = static_cast<double>(gx) * a.K0 * a.ic0;
-        const double fyi = static_cast<double>(gy) * a.K1 * a.ic1;
-        const double fzi = static_cast<double>(gz) * a.K2 * a.ic2;
-        const double pre = -a.fscale * qi;
-        a.fx[i] += pre * fxi;
-        a.fy[i] += pre * fyi;
-        a.fz[i] += pre * fzi;
-    }
+         i += (int64_t)gridDim.x * blockDim.x)
+        gather_atom_global<T, N>(a, i, real);
 }
 
 __global__ __launch_bounds__(kBlock) void k_zero3(int64_t n, double* a, double* b, double* c) {
@@ -641,23 +889,32 @@ __global__ __launch_bounds__(kBlock) void k_zero3(int64_t n, double* a, double* 
         }                                                                                      \
     } while (0)
 
+// Atom ranges of the privatised spread: the bricks of the current list (slot
+// order is brick order until the next rebuild), else chunks of kSpreadChunk
+// consecutive slots.
+struct SpreadRanges {
+    const int2* ranges;
+    int n;
+};
+
 template <class T, int N>
-void launch_spread_gather(bool spread, int grid, hipStream_t st, const PmeArgs& a, const T* real) {
+void launch_spread_gather(bool spread, hipStream_t st, const PmeArgs& a, const T* real, const SpreadRanges& sr) {
     if (spread)
-        k_pme_spread<T, N><<<grid, kBlock, 0, st>>>(a);
+        k_pme_spread_lds<T, N><<<std::min(sr.n, kSpreadGridCap), kBlock, 0, st>>>(a, sr.ranges, sr.n);
     else
-        k_pme_gather<T, N><<<grid, kBlock, 0, st>>>(a, real);
+        k_pme_gather<T, N><<<grid_capped(a.n), kBlock, 0, st>>>(a, real);
 }
 
 template <class T>
-void dispatch_order(int order, bool spread, int grid, hipStream_t st, const PmeArgs& a, const T* real) {
+void dispatch_order(int order, bool spread, hipStream_t st, const PmeArgs& a, const T* real,
+                    const SpreadRanges& sr) {
     switch (order) {
-    case 3: launch_spread_gather<T, 3>(spread, grid, st, a, real); break;
-    case 4: launch_spread_gather<T, 4>(spread, grid, st, a, real); break;
-    case 5: launch_spread_gather<T, 5>(spread, grid, st, a, real); break;
-    case 6: launch_spread_gather<T, 6>(spread, grid, st, a, real); break;
-    case 7: launch_spread_gather<T, 7>(spread, grid, st, a, real); break;
-    case 8: launch_spread_gather<T, 8>(spread, grid, st, a, real); break;
+    case 3: launch_spread_gather<T, 3>(spread, st, a, real, sr); break;
+    case 4: launch_spread_gather<T, 4>(spread, st, a, real, sr); break;
+    case 5: launch_spread_gather<T, 5>(spread, st, a, real, sr); break;
+    case 6: launch_spread_gather<T, 6>(spread, st, a, real, sr); break;
+    case 7: launch_spread_gather<T, 7>(spread, st, a, real, sr); break;
+    case 8: launch_spread_gather<T, 8>(spread, st, a, real, sr); break;
     default: throw Error(ATVH_E_ARG, "KSpacePME: order must be in [3, 8]");
     }
 }
@@ -749,6 +1006,16 @@ void coul_release(CoulState* c) {
     dev_free(c->erfc);
     dev_free(c->erfc32);
     delete c;
+}
+
+// Largest device footprint of one forward + backward plan pair seen in this
+// process (admission estimates read it).
+std::atomic<int64_t> g_fft_pair_bytes{0};
+
+void note_fft_pair_bytes(int64_t v) {
+    int64_t cur = g_fft_pair_bytes.load();
+    while (v > cur && !g_fft_pair_bytes.compare_exchange_weak(cur, v)) {
+    }
 }
 
 PmeArgs make_pme_args(atvh_system* s) {
@@ -848,25 +1115,38 @@ void pme_enqueue(atvh_system* s, int precision, bool ev) {
     const bool mixed = precision == ATVH_PREC_MIXED;
     const int64_t ntot = (int64_t)p->K[0] * p->K[1] * p->K[2];
     const int64_t nmodes = (int64_t)p->K[0] * p->K[1] * p->Kh;
-    // buffers and plans of this precision, made on first use
+    // buffers and plans of this precision, made on first use; the device
+    // memory the FFT library takes for a plan pair (work areas, tables, the
+    // kernels it loads at the first execution) is measured once it has run
+    size_t free_before = 0, total_mem = 0;
+    const bool fresh = mixed ? !p->real32 : !p->real64;
     if (mixed && !p->real32) {
         dev_alloc(&p->real32, ntot);
         dev_alloc(&p->cplx32, nmodes);
+        ATVH_HIP(hipMemGetInfo(&free_before, &total_mem));
         HIPFFT_CHECK(hipfftPlan3d(&p->f32, p->K[0], p->K[1], p->K[2], HIPFFT_R2C));
         HIPFFT_CHECK(hipfftPlan3d(&p->b32, p->K[0], p->K[1], p->K[2], HIPFFT_C2R));
     }
     if (!mixed && !p->real64) {
         dev_alloc(&p->real64, ntot);
         dev_alloc(&p->cplx64, nmodes);
+        ATVH_HIP(hipMemGetInfo(&free_before, &total_mem));
         HIPFFT_CHECK(hipfftPlan3d(&p->f64, p->K[0], p->K[1], p->K[2], HIPFFT_D2Z));
         HIPFFT_CHECK(hipfftPlan3d(&p->b64, p->K[0], p->K[1], p->K[2], HIPFFT_Z2D));
     }
     PmeArgs a = make_pme_args(s);
-    const int ga = grid_capped(s->n);
+    SpreadRanges sr;
+    if (s->brick_mode && s->nl_built && s->brick_range) {
+        sr.ranges = s->brick_range;
+        sr.n = s->nbricks;
+    } else {
+        sr.ranges = nullptr;
+        sr.n = static_cast<int>((s->n + kSpreadChunk - 1) / kSpreadChunk);
+    }
     const int gg = grid_capped(ntot);
     const int gm = std::min(grid_for(nmodes), kPmeGridCap);
     if (mixed) {
-        dispatch_order<float>(p->order, true, ga, st, a, nullptr);
+        dispatch_order<float>(p->order, true, st, a, nullptr, sr);
         k_pme_unfix<float><<<gg, kBlock, 0, st>>>(ntot, p->fix, p->real32, a.inv_fix);
         HIPFFT_CHECK(hipfftSetStream(p->f32, st));
         HIPFFT_CHECK(hipfftExecR2C(p->f32, p->real32, reinterpret_cast<hipfftComplex*>(p->cplx32)));
@@ -876,9 +1156,9 @@ void pme_enqueue(atvh_system* s, int precision, bool ev) {
             k_pme_influence<float2, false><<<gm, kBlock, 0, st>>>(a, p->cplx32);
         HIPFFT_CHECK(hipfftSetStream(p->b32, st));
         HIPFFT_CHECK(hipfftExecC2R(p->b32, reinterpret_cast<hipfftComplex*>(p->cplx32), p->real32));
-        dispatch_order<float>(p->order, false, ga, st, a, p->real32);
+        dispatch_order<float>(p->order, false, st, a, p->real32, sr);
     } else {
-        dispatch_order<double>(p->order, true, ga, st, a, nullptr);
+        dispatch_order<double>(p->order, true, st, a, nullptr, sr);
         k_pme_unfix<double><<<gg, kBlock, 0, st>>>(ntot, p->fix, p->real64, a.inv_fix);
         HIPFFT_CHECK(hipfftSetStream(p->f64, st));
         HIPFFT_CHECK(hipfftExecD2Z(p->f64, p->real64, reinterpret_cast<hipfftDoubleComplex*>(p->cplx64)));
@@ -888,7 +1168,13 @@ void pme_enqueue(atvh_system* s, int precision, bool ev) {
             k_pme_influence<double2, false><<<gm, kBlock, 0, st>>>(a, p->cplx64);
         HIPFFT_CHECK(hipfftSetStream(p->b64, st));
         HIPFFT_CHECK(hipfftExecZ2D(p->b64, reinterpret_cast<hipfftDoubleComplex*>(p->cplx64), p->real64));
-        dispatch_order<double>(p->order, false, ga, st, a, p->real64);
+        dispatch_order<double>(p->order, false, st, a, p->real64, sr);
+    }
+    if (fresh) {
+        size_t free_after = 0;
+        ATVH_HIP(hipMemGetInfo(&free_after, &total_mem));
+        if (free_before > free_after)
+            note_fft_pair_bytes(static_cast<int64_t>(free_before - free_after));
     }
     p->nb = gm;
     ATVH_HIP(hipGetLastError());
@@ -899,6 +1185,11 @@ const double* pme_partials(const atvh_system* s) { return s->pme ? s->pme->parti
 
 // escale = k_e / (2 pi V) (pme_influence.h:215); self and net-charge energies
 // (pme.cpp:222-246) from the charge sums of the uploaded frame.
+int64_t fft_pair_bytes() {
+    const int64_t v = g_fft_pair_bytes.load();
+    return v > 0 ? v : (int64_t(64) << 20);
+}
+
 void pme_energy_consts(const atvh_system* s, double& escale, double& e_self, double& e_net) {
     const PmeState* p = s->pme;
     escale = coulomb / (2.0 * kPi * p->vol);

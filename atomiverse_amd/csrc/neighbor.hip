@@ -495,20 +495,22 @@ __device__ void OpHalo::run(const BrickBuildArgs& a, int bid, int nbk) {
     }
 }
 
-// Rebuild step 2 (one workgroup per brick). Work items are (brick atom,
-// stencil column) pairs laid out column-major so the lanes of a wave share one
-// column (same x/y image and orientation). Each item culls the column's dk
-// range with a conservative geometric bound, then screens its candidates with
-// FP32 image coordinates (relative to the brick centre: |error of r^2| <=
-// 1e-5 Å^2): a candidate is accepted below (rc+skin)^2 - kBand, rejected above
-// (rc+skin)^2 + kBand, and only inside the band decided by the exact reference
-// predicate — raw FP64 positions, the shift form of the half-stencil offset in
-// its canonical orientation (refresh_half, cell_list.cpp:680-727). The pair
-// set is therefore exactly the reference's. Pass A counts per item, a
-// per-atom prefix over the columns fixes every item's output offset (column
-// order, then halo row order: deterministic), pass B writes the 16-bit rows in
-// 8-wide groups.
+// Rebuild step 2 (one workgroup per brick, one wavefront per home cell). The
+// wave lays the cell's candidates out flat — every stencil column's z-run of
+// halo rows, in column order — and then, for each atom of the cell, takes 64
+// candidates per step, one per lane, so the candidate loop does not diverge.
+// Candidates are screened with FP32 image coordinates (relative to the brick
+// centre: |error of r^2| <= 1e-5 Å^2): accepted below (rc+skin)^2 - kBand,
+// rejected above (rc+skin)^2 + kBand, and only inside the band decided by the
+// exact reference predicate — raw FP64 positions, the shift form of the
+// half-stencil offset in its canonical orientation (refresh_half,
+// cell_list.cpp:680-727). The pair set is therefore exactly the reference's.
+// Accepted rows are compacted with a ballot in candidate order (column order,
+// then halo row order: deterministic) into 16-bit rows in 8-wide groups; the
+// unused tail of the last group holds the atom's own row. A cell with more
+// than kCandCap candidates walks its columns one at a time instead.
 constexpr float kBand = 1e-4f;
+constexpr int kCandCap = 768; // flat candidates per wavefront (LDS)
 
 __device__ __noinline__ bool exact_pair(const double* __restrict__ x, const double* __restrict__ y,
                                         const double* __restrict__ z, double rc_sq, int gi, int gj,
@@ -529,31 +531,32 @@ __device__ __noinline__ bool exact_pair(const double* __restrict__ x, const doub
     return r2 < rc_sq;
 }
 
+// Number of set bits of m below this lane.
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(m), 0u));
+}
+
 struct OpPairs {
     using Args = BrickBuildArgs;
+    static constexpr int kThreads = 512;
     static __device__ void run(const BrickBuildArgs& a, int bid, int nbk);
 };
 
 __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
     if (!*a.pred)
         return;
-    __shared__ float4 sp[kHaloCap]; // image x, y, z relative to the brick centre; w = halo z index
+    constexpr int kWaves = kThreads / 64;
+    __shared__ float4 sp[kHaloCap]; // image x, y, z relative to the brick centre; w = halo cell hx<<16|hy<<8|hz
     __shared__ int h_off[kHaloCellCap + 1];
-    __shared__ unsigned short item_off[kMaxBrickAtoms * kMaxCols]; // counts, then offsets
-    __shared__ unsigned short self_q[kMaxBrickAtoms];
-    __shared__ unsigned char self_cell[kMaxBrickAtoms][4];
-    __shared__ long long sh_tot[kBlock / 64];
-    __shared__ int sh_max[kBlock / 64], sh_over[kBlock / 64];
+    __shared__ unsigned short cand[kWaves][kCandCap];
+    __shared__ long long sh_tot[kWaves];
+    __shared__ int sh_max[kWaves], sh_over[kWaves];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     long long my_cnt_sum = 0;
     int my_max = 0, my_over = 0;
-    const int nz = a.g.nc[2], ny = a.g.nc[1];
     const double Lx = a.g.L[0], Ly = a.g.L[1], Lz = a.g.L[2];
-    const float csxf = static_cast<float>(a.g.cs[0]), csyf = static_cast<float>(a.g.cs[1]);
-    const float cszf = static_cast<float>(a.g.cs[2]);
-    const float icszf = 1.0f / cszf;
-    const float rcsf = static_cast<float>(a.rcs);
-    const float reach = rcsf + 1e-4f; // culling slack >> FP32 rounding of the coordinates
-    const float reach_sq = reach * reach;
+    const int R0 = a.g.R[0], R1 = a.g.R[1], R2 = a.g.R[2];
     const float rc2f = static_cast<float>(a.rc_sq);
     const float lo_acc = rc2f - kBand, hi_rej = rc2f + kBand;
     const int ncols = a.nrows;
@@ -562,14 +565,14 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
         const int2 br = a.brick_range[b];
         const int a0 = br.x, nat = br.y;
         if (nh > kHaloCap || nat > kMaxBrickAtoms)
-            continue; // reported by k_nl_halo (block-uniform)
+            continue; // reported by OpHalo (block-uniform)
         int bx, by, bz;
         brick_coords(a.g, b, bx, by, bz);
         const int o0 = bx * a.g.bdim[0], o1 = by * a.g.bdim[1], o2 = bz * a.g.bdim[2];
         const int d0 = min(a.g.bdim[0], a.g.nc[0] - o0);
         const int d1 = min(a.g.bdim[1], a.g.nc[1] - o1), d2 = min(a.g.bdim[2], a.g.nc[2] - o2);
-        const int HY = d1 + 2 * a.g.R[1], HZ = d2 + 2 * a.g.R[2];
-        const int ncell = (d0 + 2 * a.g.R[0]) * HY * HZ;
+        const int HY = d1 + 2 * R1, HZ = d2 + 2 * R2;
+        const int ncell = (d0 + 2 * R0) * HY * HZ;
         const double bc0 = (o0 + 0.5 * d0) * a.g.cs[0];
         const double bc1 = (o1 + 0.5 * d1) * a.g.cs[1];
         const double bc2 = (o2 + 0.5 * d2) * a.g.cs[2];
@@ -578,121 +581,130 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
             h_off[h] = hoff[h];
         const int* __restrict__ rows = a.halo_rows + (int64_t)b * kHaloCap;
         __syncthreads();
+        // halo cell of every row (per halo cell), then the coordinates (per
+        // row: independent loads, so many are in flight at once)
         for (int h = threadIdx.x; h < ncell; h += blockDim.x) {
             const int q0 = h_off[h], q1 = h_off[h + 1];
-            const float hzc = __int_as_float(h % HZ);
-            for (int q = q0; q < q1; ++q) {
-                const int row = rows[q];
-                const int gs = row & kRowSlotMask;
-                const int code = static_cast<unsigned>(row) >> kRowSlotBits;
-                float4 v;
-                v.x = static_cast<float>((a.x[gs] + shift_of(code, 0, Lx)) - bc0);
-                v.y = static_cast<float>((a.y[gs] + shift_of(code, 1, Ly)) - bc1);
-                v.z = static_cast<float>((a.z[gs] + shift_of(code, 2, Lz)) - bc2);
-                v.w = hzc;
-                sp[q] = v;
-            }
+            const int code = ((h / (HY * HZ)) << 16) | (((h / HZ) % HY) << 8) | (h % HZ);
+            for (int q = q0; q < q1; ++q)
+                sp[q].w = __int_as_float(code);
         }
-        // brick atoms: own halo row and home cell
-        for (int li = threadIdx.x; li < nat; li += blockDim.x) {
-            const int i = a0 + li;
-            const int ci = a.cell_of[i];
-            const int lz = ci % nz - o2, ly = (ci / nz) % ny - o1, lx = ci / (ny * nz) - o0;
-            const int hs = ((lx + a.g.R[0]) * HY + (ly + a.g.R[1])) * HZ + (lz + a.g.R[2]);
-            const int q0 = h_off[hs];
-            const int first = rows[q0] & kRowSlotMask; // slot of the home cell's first atom
-            self_q[li] = static_cast<unsigned short>(q0 + (i - first));
-            self_cell[li][0] = static_cast<unsigned char>(lx);
-            self_cell[li][1] = static_cast<unsigned char>(ly);
-            self_cell[li][2] = static_cast<unsigned char>(lz);
+        const int nh_rows = h_off[ncell];
+        for (int q = threadIdx.x; q < nh_rows; q += blockDim.x) {
+            const int row = rows[q];
+            const int gs = row & kRowSlotMask;
+            const int code = static_cast<unsigned>(row) >> kRowSlotBits;
+            sp[q].x = static_cast<float>((a.x[gs] + shift_of(code, 0, Lx)) - bc0);
+            sp[q].y = static_cast<float>((a.y[gs] + shift_of(code, 1, Ly)) - bc1);
+            sp[q].z = static_cast<float>((a.z[gs] + shift_of(code, 2, Lz)) - bc2);
         }
         __syncthreads();
-        const int nitems = nat * ncols;
         uint16_t* __restrict__ lst = a.nbr16 + (int64_t)a0 * a.cap;
-        for (int pass = 0; pass < 2; ++pass) {
-            for (int t = threadIdx.x; t < nitems; t += blockDim.x) {
-                const int col = t / nat, li = t - col * nat;
-                const int4 rw = a.rows[col];
-                const int di = rw.x, dj = rw.y, kmax = rw.z;
-                const int lx = self_cell[li][0], ly = self_cell[li][1], lz = self_cell[li][2];
-                const int qs = self_q[li];
+        const int nbc = d0 * d1 * d2;
+        for (int c = wid; c < nbc; c += kWaves) {
+            const int lz = c % d2, ly = (c / d2) % d1, lx = c / (d1 * d2);
+            const int hz0 = lz + R2;
+            const int hs = ((lx + R0) * HY + (ly + R1)) * HZ + hz0;
+            const int hq0 = h_off[hs], hq1 = h_off[hs + 1];
+            if (hq0 == hq1)
+                continue;
+            const int cx = o0 + lx, cy = o1 + ly, cz = o2 + lz;
+            // this cell's z-run of halo rows in every stencil column (lane = column)
+            int qb = 0, len = 0;
+            if (lane < ncols) {
+                const int4 rw = a.rows[lane];
+                const int hcol = ((lx + R0 + rw.x) * HY + (ly + R1 + rw.y)) * HZ + hz0;
+                qb = h_off[hcol - rw.z];
+                len = h_off[hcol + rw.z + 1] - qb;
+            }
+            int inc = len;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int t = __shfl_up(inc, off, 64);
+                if (lane >= off)
+                    inc += t;
+            }
+            const int total = __shfl(inc, 63, 64);
+            const bool flat = total <= kCandCap;
+            if (flat) {
+                const int ex = inc - len;
+                for (int t = 0; t < len; ++t)
+                    cand[wid][ex + t] = static_cast<unsigned short>(qb + t);
+                __builtin_amdgcn_wave_barrier();
+            }
+            const int slot0 = rows[hq0] & kRowSlotMask; // the cell's first atom
+            for (int qs = hq0; qs < hq1; ++qs) {
+                const int i = slot0 + (qs - hq0);
+                const int li = i - a0;
                 const float4 pi = sp[qs];
-                // position inside the home cell, from the brick-centre frame
-                const float fx = pi.x - (lx - 0.5f * d0) * csxf;
-                const float fy = pi.y - (ly - 0.5f * d1) * csyf;
-                const float fz = pi.z - (lz - 0.5f * d2) * cszf;
-                const float gx = fmaxf(0.f, fmaxf(di * csxf - fx, fx - (di + 1) * csxf));
-                const float gy = fmaxf(0.f, fmaxf(dj * csyf - fy, fy - (dj + 1) * csyf));
-                const float g2 = gx * gx + gy * gy;
-                int cnt = 0;
-                int outpos = pass ? item_off[li * kMaxCols + col] : 0;
-                if (g2 <= reach_sq) {
-                    const float zr = sqrtf(reach_sq - g2) + 1e-4f;
-                    const int klo = max(-kmax, static_cast<int>(floorf((fz - zr) * icszf)));
-                    const int khi = min(kmax, static_cast<int>(floorf((fz + zr) * icszf)));
-                    if (klo <= khi) {
-                        const int hz0 = lz + a.g.R[2];
-                        const int hcol = ((lx + a.g.R[0] + di) * HY + (ly + a.g.R[1] + dj)) * HZ;
-                        const int qb = h_off[hcol + hz0 + klo], qe = h_off[hcol + hz0 + khi + 1];
-                        for (int q = qb; q < qe; ++q) {
-                            const float4 pj = sp[q];
-                            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                            bool acc = r2 < lo_acc;
-                            if (!acc && r2 <= hi_rej && q != qs) {
-                                // inside the band: the reference predicate, exactly
-                                const int dk = __float_as_int(pj.w) - hz0;
-                                const int cx = o0 + lx, cy = o1 + ly, cz = o2 + lz;
-                                const int ux = cx + di, uy = cy + dj, uz = cz + dk;
-                                const double shx = ux < 0 ? -Lx : (ux >= a.g.nc[0] ? Lx : 0.0);
-                                const double shy = uy < 0 ? -Ly : (uy >= a.g.nc[1] ? Ly : 0.0);
-                                const double shzv = uz < 0 ? -Lz : (uz >= nz ? Lz : 0.0);
-                                const int colkind = (di > 0 || (di == 0 && dj > 0)) ? 1 : ((di == 0 && dj == 0) ? 0 : 2);
-                                const int kind = colkind != 0 ? colkind : (dk > 0 ? 1 : (dk < 0 ? 2 : 0));
-                                acc = exact_pair(a.x, a.y, a.z, a.rc_sq, a0 + li, rows[q] & kRowSlotMask, kind,
-                                                 shx, shy, shzv);
-                            }
-                            if (acc && q != qs) {
-                                if (pass && outpos < a.cap)
-                                    lst[((int64_t)(outpos >> 3) * nat + li) * kGroup + (outpos & 7)] =
-                                        static_cast<uint16_t>(q);
-                                ++outpos;
-                                ++cnt;
-                            }
+                int count = 0;
+                // one candidate per lane: test, then append the accepted ones
+                // in lane order
+                auto step = [&](int q, bool valid) {
+                    const float4 pj = sp[q];
+                    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    const bool other = valid && q != qs;
+                    bool acc = other && r2 < lo_acc;
+                    if (other && !acc && r2 <= hi_rej) {
+                        // inside the band: the reference predicate, exactly
+                        const int hc = __float_as_int(pj.w);
+                        const int di = (hc >> 16) - (lx + R0), dj = ((hc >> 8) & 0xff) - (ly + R1);
+                        const int dk = (hc & 0xff) - hz0;
+                        const int ux = cx + di, uy = cy + dj, uz = cz + dk;
+                        const double shx = ux < 0 ? -Lx : (ux >= a.g.nc[0] ? Lx : 0.0);
+                        const double shy = uy < 0 ? -Ly : (uy >= a.g.nc[1] ? Ly : 0.0);
+                        const double shzv = uz < 0 ? -Lz : (uz >= a.g.nc[2] ? Lz : 0.0);
+                        const int colkind = (di > 0 || (di == 0 && dj > 0)) ? 1 : ((di == 0 && dj == 0) ? 0 : 2);
+                        const int kind = colkind != 0 ? colkind : (dk > 0 ? 1 : (dk < 0 ? 2 : 0));
+                        acc = exact_pair(a.x, a.y, a.z, a.rc_sq, i, rows[q] & kRowSlotMask, kind, shx, shy, shzv);
+                    }
+                    const unsigned long long m = __ballot(acc);
+                    if (acc) {
+                        const int pos = count + lanes_below(m);
+                        if (pos < a.cap)
+                            lst[((int64_t)(pos >> 3) * nat + li) * kGroup + (pos & 7)] = static_cast<uint16_t>(q);
+                    }
+                    count += __popcll(m);
+                };
+                if (flat) {
+                    for (int k0 = 0; k0 < total; k0 += 64) {
+                        const int cc = k0 + lane;
+                        const bool valid = cc < total;
+                        step(valid ? static_cast<int>(cand[wid][cc]) : qs, valid);
+                    }
+                } else {
+                    for (int col = 0; col < ncols; ++col) {
+                        const int cqb = __shfl(qb, col, 64), clen = __shfl(len, col, 64);
+                        for (int k0 = 0; k0 < clen; k0 += 64) {
+                            const bool valid = k0 + lane < clen;
+                            step(valid ? cqb + k0 + lane : qs, valid);
                         }
                     }
                 }
-                if (!pass)
-                    item_off[li * kMaxCols + col] = static_cast<unsigned short>(cnt);
-            }
-            __syncthreads();
-            if (!pass) {
-                // per-atom exclusive prefix over the columns
-                for (int li = threadIdx.x; li < nat; li += blockDim.x) {
-                    int run = 0;
-                    for (int col = 0; col < ncols; ++col) {
-                        const int c = item_off[li * kMaxCols + col];
-                        item_off[li * kMaxCols + col] = static_cast<unsigned short>(run);
-                        run += c;
-                    }
-                    const int i = a0 + li;
-                    a.numneigh[i] = run < a.cap ? run : a.cap;
-                    a.self16[i] = self_q[li];
-                    my_cnt_sum += run;
-                    my_max = max(my_max, run);
-                    if (run > a.cap)
+                // pad the last group with the atom's own row (r^2 == 0, which the
+                // pair kernels mask), so they can read whole groups
+                const int pad = ((count + kGroup - 1) & ~(kGroup - 1)) - count;
+                if (lane < pad && count + lane < a.cap) {
+                    const int pos = count + lane;
+                    lst[((int64_t)(pos >> 3) * nat + li) * kGroup + (pos & 7)] = static_cast<uint16_t>(qs);
+                }
+                if (lane == 0) {
+                    a.numneigh[i] = count < a.cap ? count : a.cap;
+                    a.self16[i] = static_cast<uint16_t>(qs);
+                    my_cnt_sum += count;
+                    my_max = max(my_max, count);
+                    if (count > a.cap)
                         ++my_over;
                 }
-                __syncthreads();
             }
         }
+        __syncthreads(); // sp and h_off are refilled for the next brick
     }
     for (int off = 32; off > 0; off >>= 1) {
         my_cnt_sum += __shfl_down(my_cnt_sum, off, 64);
         my_max = max(my_max, __shfl_down(my_max, off, 64));
         my_over += __shfl_down(my_over, off, 64);
     }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) {
         sh_tot[wid] = my_cnt_sum;
         sh_max[wid] = my_max;
@@ -702,7 +714,7 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
     if (threadIdx.x == 0) {
         long long t = 0;
         int m = 0, ov = 0;
-        for (int w = 0; w < kBlock / 64; ++w) {
+        for (int w = 0; w < kWaves; ++w) {
             t += sh_tot[w];
             m = max(m, sh_max[w]);
             ov += sh_over[w];
@@ -1233,7 +1245,7 @@ void nl_stage_grids(const atvh_system* s, int g[kNlStages]) {
     g[6] = ga;                            // permute
     g[7] = ga;                            // copy back
     g[8] = std::min(s->nbricks, 4096);    // halo rows
-    g[9] = std::min(s->nbricks, 2048);    // pair build
+    g[9] = std::min(s->nbricks, 16384);   // pair build
 }
 
 void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
@@ -1253,7 +1265,7 @@ void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
     if (s->brick_mode) {
         const BrickBuildArgs ba = nl_brick_args(s, pred);
         k_single<OpHalo><<<g[8], kBlock, 0, st>>>(ba);
-        k_single<OpPairs><<<g[9], kBlock, 0, st>>>(ba);
+        k_single<OpPairs><<<g[9], OpPairs::kThreads, 0, st>>>(ba);
     } else {
         BuildArgs ba{s->x, s->y, s->z, s->cell_of, s->cell_rank, s->cell_start,
                      reinterpret_cast<const int4*>(s->d_stencil), s->nstencil, s->nc[0], s->nc[1],
@@ -1301,7 +1313,7 @@ void nl_batch_build(atvh_batch* b) {
 template <class Op>
 static void nl_batch_one(atvh_batch* b, const char* name) {
     const BatchStage& st = b->stages.at(name);
-    k_batched<Op><<<st.nblocks, kBlock, 0, b->ctx->stream>>>(static_cast<const typename Op::Args*>(st.args),
+    k_batched<Op><<<st.nblocks, op_threads<Op>::value, 0, b->ctx->stream>>>(static_cast<const typename Op::Args*>(st.args),
                                                               st.base, static_cast<int>(b->sys.size()));
 }
 

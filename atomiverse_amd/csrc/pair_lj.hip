@@ -234,7 +234,12 @@ __device__ void OpLjBrickFp64<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid,
     const double4* __restrict__ tab = a.tab;
     const int ntypes = a.ntypes;
     const uint4* __restrict__ base = reinterpret_cast<const uint4*>(a.nbr16 + (int64_t)a0 * a.cap);
-    for (int li = threadIdx.x; li < nat; li += blockDim.x) {
+    // two threads per atom when the brick has at most half a block of atoms:
+    // lanes 2l and 2l+1 take alternate neighbour groups of atom l
+    const int tpa = (2 * nat <= static_cast<int>(blockDim.x)) ? 2 : 1;
+    for (int t = threadIdx.x; t < nat * tpa; t += blockDim.x) {
+        const int li = tpa == 2 ? (t >> 1) : t;
+        const int h = tpa == 2 ? (t & 1) : 0;
         const int i = a0 + li;
         const int qs = a.self16[i];
         const double xi = sx[qs], yi = sy[qs], zi = sz[qs];
@@ -243,57 +248,70 @@ __device__ void OpLjBrickFp64<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid,
         const uint4* __restrict__ grp = base + li;
         const int ng = (nn + kGroup - 1) / kGroup;
         double fxi = 0.0, fyi = 0.0, fzi = 0.0;
-        uint4 cur = grp[0]; // in bounds even when nn == 0: the brick's rows always exist
-        for (int g = 0; g < ng; ++g) {
-            // prefetch the next 8 indices (the last group re-reads itself)
-            const uint4 nxt = grp[(int64_t)min(g + 1, ng - 1) * nat];
-            const int rem = nn - g * kGroup;
-            auto pair = [&](int q) {
-                {
-                    const double dx = xi - sx[q];
-                    const double dy = yi - sy[q];
-                    const double dz = zi - sz[q];
-                    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                    if (r2 < rc_sq) {
-                        double eps24 = p_eps24, eps4 = p_eps4, sig6 = p_sig6, shift = p_shift;
-                        if (TYPED) {
-                            const double4 t = tab[ti * ntypes + stype[q]];
-                            eps24 = t.x;
-                            eps4 = t.y;
-                            sig6 = t.z;
-                            shift = t.w;
-                        }
-                        const double r2inv = 1.0 / r2;
-                        const double r6inv = r2inv * r2inv * r2inv * sig6;
-                        const double fpair = eps24 * r2inv * fma(2.0 * r6inv, r6inv, -r6inv);
-                        fxi = fma(fpair, dx, fxi);
-                        fyi = fma(fpair, dy, fyi);
-                        fzi = fma(fpair, dz, fzi);
-                        if (EF)
-                            acc[0] += fma(eps4, r6inv * r6inv - r6inv, -shift);
-                        if (VF) {
-                            const double fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
-                            acc[1] += dx * fxp;
-                            acc[2] += dy * fyp;
-                            acc[3] += dz * fzp;
-                            acc[4] += dx * fyp;
-                            acc[5] += dx * fzp;
-                            acc[6] += dy * fzp;
-                        }
-                    }
-                }
-            };
-            if (rem >= kGroup) {
-                pair(cur.x & 0xffffu); pair(cur.x >> 16); pair(cur.y & 0xffffu); pair(cur.y >> 16);
-                pair(cur.z & 0xffffu); pair(cur.z >> 16); pair(cur.w & 0xffffu); pair(cur.w >> 16);
-            } else {
-                const unsigned long long lo = cur.x | (static_cast<unsigned long long>(cur.y) << 32);
-                const unsigned long long hi = cur.z | (static_cast<unsigned long long>(cur.w) << 32);
-                for (int u = 0; u < rem; ++u)
-                    pair(static_cast<int>(((u < 4 ? lo : hi) >> (16 * (u & 3))) & 0xffffu));
+        // group indices clamp to the last group (to group 0 when nn == 0: the
+        // brick's rows always exist); loads run two groups ahead
+        const int gl = max(ng - 1, 0);
+        uint4 cur = grp[(int64_t)min(h, gl) * nat];
+        uint4 nx1 = grp[(int64_t)min(h + tpa, gl) * nat];
+        for (int g = h; g < ng; g += tpa) {
+            const uint4 nx2 = grp[(int64_t)min(g + 2 * tpa, gl) * nat];
+            // the whole group's LDS reads issue before any arithmetic; padding
+            // rows (the atom itself, r^2 == 0) and pairs beyond r_c add zero
+            const unsigned w[4] = {cur.x, cur.y, cur.z, cur.w};
+            double jx[kGroup], jy[kGroup], jz[kGroup];
+            int tj[kGroup];
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) {
+                const int q = (w[u >> 1] >> (16 * (u & 1))) & 0xffff;
+                jx[u] = sx[q];
+                jy[u] = sy[q];
+                jz[u] = sz[q];
+                if (TYPED)
+                    tj[u] = stype[q];
             }
-            cur = nxt;
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) {
+                const double dx = xi - jx[u];
+                const double dy = yi - jy[u];
+                const double dz = zi - jz[u];
+                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                const bool in = r2 < rc_sq && r2 > 0.0;
+                double eps24 = p_eps24, eps4 = p_eps4, sig6 = p_sig6, shift = p_shift;
+                if (TYPED) {
+                    const double4 t = tab[ti * ntypes + tj[u]];
+                    eps24 = t.x;
+                    eps4 = t.y;
+                    sig6 = t.z;
+                    shift = t.w;
+                }
+                const double r2inv = 1.0 / r2;
+                const double r6inv = r2inv * r2inv * r2inv * sig6;
+                const double fpair = in ? eps24 * r2inv * fma(2.0 * r6inv, r6inv, -r6inv) : 0.0;
+                fxi = fma(fpair, dx, fxi);
+                fyi = fma(fpair, dy, fyi);
+                fzi = fma(fpair, dz, fzi);
+                if (EF && in)
+                    acc[0] += fma(eps4, r6inv * r6inv - r6inv, -shift);
+                if (VF && in) {
+                    const double fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
+                    acc[1] += dx * fxp;
+                    acc[2] += dy * fyp;
+                    acc[3] += dz * fzp;
+                    acc[4] += dx * fyp;
+                    acc[5] += dx * fzp;
+                    acc[6] += dy * fzp;
+                }
+            }
+            cur = nx1;
+            nx1 = nx2;
         }
+        if (tpa == 2) {
+            fxi += __shfl_xor(fxi, 1, 64);
+            fyi += __shfl_xor(fyi, 1, 64);
+            fzi += __shfl_xor(fzi, 1, 64);
+        }
+        if (h != 0)
+            continue;
         a.fx[i] = fxi;
         a.fy[i] = fyi;
         a.fz[i] = fzi;
@@ -349,7 +367,12 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
     const double4* __restrict__ tab = a.tab;
     const int ntypes = a.ntypes;
     const uint4* __restrict__ base = reinterpret_cast<const uint4*>(a.nbr16 + (int64_t)a0 * a.cap);
-    for (int li = threadIdx.x; li < nat; li += blockDim.x) {
+    // two threads per atom when the brick has at most half a block of atoms:
+    // lanes 2l and 2l+1 take alternate neighbour groups of atom l
+    const int tpa = (2 * nat <= static_cast<int>(blockDim.x)) ? 2 : 1;
+    for (int t = threadIdx.x; t < nat * tpa; t += blockDim.x) {
+        const int li = tpa == 2 ? (t >> 1) : t;
+        const int h = tpa == 2 ? (t & 1) : 0;
         const int i = a0 + li;
         const float4 pi = sp[a.self16[i]];
         const int ti = TYPED ? __float_as_int(pi.w) : 0;
@@ -357,58 +380,63 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
         const uint4* __restrict__ grp = base + li;
         const int ng = (nn + kGroup - 1) / kGroup;
         float fxi = 0.f, fyi = 0.f, fzi = 0.f;
-        uint4 cur = grp[0]; // in bounds even when nn == 0: the brick's rows always exist
-        for (int g = 0; g < ng; ++g) {
-            // prefetch the next 8 indices (the last group re-reads itself)
-            const uint4 nxt = grp[(int64_t)min(g + 1, ng - 1) * nat];
-            const int rem = nn - g * kGroup;
-            auto pair = [&](int q) {
-                {
-                    const float4 pj = sp[q];
-                    const float dx = pi.x - pj.x;
-                    const float dy = pi.y - pj.y;
-                    const float dz = pi.z - pj.z;
-                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                    if (r2 < rc_sq) {
-                        float eps24 = eps24s, eps4 = eps4s, sig6 = sig6s, shift = shifts;
-                        if (TYPED) {
-                            const double4 t = tab[ti * ntypes + __float_as_int(pj.w)];
-                            eps24 = static_cast<float>(t.x);
-                            eps4 = static_cast<float>(t.y);
-                            sig6 = static_cast<float>(t.z);
-                            shift = static_cast<float>(t.w);
-                        }
-                        const float r2inv = __builtin_amdgcn_rcpf(r2);
-                        const float r6inv = r2inv * r2inv * r2inv * sig6;
-                        const float fpair = eps24 * r2inv * __builtin_fmaf(2.0f * r6inv, r6inv, -r6inv);
-                        fxi = __builtin_fmaf(fpair, dx, fxi);
-                        fyi = __builtin_fmaf(fpair, dy, fyi);
-                        fzi = __builtin_fmaf(fpair, dz, fzi);
-                        if (EF)
-                            acc[0] += static_cast<double>(__builtin_fmaf(eps4, r6inv * r6inv - r6inv, -shift));
-                        if (VF) {
-                            const float fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
-                            acc[1] += static_cast<double>(dx * fxp);
-                            acc[2] += static_cast<double>(dy * fyp);
-                            acc[3] += static_cast<double>(dz * fzp);
-                            acc[4] += static_cast<double>(dx * fyp);
-                            acc[5] += static_cast<double>(dx * fzp);
-                            acc[6] += static_cast<double>(dy * fzp);
-                        }
-                    }
+        // group indices clamp to the last group (to group 0 when nn == 0: the
+        // brick's rows always exist); loads run two groups ahead
+        const int gl = max(ng - 1, 0);
+        uint4 cur = grp[(int64_t)min(h, gl) * nat];
+        uint4 nx1 = grp[(int64_t)min(h + tpa, gl) * nat];
+        for (int g = h; g < ng; g += tpa) {
+            const uint4 nx2 = grp[(int64_t)min(g + 2 * tpa, gl) * nat];
+            // the whole group's LDS reads issue before any arithmetic; padding
+            // rows (the atom itself, r^2 == 0) and pairs beyond r_c add zero
+            const unsigned w[4] = {cur.x, cur.y, cur.z, cur.w};
+            float4 pj[kGroup];
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u)
+                pj[u] = sp[(w[u >> 1] >> (16 * (u & 1))) & 0xffff];
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) {
+                const float dx = pi.x - pj[u].x;
+                const float dy = pi.y - pj[u].y;
+                const float dz = pi.z - pj[u].z;
+                const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                const bool in = r2 < rc_sq && r2 > 0.0f;
+                float eps24 = eps24s, eps4 = eps4s, sig6 = sig6s, shift = shifts;
+                if (TYPED) {
+                    const double4 t = tab[ti * ntypes + __float_as_int(pj[u].w)];
+                    eps24 = static_cast<float>(t.x);
+                    eps4 = static_cast<float>(t.y);
+                    sig6 = static_cast<float>(t.z);
+                    shift = static_cast<float>(t.w);
                 }
-            };
-            if (rem >= kGroup) {
-                pair(cur.x & 0xffffu); pair(cur.x >> 16); pair(cur.y & 0xffffu); pair(cur.y >> 16);
-                pair(cur.z & 0xffffu); pair(cur.z >> 16); pair(cur.w & 0xffffu); pair(cur.w >> 16);
-            } else {
-                const unsigned long long lo = cur.x | (static_cast<unsigned long long>(cur.y) << 32);
-                const unsigned long long hi = cur.z | (static_cast<unsigned long long>(cur.w) << 32);
-                for (int u = 0; u < rem; ++u)
-                    pair(static_cast<int>(((u < 4 ? lo : hi) >> (16 * (u & 3))) & 0xffffu));
+                const float r2inv = __builtin_amdgcn_rcpf(r2);
+                const float r6inv = r2inv * r2inv * r2inv * sig6;
+                const float fpair = in ? eps24 * r2inv * __builtin_fmaf(2.0f * r6inv, r6inv, -r6inv) : 0.0f;
+                fxi = __builtin_fmaf(fpair, dx, fxi);
+                fyi = __builtin_fmaf(fpair, dy, fyi);
+                fzi = __builtin_fmaf(fpair, dz, fzi);
+                if (EF && in)
+                    acc[0] += static_cast<double>(__builtin_fmaf(eps4, r6inv * r6inv - r6inv, -shift));
+                if (VF && in) {
+                    const float fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
+                    acc[1] += static_cast<double>(dx * fxp);
+                    acc[2] += static_cast<double>(dy * fyp);
+                    acc[3] += static_cast<double>(dz * fzp);
+                    acc[4] += static_cast<double>(dx * fyp);
+                    acc[5] += static_cast<double>(dx * fzp);
+                    acc[6] += static_cast<double>(dy * fzp);
+                }
             }
-            cur = nxt;
+            cur = nx1;
+            nx1 = nx2;
         }
+        if (tpa == 2) {
+            fxi += __shfl_xor(fxi, 1, 64);
+            fyi += __shfl_xor(fyi, 1, 64);
+            fzi += __shfl_xor(fzi, 1, 64);
+        }
+        if (h != 0)
+            continue;
         a.fx[i] = static_cast<double>(fxi);
         a.fy[i] = static_cast<double>(fyi);
         a.fz[i] = static_cast<double>(fzi);

@@ -75,8 +75,10 @@ void check_rc(int rc, atvh_ctx* c, const char* what) {
 
 // Device bytes a segment occupies: per-atom state and sort scratch
 // (~45 words), the 16-bit full neighbour list at 1.5x the mean count + 32,
-// per-brick halo tables, and the PME mesh (fixed-point, real and half
-// spectrum grids of both precisions, influence table) for k-space segments.
+// per-brick halo tables, and for k-space segments the PME mesh (fixed-point,
+// real and half spectrum grids of both precisions, influence table) plus two
+// FFT plan pairs at the largest footprint one pair has taken in this process
+// (64 MiB each before any plan was made).
 int64_t estimate_bytes(const atvh_segment& g) {
     const double vol = g.box[0] * g.box[1] * g.box[2];
     const double rcs = g.cutoff + (g.skin < 0 ? 0.3 * g.cutoff : g.skin);
@@ -92,7 +94,7 @@ int64_t estimate_bytes(const atvh_segment& g) {
         for (int d = 0; d < 3; ++d)
             K *= std::ceil(g.box[d] / g.pme_spacing) + 8;
         bytes += K * (8.0 + 8.0 + 4.0 + 8.0 + 4.0 + 4.0);
-        bytes += 4.0 * 1024 * 1024; // FFT plans and work areas
+        bytes += 2.0 * static_cast<double>(fft_pair_bytes()); // FFT plan pairs of both precisions
     }
     return static_cast<int64_t>(bytes * 1.1 + (1 << 20));
 }
