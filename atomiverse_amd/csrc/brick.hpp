@@ -41,6 +41,19 @@ struct HaloTable {
     int natoms;    // halo atoms
 };
 
+// Brick taken by work index t of n. The hardware deals consecutive
+// workgroups round-robin to the 8 XCDs, each with its own L2; this mapping
+// gives XCD k (t % 8 == k) one contiguous run of bricks, so bricks that
+// share halo atoms (numbered z-fastest, neighbours are close in index) are
+// read through the same L2. A bijection of [0, n) for any n.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int xcd_brick(int t, int n) {
+    const int per = n / kXcds, rem = n % kXcds;
+    const int k = t % kXcds, j = t / kXcds;
+    return k * per + min(k, rem) + j;
+}
+
 __device__ __forceinline__ void brick_coords(const BrickGeom& g, int b, int& bx, int& by, int& bz) {
     bz = b % g.nb[2];
     by = (b / g.nb[2]) % g.nb[1];

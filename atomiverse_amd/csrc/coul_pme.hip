@@ -131,7 +131,7 @@ template <bool EF, bool VF>
 __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
     __shared__ double sx[kHaloCap], sy[kHaloCap], sz[kHaloCap], sq[kHaloCap];
     __shared__ int stype[kHaloCap];
-    const int b = blockIdx.x;
+    const int b = xcd_brick(blockIdx.x, gridDim.x); // one launch block per brick
     const int nh = a.halo_n[b];
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     if (nh > kHaloCap) { // reported by the build; nothing valid to compute
@@ -249,7 +249,7 @@ template <bool EF, bool VF>
 __global__ __launch_bounds__(kBlock) void k_ljc_brick_mixed(LjcArgs a) {
     __shared__ float4 sp[kHaloCap]; // x, y, z relative to the brick centre; w = charge
     __shared__ unsigned char stype[kHaloCap];
-    const int b = blockIdx.x;
+    const int b = xcd_brick(blockIdx.x, gridDim.x); // one launch block per brick
     const int nh = a.halo_n[b];
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     if (nh > kHaloCap) {
@@ -676,7 +676,8 @@ __global__ __launch_bounds__(kBlock) void k_pme_spread_lds(PmeArgs a, const int2
                                                            int nranges) {
     __shared__ unsigned long long acc[kSpreadCap];
     __shared__ int red[kBlock / 64][6];
-    for (int r = blockIdx.x; r < nranges; r += gridDim.x) {
+    for (int t = blockIdx.x; t < nranges; t += gridDim.x) {
+        const int r = xcd_brick(t, nranges);
         RangeBox rb;
         if (!range_of(a, ranges, r, rb))
             continue;

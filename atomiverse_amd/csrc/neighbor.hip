@@ -468,7 +468,8 @@ __device__ void OpHalo::run(const BrickBuildArgs& a, int bid, int nbk) {
     __shared__ int h_off[kHaloCellCap + 1];
     __shared__ int h_shift[kHaloCellCap];
     __shared__ int scan_tmp[kBlock];
-    for (int b = bid; b < a.nbricks; b += nbk) {
+    for (int t = bid; t < a.nbricks; t += nbk) {
+        const int b = xcd_brick(t, a.nbricks);
         HaloTable ht;
         build_halo_table(a.g, a.halo_cells, a.start, b, ht, h_start, h_off, h_shift, scan_tmp);
         const int a0 = a.start[a.g.brick_cell_base[b]];
@@ -497,18 +498,20 @@ __device__ void OpHalo::run(const BrickBuildArgs& a, int bid, int nbk) {
 
 // Rebuild step 2 (one workgroup per brick, one wavefront per home cell). The
 // wave lays the cell's candidates out flat — every stencil column's z-run of
-// halo rows, in column order — and then, for each atom of the cell, takes 64
-// candidates per step, one per lane, so the candidate loop does not diverge.
-// Candidates are screened with FP32 image coordinates (relative to the brick
-// centre: |error of r^2| <= 1e-5 Å^2): accepted below (rc+skin)^2 - kBand,
-// rejected above (rc+skin)^2 + kBand, and only inside the band decided by the
-// exact reference predicate — raw FP64 positions, the shift form of the
-// half-stencil offset in its canonical orientation (refresh_half,
-// cell_list.cpp:680-727). The pair set is therefore exactly the reference's.
-// Accepted rows are compacted with a ballot in candidate order (column order,
-// then halo row order: deterministic) into 16-bit rows in 8-wide groups; the
-// unused tail of the last group holds the atom's own row. A cell with more
-// than kCandCap candidates walks its columns one at a time instead.
+// halo rows, in column order — and takes them 64 at a time, one per lane,
+// testing each against every atom of the cell (the atoms' coordinates and
+// running counts live one per lane and are read with readlane), so the
+// candidate loop does not diverge. Candidates are screened with FP32 image
+// coordinates (relative to the brick centre: |error of r^2| <= 1e-5 Å^2):
+// accepted below (rc+skin)^2 - kBand, rejected above (rc+skin)^2 + kBand, and
+// only inside the band decided by the exact reference predicate — raw FP64
+// positions, the shift form of the half-stencil offset in its canonical
+// orientation (refresh_half, cell_list.cpp:680-727). The pair set is
+// therefore exactly the reference's. Accepted rows are compacted with a
+// ballot in candidate order (column order, then halo row order:
+// deterministic) into 16-bit rows in 8-wide groups; the unused tail of the
+// last group holds the atom's own row. A cell with more than kCandCap
+// candidates walks its columns one at a time instead.
 constexpr float kBand = 1e-4f;
 constexpr int kCandCap = 768; // flat candidates per wavefront (LDS)
 
@@ -531,10 +534,45 @@ __device__ __noinline__ bool exact_pair(const double* __restrict__ x, const doub
     return r2 < rc_sq;
 }
 
+// The reference predicate for a candidate inside the FP32 band: the stencil
+// offset (di, dj, dk) from the halo cell packed in hc, the periodic shift of
+// the neighbour cell, and the exact FP64 test in the canonical orientation.
+// (Arguments by value: a reference to the kernel's argument block would be
+// copied to scratch to take its address.)
+__device__ __noinline__ bool band_pair(const double* __restrict__ x, const double* __restrict__ y,
+                                       const double* __restrict__ z, double rc_sq, double Lx, double Ly, double Lz,
+                                       int ncx, int ncy, int ncz, int hc, int hx0, int hy0, int hz0, int cx, int cy,
+                                       int cz, int gi, int gj) {
+    const int di = (hc >> 16) - hx0, dj = ((hc >> 8) & 0xff) - hy0, dk = (hc & 0xff) - hz0;
+    const int ux = cx + di, uy = cy + dj, uz = cz + dk;
+    const double shx = ux < 0 ? -Lx : (ux >= ncx ? Lx : 0.0);
+    const double shy = uy < 0 ? -Ly : (uy >= ncy ? Ly : 0.0);
+    const double shz = uz < 0 ? -Lz : (uz >= ncz ? Lz : 0.0);
+    const int colkind = (di > 0 || (di == 0 && dj > 0)) ? 1 : ((di == 0 && dj == 0) ? 0 : 2);
+    const int kind = colkind != 0 ? colkind : (dk > 0 ? 1 : (dk < 0 ? 2 : 0));
+    return exact_pair(x, y, z, rc_sq, gi, gj, kind, shx, shy, shz);
+}
+
 // Number of set bits of m below this lane.
 __device__ __forceinline__ int lanes_below(unsigned long long m) {
     return __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(m >> 32),
                                      __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(m), 0u));
+}
+
+// Inclusive prefix sum over the 64 lanes (DPP row shifts, then the row
+// broadcasts of lanes 15 and 31).
+__device__ __forceinline__ int wave_inclusive_sum(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true); // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true); // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true); // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true); // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false); // row_bcast:15 -> rows 1, 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false); // row_bcast:31 -> rows 2, 3
+    return x;
+}
+
+__device__ __forceinline__ float lane_f(float v, int k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
 struct OpPairs {
@@ -552,7 +590,10 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
     __shared__ unsigned short cand[kWaves][kCandCap];
     __shared__ long long sh_tot[kWaves];
     __shared__ int sh_max[kWaves], sh_over[kWaves];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    // wave-uniform values go through readfirstlane so their loops run on
+    // scalar registers
+    const int lane = threadIdx.x & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     long long my_cnt_sum = 0;
     int my_max = 0, my_over = 0;
     const double Lx = a.g.L[0], Ly = a.g.L[1], Lz = a.g.L[2];
@@ -560,12 +601,16 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
     const float rc2f = static_cast<float>(a.rc_sq);
     const float lo_acc = rc2f - kBand, hi_rej = rc2f + kBand;
     const int ncols = a.nrows;
-    for (int b = bid; b < a.nbricks; b += nbk) {
+    // stencil column of this lane (lanes >= ncols have none)
+    const int4 rw = lane < ncols ? a.rows[lane] : make_int4(0, 0, 0, 0);
+    for (int t = bid; t < a.nbricks; t += nbk) {
+        const int b = xcd_brick(t, a.nbricks);
         const int nh = a.halo_n[b];
         const int2 br = a.brick_range[b];
         const int a0 = br.x, nat = br.y;
         if (nh > kHaloCap || nat > kMaxBrickAtoms)
             continue; // reported by OpHalo (block-uniform)
+        const int cbase = a.g.brick_cell_base[b];
         int bx, by, bz;
         brick_coords(a.g, b, bx, by, bz);
         const int o0 = bx * a.g.bdim[0], o1 = by * a.g.bdim[1], o2 = bz * a.g.bdim[2];
@@ -602,28 +647,26 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
         uint16_t* __restrict__ lst = a.nbr16 + (int64_t)a0 * a.cap;
         const int nbc = d0 * d1 * d2;
         for (int c = wid; c < nbc; c += kWaves) {
+            // cells of a brick are ranked in this order, so the cell's atoms are
+            // the slots from cell_start of its rank on
+            const int slot0 = __builtin_amdgcn_readfirstlane(a.start[cbase + c]);
             const int lz = c % d2, ly = (c / d2) % d1, lx = c / (d1 * d2);
             const int hz0 = lz + R2;
             const int hs = ((lx + R0) * HY + (ly + R1)) * HZ + hz0;
-            const int hq0 = h_off[hs], hq1 = h_off[hs + 1];
+            const int hq0 = __builtin_amdgcn_readfirstlane(h_off[hs]);
+            const int hq1 = __builtin_amdgcn_readfirstlane(h_off[hs + 1]);
             if (hq0 == hq1)
                 continue;
             const int cx = o0 + lx, cy = o1 + ly, cz = o2 + lz;
             // this cell's z-run of halo rows in every stencil column (lane = column)
             int qb = 0, len = 0;
             if (lane < ncols) {
-                const int4 rw = a.rows[lane];
                 const int hcol = ((lx + R0 + rw.x) * HY + (ly + R1 + rw.y)) * HZ + hz0;
                 qb = h_off[hcol - rw.z];
                 len = h_off[hcol + rw.z + 1] - qb;
             }
-            int inc = len;
-            for (int off = 1; off < 64; off <<= 1) {
-                const int t = __shfl_up(inc, off, 64);
-                if (lane >= off)
-                    inc += t;
-            }
-            const int total = __shfl(inc, 63, 64);
+            const int inc = wave_inclusive_sum(len);
+            const int total = __builtin_amdgcn_readlane(inc, 63);
             const bool flat = total <= kCandCap;
             if (flat) {
                 const int ex = inc - len;
@@ -631,69 +674,75 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
                     cand[wid][ex + t] = static_cast<unsigned short>(qb + t);
                 __builtin_amdgcn_wave_barrier();
             }
-            const int slot0 = rows[hq0] & kRowSlotMask; // the cell's first atom
-            for (int qs = hq0; qs < hq1; ++qs) {
-                const int i = slot0 + (qs - hq0);
-                const int li = i - a0;
-                const float4 pi = sp[qs];
-                int count = 0;
-                // one candidate per lane: test, then append the accepted ones
-                // in lane order
+            const int li0 = slot0 - a0; // brick-local index of the cell's first atom
+            const unsigned gstride = static_cast<unsigned>(nat) * kGroup; // list entries between groups
+            for (int ka = hq0; ka < hq1; ka += 64) {
+                // up to 64 atoms of the cell, atom ka + k on lane k
+                const int na = __builtin_amdgcn_readfirstlane(min(64, hq1 - ka));
+                const float4 pa = sp[ka + min(lane, na - 1)];
+                int cnt = 0;
+                // one candidate per lane against every atom of the group
                 auto step = [&](int q, bool valid) {
                     const float4 pj = sp[q];
-                    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                    const bool other = valid && q != qs;
-                    bool acc = other && r2 < lo_acc;
-                    if (other && !acc && r2 <= hi_rej) {
-                        // inside the band: the reference predicate, exactly
-                        const int hc = __float_as_int(pj.w);
-                        const int di = (hc >> 16) - (lx + R0), dj = ((hc >> 8) & 0xff) - (ly + R1);
-                        const int dk = (hc & 0xff) - hz0;
-                        const int ux = cx + di, uy = cy + dj, uz = cz + dk;
-                        const double shx = ux < 0 ? -Lx : (ux >= a.g.nc[0] ? Lx : 0.0);
-                        const double shy = uy < 0 ? -Ly : (uy >= a.g.nc[1] ? Ly : 0.0);
-                        const double shzv = uz < 0 ? -Lz : (uz >= a.g.nc[2] ? Lz : 0.0);
-                        const int colkind = (di > 0 || (di == 0 && dj > 0)) ? 1 : ((di == 0 && dj == 0) ? 0 : 2);
-                        const int kind = colkind != 0 ? colkind : (dk > 0 ? 1 : (dk < 0 ? 2 : 0));
-                        acc = exact_pair(a.x, a.y, a.z, a.rc_sq, i, rows[q] & kRowSlotMask, kind, shx, shy, shzv);
+                    for (int k = 0; k < na; ++k) {
+                        const int qs = ka + k;
+                        const float dx = lane_f(pa.x, k) - pj.x;
+                        const float dy = lane_f(pa.y, k) - pj.y;
+                        const float dz = lane_f(pa.z, k) - pj.z;
+                        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                        const bool other = valid && q != qs;
+                        bool acc = other && r2 < lo_acc;
+                        const int li = li0 + (qs - hq0);
+                        if (other && !acc && r2 <= hi_rej) // inside the band: the reference predicate
+                            acc = band_pair(a.x, a.y, a.z, a.rc_sq, Lx, Ly, Lz, a.g.nc[0], a.g.nc[1], a.g.nc[2],
+                                            __float_as_int(pj.w), lx + R0, ly + R1, hz0, cx, cy, cz, a0 + li,
+                                            rows[q] & kRowSlotMask);
+                        const unsigned long long m = __ballot(acc);
+                        const int base = __builtin_amdgcn_readlane(cnt, k);
+                        if (acc) {
+                            const unsigned pos = static_cast<unsigned>(base + lanes_below(m));
+                            if (pos < static_cast<unsigned>(a.cap))
+                                lst[static_cast<unsigned>(li) * kGroup + (pos >> 3) * gstride + (pos & 7)] =
+                                    static_cast<uint16_t>(q);
+                        }
+                        if (lane == k)
+                            cnt = base + static_cast<int>(__popcll(m));
                     }
-                    const unsigned long long m = __ballot(acc);
-                    if (acc) {
-                        const int pos = count + lanes_below(m);
-                        if (pos < a.cap)
-                            lst[((int64_t)(pos >> 3) * nat + li) * kGroup + (pos & 7)] = static_cast<uint16_t>(q);
-                    }
-                    count += __popcll(m);
                 };
                 if (flat) {
                     for (int k0 = 0; k0 < total; k0 += 64) {
                         const int cc = k0 + lane;
                         const bool valid = cc < total;
-                        step(valid ? static_cast<int>(cand[wid][cc]) : qs, valid);
+                        step(valid ? static_cast<int>(cand[wid][cc]) : ka, valid);
                     }
                 } else {
                     for (int col = 0; col < ncols; ++col) {
-                        const int cqb = __shfl(qb, col, 64), clen = __shfl(len, col, 64);
+                        const int cqb = __builtin_amdgcn_readlane(qb, col);
+                        const int clen = __builtin_amdgcn_readlane(len, col);
                         for (int k0 = 0; k0 < clen; k0 += 64) {
                             const bool valid = k0 + lane < clen;
-                            step(valid ? cqb + k0 + lane : qs, valid);
+                            step(valid ? cqb + k0 + lane : ka, valid);
                         }
                     }
                 }
-                // pad the last group with the atom's own row (r^2 == 0, which the
-                // pair kernels mask), so they can read whole groups
-                const int pad = ((count + kGroup - 1) & ~(kGroup - 1)) - count;
-                if (lane < pad && count + lane < a.cap) {
-                    const int pos = count + lane;
-                    lst[((int64_t)(pos >> 3) * nat + li) * kGroup + (pos & 7)] = static_cast<uint16_t>(qs);
+                // pad each atom's last group with its own row (r^2 == 0, which
+                // the pair kernels mask), so they can read whole groups
+                for (int k = 0; k < na; ++k) {
+                    const int base = __builtin_amdgcn_readlane(cnt, k);
+                    const int pos = base + lane;
+                    if (pos < ((base + kGroup - 1) & ~(kGroup - 1)) && pos < a.cap) {
+                        const unsigned li = static_cast<unsigned>(li0 + (ka + k - hq0));
+                        lst[li * kGroup + static_cast<unsigned>(pos >> 3) * gstride + (pos & 7)] =
+                            static_cast<uint16_t>(ka + k);
+                    }
                 }
-                if (lane == 0) {
-                    a.numneigh[i] = count < a.cap ? count : a.cap;
-                    a.self16[i] = static_cast<uint16_t>(qs);
-                    my_cnt_sum += count;
-                    my_max = max(my_max, count);
-                    if (count > a.cap)
+                if (lane < na) {
+                    const int i = slot0 + (ka + lane - hq0);
+                    a.numneigh[i] = cnt < a.cap ? cnt : a.cap;
+                    a.self16[i] = static_cast<uint16_t>(ka + lane);
+                    my_cnt_sum += cnt;
+                    my_max = max(my_max, cnt);
+                    if (cnt > a.cap)
                         ++my_over;
                 }
             }
@@ -1245,7 +1294,7 @@ void nl_stage_grids(const atvh_system* s, int g[kNlStages]) {
     g[6] = ga;                            // permute
     g[7] = ga;                            // copy back
     g[8] = std::min(s->nbricks, 4096);    // halo rows
-    g[9] = std::min(s->nbricks, 16384);   // pair build
+    g[9] = s->nbricks;                    // pair build
 }
 
 void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {

@@ -202,10 +202,10 @@ struct OpLjBrickFp64 {
 };
 
 template <bool EF, bool VF, bool TYPED>
-__device__ void OpLjBrickFp64<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid, int) {
+__device__ void OpLjBrickFp64<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid, int nbk) {
     __shared__ double sx[kHaloCap], sy[kHaloCap], sz[kHaloCap];
     __shared__ int stype[TYPED ? kHaloCap : 1];
-    const int b = bid;
+    const int b = xcd_brick(bid, nbk); // one launch block per brick
     const int nh = a.halo_n[b];
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     if (nh > kHaloCap) { // reported by the build; nothing valid to compute
@@ -331,9 +331,9 @@ struct OpLjBrickMixed {
 };
 
 template <bool EF, bool VF, bool TYPED>
-__device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid, int) {
+__device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid, int nbk) {
     __shared__ float4 sp[kHaloCap]; // x, y, z relative to the brick centre; w = type bits
-    const int b = bid;
+    const int b = xcd_brick(bid, nbk); // one launch block per brick
     const int nh = a.halo_n[b];
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     if (nh > kHaloCap) {

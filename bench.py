@@ -348,7 +348,7 @@ def main():
         "fp64_ms_per_step": res["fp64"]["dt"] * 1e3 / steps,
         "mixed_over_fp64": value / fp64_value,
         "force_rms_rel_err_mixed_vs_fp64": rms,
-        "roofline": {"bound": "hbm", "kernel": "k_lj_brick_mixed", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": "k_single<OpLjBrickMixed<false, false, false>>", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "bytes_alg_per_launch": bytes_alg, "avg_launch_us": avg_s * 1e6},
         "kernel_avg_us": {name: {k: v[0] * 1e3 / max(v[1], 1) for k, v in r["kt"].items()}
