@@ -115,12 +115,48 @@ def charged_section(av, W, ctx, steps, warmup):
     return out
 
 
+def nve_section(av, W, ctx, c1_steps, c2_steps, every=100):
+    """NVE side runs from the C1 and C2 starting states (thermostat off), both
+    precisions: total energy per atom recorded every `every` steps, its linear
+    drift in eV/atom/ns, and the largest deviation from the first record."""
+    out = {}
+    for label, make, steps in (("lj_fcc_2048000", lambda: W.lj_c1(thermostat=0), c1_steps),
+                               ("charged_melt_512000", lambda: W.charged_melt(80, thermostat=0), c2_steps)):
+        if steps <= 0:
+            continue
+        out[label] = {"steps": steps, "record_every": every}
+        for name, prec in (("fp64", av.PREC_FP64), ("mixed", av.PREC_MIXED)):
+            w = make()
+            log(f"bench: NVE side run {label} {name}, {steps} steps")
+            s = av.System(ctx, w.n, w.box)
+            W.upload(s, w)
+            W.configure(s, w)
+            s.nlist_build()
+            s.run(steps, prec, energy_every=every)
+            pe, ke = s.thermo(steps)
+            s.close()
+            e = (pe + ke) / w.n
+            k = np.nonzero(np.isfinite(e))[0]
+            t_fs = (k + 1) * w.dt
+            slope = float(np.polyfit(t_fs, e[k], 1)[0])  # eV/atom/fs
+            out[label][name] = {"drift_ev_per_atom_per_ns": slope * 1e6,
+                                "max_abs_dev_ev_per_atom": float(np.max(np.abs(e[k] - e[k[0]]))),
+                                "span_fs": float(t_fs[-1] - t_fs[0])}
+            log(f"  drift {slope * 1e6:.3e} eV/atom/ns, max |E-E0| {out[label][name]['max_abs_dev_ev_per_atom']:.3e}"
+                " eV/atom")
+    return out
+
+
 def ladder_section(av, W, ctx, steps, loop_steps):
     """Batched REMD-style ensemble: 256 LJ fluids (N log-uniform in [500, 20000],
     T* ladder 0.7..1.5, NVT, mixed), one batched step per launch stage (hipGraph
-    replay) against a loop of plain per-system launches over the same members."""
+    replay) against a loop of plain per-system launches over the same members.
+    Brick shapes come from the batched autotuner (atomiverse_amd.tuning), per
+    size class; the loop uses the same shapes."""
+    from atomiverse_amd.tuning import autotune_batch
     ws = W.lj_ladder(256)
     total = sum(w.n for w in ws)
+    shapes = []
 
     def make():
         out = []
@@ -130,10 +166,18 @@ def ladder_section(av, W, ctx, steps, loop_steps):
             W.configure(s, w)
             s.nlist_build()
             out.append(s)
+        if shapes:
+            for s, (k, _) in zip(out, shapes):
+                s.set_brick_shape(k)
+                s.nlist_build()
         return out
 
     log(f"bench: ladder batched ({len(ws)} systems, {total} atoms), {steps} steps")
     ms = make()
+    tuned = autotune_batch(ctx, ms, av.PREC_MIXED)
+    shapes.extend(s.brick_shape() for s in ms)
+    log(f"  autotuned brick shapes: step {tuned['step_ms_before']:.3f} -> {tuned['step_ms_after']:.3f} ms; "
+        + ", ".join(f"{c['class_atoms']}->{c['brick']}" for c in tuned["classes"]))
     b = av.Batch(ctx, ms)
     b.set_graph(True)
     b.run(20, av.PREC_MIXED)  # warm (graph capture, list growth)
@@ -160,7 +204,7 @@ def ladder_section(av, W, ctx, steps, loop_steps):
         s.close()
     log(f"  loop: {dt_l * 1e3 / loop_steps:.3f} ms/step, {total * loop_steps / dt_l:.4e} atom-steps/s")
     return {"workload": "lj_ladder_256 (N log-uniform 500..20000, T* 0.7..1.5, NVT, mixed)",
-            "atoms": total, "systems": len(ws),
+            "atoms": total, "systems": len(ws), "autotuned_brick_shapes": tuned,
             "batched_atom_steps_per_s": total * steps / dt_b, "batched_steps": steps,
             "loop_atom_steps_per_s": total * loop_steps / dt_l, "loop_steps": loop_steps,
             "batched_over_loop": (total * steps / dt_b) / (total * loop_steps / dt_l)}
@@ -270,7 +314,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the serving measurement; 0 skips it")
@@ -278,6 +322,10 @@ def main():
                     help="timed steps of the batched 256-system ladder; 0 skips it")
     ap.add_argument("--charged-steps", type=int, default=100,
                     help="timed steps of the charged-melt (PME) side measurement; 0 skips it")
+    ap.add_argument("--nve-steps", type=int, default=10000,
+                    help="NVE side run length at 2.048M atoms, both precisions (0: skip)")
+    ap.add_argument("--nve-charged-steps", type=int, default=2000,
+                    help="NVE side run length for the charged melt, both precisions (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (the reference for byte-counter calibration)")
     args = ap.parse_args()
@@ -358,6 +406,8 @@ def main():
         r["s"].close()
     if args.charged_steps > 0:
         out["charged_melt"] = charged_section(av, W, ctx, args.charged_steps, 10)
+    if args.nve_steps > 0 or args.nve_charged_steps > 0:
+        out["nve_drift"] = nve_section(av, W, ctx, args.nve_steps, args.nve_charged_steps)
     if args.ladder_steps > 0:
         out["batched_ladder"] = ladder_section(av, W, ctx, args.ladder_steps, max(20, args.ladder_steps // 5))
     if args.serve_requests > 0:

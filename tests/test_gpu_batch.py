@@ -156,3 +156,40 @@ def test_replica_exchange_on_device(hip_ctx):
         np.testing.assert_allclose(ke2, ke * T2 / T, rtol=1e-12)
     assert total > 0
     b.close()
+
+
+def test_batched_autotune_chooses_valid_shapes(hip_ctx):
+    """atomiverse_amd.tuning.autotune_batch: one record per size class, every
+    member left on a valid brick shape with a built list; with the chosen
+    shapes, a batched run still equals each member run alone."""
+    from atomiverse_amd.tuning import autotune_batch
+
+    ms = members(hip_ctx, SIZES)
+    systems = [s for _, s in ms]
+    rep = autotune_batch(hip_ctx, systems, av.PREC_MIXED, nclass=3, steps=4)
+    assert 1 <= len(rep["classes"]) <= 3
+    assert rep["step_ms_after"] <= rep["step_ms_before"]
+    shapes = [s.brick_shape()[0] for s in systems]
+    assert all(0 <= k <= 10 for k in shapes)
+    for s in systems:
+        s.close()
+    nsteps = 60
+    alone = []
+    for (w, s), k in zip(members(hip_ctx, SIZES), shapes):
+        s.set_brick_shape(k)
+        s.run(nsteps, av.PREC_MIXED)
+        alone.append(s.download())
+        s.close()
+    ms = members(hip_ctx, SIZES)
+    for (_, s), k in zip(ms, shapes):
+        s.set_brick_shape(k)
+        s.nlist_build()
+    b = av.Batch(hip_ctx, [s for _, s in ms])
+    b.set_graph(True)
+    b.run(nsteps, av.PREC_MIXED)
+    for (w, s), d0 in zip(ms, alone):
+        d1 = s.download()
+        for k in d0:
+            np.testing.assert_array_equal(d1[k], d0[k], err_msg=f"{w.name} {k}")
+        s.close()
+    b.close()
