@@ -173,10 +173,10 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
         const int ng = (nn + kGroup - 1) / kGroup;
         double fxi = 0.0, fyi = 0.0, fzi = 0.0;
         const int gl = max(ng - 1, 0); // group 0 exists even when nn == 0
-        uint4 cur = grp[(int64_t)min(h, gl) * nat];
-        uint4 nx1 = grp[(int64_t)min(h + tpa, gl) * nat];
+        uint4 cur = grp[__umul24(min(h, gl), nat)];
+        uint4 nx1 = grp[__umul24(min(h + tpa, gl), nat)];
         for (int g = h; g < ng; g += tpa) {
-            const uint4 nx2 = grp[(int64_t)min(g + 2 * tpa, gl) * nat];
+            const uint4 nx2 = grp[__umul24(min(g + 2 * tpa, gl), nat)];
             // the whole group's LDS reads issue before any arithmetic; padding
             // rows are the atom itself (r^2 == 0, skipped like r >= r_c)
             double jx[kGroup], jy[kGroup], jz[kGroup], jq[kGroup];
@@ -295,10 +295,10 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_mixed(LjcArgs a) {
         const int ng = (nn + kGroup - 1) / kGroup;
         float fxi = 0.f, fyi = 0.f, fzi = 0.f;
         const int gl = max(ng - 1, 0); // group 0 exists even when nn == 0
-        uint4 cur = grp[(int64_t)min(h, gl) * nat];
-        uint4 nx1 = grp[(int64_t)min(h + tpa, gl) * nat];
+        uint4 cur = grp[__umul24(min(h, gl), nat)];
+        uint4 nx1 = grp[__umul24(min(h + tpa, gl), nat)];
         for (int g = h; g < ng; g += tpa) {
-            const uint4 nx2 = grp[(int64_t)min(g + 2 * tpa, gl) * nat];
+            const uint4 nx2 = grp[__umul24(min(g + 2 * tpa, gl), nat)];
             // the whole group's LDS reads issue before any arithmetic; padding
             // rows are the atom itself (r^2 == 0, skipped like r >= r_c)
             float4 pjs[kGroup];

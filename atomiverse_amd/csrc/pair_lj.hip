@@ -189,6 +189,7 @@ struct BrickLJArgs {
     int nc[3], nb[3], bdim[3];
     double rc_sq;
     double eps24, eps4, sig6, vshift;
+    double inv_sigma; // 1/sigma of the single-type form (coordinates in sigma units)
     const double4* tab;
     int ntypes;
     double *fx, *fy, *fz;
@@ -251,10 +252,10 @@ __device__ void OpLjBrickFp64<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid,
         // group indices clamp to the last group (to group 0 when nn == 0: the
         // brick's rows always exist); loads run two groups ahead
         const int gl = max(ng - 1, 0);
-        uint4 cur = grp[(int64_t)min(h, gl) * nat];
-        uint4 nx1 = grp[(int64_t)min(h + tpa, gl) * nat];
+        uint4 cur = grp[__umul24(min(h, gl), nat)];
+        uint4 nx1 = grp[__umul24(min(h + tpa, gl), nat)];
         for (int g = h; g < ng; g += tpa) {
-            const uint4 nx2 = grp[(int64_t)min(g + 2 * tpa, gl) * nat];
+            const uint4 nx2 = grp[__umul24(min(g + 2 * tpa, gl), nat)];
             // the whole group's LDS reads issue before any arithmetic; padding
             // rows (the atom itself, r^2 == 0) and pairs beyond r_c add zero
             const unsigned w[4] = {cur.x, cur.y, cur.z, cur.w};
@@ -347,23 +348,28 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
     // reference point: the brick centre (every halo row lies within the brick
     // widened by the stencil reach, so |x - centre| < ~7 Å and the FP32
     // rounding of the relative coordinate is <= 2.4e-7 Å)
+    // Single type: coordinates in units of sigma, so the pair term is
+    // r6 (2 r6 - 1) / r^2 with r6 = r^-6, and 24 eps / sigma is applied once
+    // per atom (the virial takes 24 eps); the typed form keeps Å and the
+    // per-pair coefficients.
     double r0x, r0y, r0z;
     brick_centre(b, a.nb, a.bdim, a.nc, a.cs, r0x, r0y, r0z);
+    const double us = TYPED ? 1.0 : a.inv_sigma;
     for (int q = threadIdx.x; q < nh; q += blockDim.x) {
         int gs;
         double xx, yy, zz;
         row_image(a.x, a.y, a.z, a.L[0], a.L[1], a.L[2], rows[q], gs, xx, yy, zz);
         float4 v;
-        v.x = static_cast<float>(xx - r0x);
-        v.y = static_cast<float>(yy - r0y);
-        v.z = static_cast<float>(zz - r0z);
+        v.x = static_cast<float>((xx - r0x) * us);
+        v.y = static_cast<float>((yy - r0y) * us);
+        v.z = static_cast<float>((zz - r0z) * us);
         v.w = TYPED ? __int_as_float(a.type[gs]) : 0.0f;
         sp[q] = v;
     }
     __syncthreads();
-    const float rc_sq = static_cast<float>(a.rc_sq);
-    const float eps24s = static_cast<float>(a.eps24), eps4s = static_cast<float>(a.eps4);
-    const float sig6s = static_cast<float>(a.sig6), shifts = static_cast<float>(a.vshift);
+    const float rc_sq = static_cast<float>(a.rc_sq * us * us);
+    const float eps4s = static_cast<float>(a.eps4), shifts = static_cast<float>(a.vshift);
+    const float fscale = TYPED ? 1.0f : static_cast<float>(a.eps24 * a.inv_sigma);
     const double4* __restrict__ tab = a.tab;
     const int ntypes = a.ntypes;
     const uint4* __restrict__ base = reinterpret_cast<const uint4*>(a.nbr16 + (int64_t)a0 * a.cap);
@@ -383,10 +389,10 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
         // group indices clamp to the last group (to group 0 when nn == 0: the
         // brick's rows always exist); loads run two groups ahead
         const int gl = max(ng - 1, 0);
-        uint4 cur = grp[(int64_t)min(h, gl) * nat];
-        uint4 nx1 = grp[(int64_t)min(h + tpa, gl) * nat];
+        uint4 cur = grp[__umul24(min(h, gl), nat)];
+        uint4 nx1 = grp[__umul24(min(h + tpa, gl), nat)];
         for (int g = h; g < ng; g += tpa) {
-            const uint4 nx2 = grp[(int64_t)min(g + 2 * tpa, gl) * nat];
+            const uint4 nx2 = grp[__umul24(min(g + 2 * tpa, gl), nat)];
             // the whole group's LDS reads issue before any arithmetic; padding
             // rows (the atom itself, r^2 == 0) and pairs beyond r_c add zero
             const unsigned w[4] = {cur.x, cur.y, cur.z, cur.w};
@@ -401,20 +407,25 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
                 const float dz = pi.z - pj[u].z;
                 const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                 const bool in = r2 < rc_sq && r2 > 0.0f;
-                float eps24 = eps24s, eps4 = eps4s, sig6 = sig6s, shift = shifts;
+                float eps4 = eps4s, shift = shifts;
+                const float r2inv = __builtin_amdgcn_rcpf(r2);
+                float r6inv, fpair;
                 if (TYPED) {
                     const double4 t = tab[ti * ntypes + __float_as_int(pj[u].w)];
-                    eps24 = static_cast<float>(t.x);
+                    const float eps24 = static_cast<float>(t.x);
                     eps4 = static_cast<float>(t.y);
-                    sig6 = static_cast<float>(t.z);
                     shift = static_cast<float>(t.w);
+                    r6inv = r2inv * r2inv * r2inv * static_cast<float>(t.z);
+                    fpair = in ? eps24 * r2inv * __builtin_fmaf(2.0f * r6inv, r6inv, -r6inv) : 0.0f;
+                } else {
+                    r6inv = r2inv * r2inv * r2inv;
+                    fpair = r2inv * (r6inv * __builtin_fmaf(2.0f, r6inv, -1.0f));
                 }
-                const float r2inv = __builtin_amdgcn_rcpf(r2);
-                const float r6inv = r2inv * r2inv * r2inv * sig6;
-                const float fpair = in ? eps24 * r2inv * __builtin_fmaf(2.0f * r6inv, r6inv, -r6inv) : 0.0f;
-                fxi = __builtin_fmaf(fpair, dx, fxi);
-                fyi = __builtin_fmaf(fpair, dy, fyi);
-                fzi = __builtin_fmaf(fpair, dz, fzi);
+                if (in) {
+                    fxi = __builtin_fmaf(fpair, dx, fxi);
+                    fyi = __builtin_fmaf(fpair, dy, fyi);
+                    fzi = __builtin_fmaf(fpair, dz, fzi);
+                }
                 if (EF && in)
                     acc[0] += static_cast<double>(__builtin_fmaf(eps4, r6inv * r6inv - r6inv, -shift));
                 if (VF && in) {
@@ -437,14 +448,16 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
         }
         if (h != 0)
             continue;
-        a.fx[i] = static_cast<double>(fxi);
-        a.fy[i] = static_cast<double>(fyi);
-        a.fz[i] = static_cast<double>(fzi);
+        a.fx[i] = static_cast<double>(fxi * fscale);
+        a.fy[i] = static_cast<double>(fyi * fscale);
+        a.fz[i] = static_cast<double>(fzi * fscale);
     }
     if (EF || VF) {
+        acc[0] *= 0.5;
+        const double vscale = TYPED ? 0.5 : 0.5 * a.eps24;
 #pragma unroll
-        for (int k = 0; k < 7; ++k)
-            acc[k] *= 0.5;
+        for (int k = 1; k < 7; ++k)
+            acc[k] *= vscale;
         block_partials(acc, a.partials, bid);
     }
 }
@@ -578,6 +591,7 @@ static BrickLJArgs brick_lj_args(atvh_system* s, const LJArgs& a) {
     b.eps4 = a.eps4;
     b.sig6 = a.sig6;
     b.vshift = a.vshift;
+    b.inv_sigma = a.sig6 > 0.0 ? std::pow(a.sig6, -1.0 / 6.0) : 1.0;
     b.tab = a.tab;
     b.ntypes = a.ntypes;
     b.fx = s->fx;
