@@ -16,10 +16,11 @@ __device__ __forceinline__ T wave_sum(T v) {
 }
 
 // Fixed-order block reduction of 7 doubles into partials[slot*8 + k]
-// (256-thread blocks: 4 waves, summed wave 0..3 by thread k); slot is the
-// block's index within its system's launch.
+// (kThreads-thread blocks: the waves summed in order by thread k); slot is
+// the block's index within its system's launch.
+template <int kThreads = kBlock>
 __device__ __forceinline__ void block_partials(double (&acc)[7], double* partials, int slot) {
-    __shared__ double sh[kBlock / 64][7];
+    __shared__ double sh[kThreads / 64][7];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < 7; ++k)
@@ -31,7 +32,7 @@ __device__ __forceinline__ void block_partials(double (&acc)[7], double* partial
     __syncthreads();
     if (threadIdx.x < 7) {
         double t = 0.0;
-        for (int w = 0; w < kBlock / 64; ++w)
+        for (int w = 0; w < kThreads / 64; ++w)
             t += sh[w][threadIdx.x];
         partials[slot * kPartialStride + threadIdx.x] = t;
     }
