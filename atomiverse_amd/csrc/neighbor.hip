@@ -571,10 +571,6 @@ __device__ __forceinline__ int wave_inclusive_sum(int x) {
     return x;
 }
 
-__device__ __forceinline__ float lane_f(float v, int k) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
-}
-
 struct OpPairs {
     using Args = BrickBuildArgs;
     static constexpr int kThreads = 512;
@@ -676,44 +672,42 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
             }
             const int li0 = slot0 - a0; // brick-local index of the cell's first atom
             const unsigned gstride = static_cast<unsigned>(nat) * kGroup; // list entries between groups
-            for (int ka = hq0; ka < hq1; ka += 64) {
-                // up to 64 atoms of the cell, atom ka + k on lane k
-                const int na = __builtin_amdgcn_readfirstlane(min(64, hq1 - ka));
-                const float4 pa = sp[ka + min(lane, na - 1)];
-                int cnt = 0;
-                // one candidate per lane against every atom of the group
+            // one atom of the cell at a time (coordinates and count are
+            // wave-uniform scalars), its candidates 64 per step, one per lane
+            for (int qs = hq0; qs < hq1; ++qs) {
+                const int li = li0 + (qs - hq0);
+                const float4 pv = sp[qs];
+                const float px = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pv.x)));
+                const float py = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pv.y)));
+                const float pz = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pv.z)));
+                uint16_t* __restrict__ dst = lst + static_cast<unsigned>(li) * kGroup;
+                int count = 0;
                 auto step = [&](int q, bool valid) {
                     const float4 pj = sp[q];
-                    for (int k = 0; k < na; ++k) {
-                        const int qs = ka + k;
-                        const float dx = lane_f(pa.x, k) - pj.x;
-                        const float dy = lane_f(pa.y, k) - pj.y;
-                        const float dz = lane_f(pa.z, k) - pj.z;
-                        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                        const bool other = valid && q != qs;
-                        bool acc = other && r2 < lo_acc;
-                        const int li = li0 + (qs - hq0);
-                        if (other && !acc && r2 <= hi_rej) // inside the band: the reference predicate
+                    const float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
+                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    const bool other = valid && q != qs;
+                    bool acc = other && r2 < lo_acc;
+                    const bool band = other && !acc && r2 <= hi_rej;
+                    if (__builtin_amdgcn_ballot_w64(band)) { // rare: the reference predicate
+                        if (band)
                             acc = band_pair(a.x, a.y, a.z, a.rc_sq, Lx, Ly, Lz, a.g.nc[0], a.g.nc[1], a.g.nc[2],
                                             __float_as_int(pj.w), lx + R0, ly + R1, hz0, cx, cy, cz, a0 + li,
                                             rows[q] & kRowSlotMask);
-                        const unsigned long long m = __ballot(acc);
-                        const int base = __builtin_amdgcn_readlane(cnt, k);
-                        if (acc) {
-                            const unsigned pos = static_cast<unsigned>(base + lanes_below(m));
-                            if (pos < static_cast<unsigned>(a.cap))
-                                lst[static_cast<unsigned>(li) * kGroup + (pos >> 3) * gstride + (pos & 7)] =
-                                    static_cast<uint16_t>(q);
-                        }
-                        if (lane == k)
-                            cnt = base + static_cast<int>(__popcll(m));
                     }
+                    const unsigned long long m = __builtin_amdgcn_ballot_w64(acc);
+                    if (acc) {
+                        const unsigned pos = static_cast<unsigned>(count + lanes_below(m));
+                        if (pos < static_cast<unsigned>(a.cap))
+                            dst[__umul24(pos >> 3, gstride) + (pos & 7)] = static_cast<uint16_t>(q);
+                    }
+                    count += static_cast<int>(__popcll(m));
                 };
                 if (flat) {
                     for (int k0 = 0; k0 < total; k0 += 64) {
                         const int cc = k0 + lane;
                         const bool valid = cc < total;
-                        step(valid ? static_cast<int>(cand[wid][cc]) : ka, valid);
+                        step(valid ? static_cast<int>(cand[wid][cc]) : qs, valid);
                     }
                 } else {
                     for (int col = 0; col < ncols; ++col) {
@@ -721,28 +715,21 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
                         const int clen = __builtin_amdgcn_readlane(len, col);
                         for (int k0 = 0; k0 < clen; k0 += 64) {
                             const bool valid = k0 + lane < clen;
-                            step(valid ? cqb + k0 + lane : ka, valid);
+                            step(valid ? cqb + k0 + lane : qs, valid);
                         }
                     }
                 }
-                // pad each atom's last group with its own row (r^2 == 0, which
-                // the pair kernels mask), so they can read whole groups
-                for (int k = 0; k < na; ++k) {
-                    const int base = __builtin_amdgcn_readlane(cnt, k);
-                    const int pos = base + lane;
-                    if (pos < ((base + kGroup - 1) & ~(kGroup - 1)) && pos < a.cap) {
-                        const unsigned li = static_cast<unsigned>(li0 + (ka + k - hq0));
-                        lst[li * kGroup + static_cast<unsigned>(pos >> 3) * gstride + (pos & 7)] =
-                            static_cast<uint16_t>(ka + k);
-                    }
-                }
-                if (lane < na) {
-                    const int i = slot0 + (ka + lane - hq0);
-                    a.numneigh[i] = cnt < a.cap ? cnt : a.cap;
-                    a.self16[i] = static_cast<uint16_t>(ka + lane);
-                    my_cnt_sum += cnt;
-                    my_max = max(my_max, cnt);
-                    if (cnt > a.cap)
+                // pad the last group with the atom's own row (r^2 == 0, which the
+                // pair kernels mask), so they can read whole groups
+                const int pos = count + lane;
+                if (pos < ((count + kGroup - 1) & ~(kGroup - 1)) && pos < a.cap)
+                    dst[__umul24(static_cast<unsigned>(pos) >> 3, gstride) + (pos & 7)] = static_cast<uint16_t>(qs);
+                if (lane == 0) {
+                    a.numneigh[a0 + li] = count < a.cap ? count : a.cap;
+                    a.self16[a0 + li] = static_cast<uint16_t>(qs);
+                    my_cnt_sum += count;
+                    my_max = max(my_max, count);
+                    if (count > a.cap)
                         ++my_over;
                 }
             }
