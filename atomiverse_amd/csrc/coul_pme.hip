@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
     const int b = xcd_brick(blockIdx.x, gridDim.x); // one launch block per brick
     const int nh = a.halo_n[b];
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (nh > kHaloCap) { // reported by the build; nothing valid to compute
+    if (nh > kHaloRows) { // reported by the build; nothing valid to compute
         if (EF || VF)
             block_partials(acc, a.partials, blockIdx.x);
         return;
@@ -149,6 +149,11 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
         sz[r] = zz;
         sq[r] = a.q[gs];
         stype[r] = a.type[gs];
+    }
+    if (threadIdx.x == 0) { // the list's padding row: out of every cutoff
+        sx[kSentinelRow] = sy[kSentinelRow] = sz[kSentinelRow] = kSentinelFar;
+        sq[kSentinelRow] = 0.0;
+        stype[kSentinelRow] = 0;
     }
     __syncthreads();
     const int2 br = a.brick_range[b];
@@ -254,7 +259,7 @@ __global__ __launch_bounds__(kLjcMixedThreads) void k_ljc_brick_mixed(LjcArgs a)
     const int b = xcd_brick(blockIdx.x, gridDim.x); // one launch block per brick
     const int nh = a.halo_n[b];
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (nh > kHaloCap) {
+    if (nh > kHaloRows) {
         if (EF || VF)
             block_partials<kLjcMixedThreads>(acc, a.partials, blockIdx.x);
         return;
@@ -275,6 +280,10 @@ __global__ __launch_bounds__(kLjcMixedThreads) void k_ljc_brick_mixed(LjcArgs a)
         v.w = static_cast<float>(a.q[gs]);
         sp[r] = v;
         stype[r] = static_cast<unsigned char>(a.type[gs]);
+    }
+    if (threadIdx.x == 0) { // the list's padding row: out of every cutoff
+        sp[kSentinelRow] = make_float4(kSentinelFarF, kSentinelFarF, kSentinelFarF, 0.0f);
+        stype[kSentinelRow] = 0;
     }
     __syncthreads();
     const float rc_sq = static_cast<float>(a.rc_sq), inv_dr2 = static_cast<float>(a.inv_dr2);

@@ -477,13 +477,13 @@ __device__ void OpHalo::run(const BrickBuildArgs& a, int bid, int nbk) {
         if (threadIdx.x == 0) {
             a.halo_n[b] = ht.natoms;
             a.brick_range[b] = make_int2(a0, nat);
-            if (ht.natoms > kHaloCap || nat > kMaxBrickAtoms)
+            if (ht.natoms > kHaloRows || nat > kMaxBrickAtoms)
                 atomicAdd(&a.ctl->halo_overflow, 1);
         }
         int* __restrict__ hoff = a.halo_off + (int64_t)b * (kHaloCellCap + 1);
         for (int h = threadIdx.x; h <= ht.ncell; h += blockDim.x)
             hoff[h] = h_off[h];
-        if (ht.natoms <= kHaloCap) {
+        if (ht.natoms <= kHaloRows) {
             int* __restrict__ rows = a.halo_rows + (int64_t)b * kHaloCap;
             for (int h = threadIdx.x; h < ht.ncell; h += blockDim.x) {
                 const int q0 = h_off[h], nq = h_off[h + 1] - q0, g0 = h_start[h];
@@ -510,7 +510,7 @@ __device__ void OpHalo::run(const BrickBuildArgs& a, int bid, int nbk) {
 // therefore exactly the reference's. Accepted rows are compacted with a
 // ballot in candidate order (column order, then halo row order:
 // deterministic) into 16-bit rows in 8-wide groups; the unused tail of the
-// last group holds the atom's own row. A cell with more than kCandCap
+// last group holds the sentinel row. A cell with more than kCandCap
 // candidates walks its columns one at a time instead.
 constexpr float kBand = 1e-4f;
 constexpr int kCandCap = 768; // flat candidates per wavefront (LDS)
@@ -604,7 +604,7 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
         const int nh = a.halo_n[b];
         const int2 br = a.brick_range[b];
         const int a0 = br.x, nat = br.y;
-        if (nh > kHaloCap || nat > kMaxBrickAtoms)
+        if (nh > kHaloRows || nat > kMaxBrickAtoms)
             continue; // reported by OpHalo (block-uniform)
         const int cbase = a.g.brick_cell_base[b];
         int bx, by, bz;
@@ -719,11 +719,12 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
                         }
                     }
                 }
-                // pad the last group with the atom's own row (r^2 == 0, which the
-                // pair kernels mask), so they can read whole groups
+                // pad the last group with the sentinel row (staged out of reach by
+                // the pair kernels), so they can read whole groups
                 const int pos = count + lane;
                 if (pos < ((count + kGroup - 1) & ~(kGroup - 1)) && pos < a.cap)
-                    dst[__umul24(static_cast<unsigned>(pos) >> 3, gstride) + (pos & 7)] = static_cast<uint16_t>(qs);
+                    dst[__umul24(static_cast<unsigned>(pos) >> 3, gstride) + (pos & 7)] =
+                        static_cast<uint16_t>(kSentinelRow);
                 if (lane == 0) {
                     a.numneigh[a0 + li] = count < a.cap ? count : a.cap;
                     a.self16[a0 + li] = static_cast<uint16_t>(qs);

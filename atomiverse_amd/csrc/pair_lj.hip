@@ -210,7 +210,7 @@ __device__ void OpLjBrickFp64<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid,
     const int b = xcd_brick(bid, nbk); // one launch block per brick
     const int nh = a.halo_n[b];
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (nh > kHaloCap) { // reported by the build; nothing valid to compute
+    if (nh > kHaloRows) { // reported by the build; nothing valid to compute
         if (EF || VF)
             block_partials<kThreads>(acc, a.partials, bid);
         return;
@@ -225,6 +225,11 @@ __device__ void OpLjBrickFp64<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid,
         sz[q] = zz;
         if (TYPED)
             stype[q] = a.type[gs];
+    }
+    if (threadIdx.x == 0) { // the list's padding row: out of every cutoff
+        sx[kSentinelRow] = sy[kSentinelRow] = sz[kSentinelRow] = kSentinelFar;
+        if (TYPED)
+            stype[kSentinelRow] = 0;
     }
     __syncthreads();
     const int2 br = a.brick_range[b];
@@ -339,7 +344,7 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
     const int b = xcd_brick(bid, nbk); // one launch block per brick
     const int nh = a.halo_n[b];
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (nh > kHaloCap) {
+    if (nh > kHaloRows) {
         if (EF || VF)
             block_partials<kThreads>(acc, a.partials, bid);
         return;
@@ -368,6 +373,8 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
         v.w = TYPED ? __int_as_float(a.type[gs]) : 0.0f;
         sp[q] = v;
     }
+    if (threadIdx.x == 0) // the list's padding row: out of every cutoff
+        sp[kSentinelRow] = make_float4(kSentinelFarF, kSentinelFarF, kSentinelFarF, __int_as_float(0));
     __syncthreads();
     const float rc_sq = static_cast<float>(a.rc_sq * us * us);
     const float eps4s = static_cast<float>(a.eps4), shifts = static_cast<float>(a.vshift);
@@ -408,7 +415,7 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
                 const float dy = pi.y - pj[u].y;
                 const float dz = pi.z - pj[u].z;
                 const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                const bool in = r2 < rc_sq && r2 > 0.0f;
+                const bool in = r2 < rc_sq; // padding rows are the far sentinel
                 float eps4 = eps4s, shift = shifts;
                 const float r2inv = __builtin_amdgcn_rcpf(r2);
                 float r6inv, fpair;
