@@ -672,18 +672,27 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
             }
             const int li0 = slot0 - a0; // brick-local index of the cell's first atom
             const unsigned gstride = static_cast<unsigned>(nat) * kGroup; // list entries between groups
-            // one atom of the cell at a time (coordinates and count are
-            // wave-uniform scalars), its candidates 64 per step, one per lane
-            for (int qs = hq0; qs < hq1; ++qs) {
-                const int li = li0 + (qs - hq0);
-                const float4 pv = sp[qs];
-                const float px = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pv.x)));
-                const float py = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pv.y)));
-                const float pz = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pv.z)));
-                uint16_t* __restrict__ dst = lst + static_cast<unsigned>(li) * kGroup;
-                int count = 0;
-                auto step = [&](int q, bool valid) {
-                    const float4 pj = sp[q];
+            // the cell's atoms two at a time (coordinates and counts are
+            // wave-uniform scalars; two independent test chains per candidate
+            // load), their candidates 64 per step, one per lane
+            for (int qa = hq0; qa < hq1; qa += 2) {
+                const bool two = qa + 1 < hq1; // wave-uniform
+                const int qb2 = two ? qa + 1 : qa;
+                const float4 pva = sp[qa], pvb = sp[qb2];
+                const float ax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pva.x)));
+                const float ay = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pva.y)));
+                const float az = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pva.z)));
+                const float bx = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pvb.x)));
+                const float by = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pvb.y)));
+                const float bz = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pvb.z)));
+                const int lia = li0 + (qa - hq0), lib = li0 + (qb2 - hq0);
+                uint16_t* __restrict__ dsta = lst + static_cast<unsigned>(lia) * kGroup;
+                uint16_t* __restrict__ dstb = lst + static_cast<unsigned>(lib) * kGroup;
+                int ca = 0, cb = 0;
+                // candidate q (one per lane) against one atom: test, then append
+                // the accepted rows in lane order
+                auto test = [&](const float4& pj, int q, bool valid, float px, float py, float pz, int qs, int li,
+                                uint16_t* __restrict__ dst, int& count) {
                     const float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
                     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                     const bool other = valid && q != qs;
@@ -703,11 +712,17 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
                     }
                     count += static_cast<int>(__popcll(m));
                 };
+                auto step = [&](int q, bool valid) {
+                    const float4 pj = sp[q];
+                    test(pj, q, valid, ax, ay, az, qa, lia, dsta, ca);
+                    if (two)
+                        test(pj, q, valid, bx, by, bz, qb2, lib, dstb, cb);
+                };
                 if (flat) {
                     for (int k0 = 0; k0 < total; k0 += 64) {
                         const int cc = k0 + lane;
                         const bool valid = cc < total;
-                        step(valid ? static_cast<int>(cand[wid][cc]) : qs, valid);
+                        step(valid ? static_cast<int>(cand[wid][cc]) : qa, valid);
                     }
                 } else {
                     for (int col = 0; col < ncols; ++col) {
@@ -715,24 +730,29 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
                         const int clen = __builtin_amdgcn_readlane(len, col);
                         for (int k0 = 0; k0 < clen; k0 += 64) {
                             const bool valid = k0 + lane < clen;
-                            step(valid ? cqb + k0 + lane : qs, valid);
+                            step(valid ? cqb + k0 + lane : qa, valid);
                         }
                     }
                 }
                 // pad the last group with the sentinel row (staged out of reach by
-                // the pair kernels), so they can read whole groups
-                const int pos = count + lane;
-                if (pos < ((count + kGroup - 1) & ~(kGroup - 1)) && pos < a.cap)
-                    dst[__umul24(static_cast<unsigned>(pos) >> 3, gstride) + (pos & 7)] =
-                        static_cast<uint16_t>(kSentinelRow);
-                if (lane == 0) {
-                    a.numneigh[a0 + li] = count < a.cap ? count : a.cap;
-                    a.self16[a0 + li] = static_cast<uint16_t>(qs);
-                    my_cnt_sum += count;
-                    my_max = max(my_max, count);
-                    if (count > a.cap)
-                        ++my_over;
-                }
+                // the pair kernels) so they can read whole groups; counts, stats
+                auto finish = [&](int qs, int li, uint16_t* __restrict__ dst, int count) {
+                    const int pos = count + lane;
+                    if (pos < ((count + kGroup - 1) & ~(kGroup - 1)) && pos < a.cap)
+                        dst[__umul24(static_cast<unsigned>(pos) >> 3, gstride) + (pos & 7)] =
+                            static_cast<uint16_t>(kSentinelRow);
+                    if (lane == 0) {
+                        a.numneigh[a0 + li] = count < a.cap ? count : a.cap;
+                        a.self16[a0 + li] = static_cast<uint16_t>(qs);
+                        my_cnt_sum += count;
+                        my_max = max(my_max, count);
+                        if (count > a.cap)
+                            ++my_over;
+                    }
+                };
+                finish(qa, lia, dsta, ca);
+                if (two)
+                    finish(qb2, lib, dstb, cb);
             }
         }
         __syncthreads(); // sp and h_off are refilled for the next brick
