@@ -59,36 +59,70 @@ struct SortArgs {
     const int* tag;
 };
 
-// CellList::bin_atoms: cell of every slot (current order) + histogram.
+// Runs of equal keys in consecutive lanes (slots are in cell order from the
+// previous build, and most atoms keep their cell, so runs average the atoms
+// per cell). For every lane: the first lane of its run and the run length
+// (meaningful on the first lane). Every lane of the wave must call it;
+// inactive lanes pass a key no other lane holds.
+struct LaneRun {
+    int head;   // first lane of this lane's run
+    int length; // lanes in the run
+};
+
+__device__ __forceinline__ LaneRun lane_run(int key) {
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1, 64);
+    const bool first = lane == 0 || prev != key;
+    const unsigned long long heads = __ballot(first);
+    const unsigned long long below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1)); // heads <= lane
+    const unsigned long long above = heads & ~((lane == 63) ? ~0ull : ((2ull << lane) - 1)); // heads > lane
+    LaneRun r;
+    r.head = 63 - __clzll(below);
+    r.length = (above ? __ffsll(static_cast<long long>(above)) - 1 : 64) - lane;
+    return r;
+}
+
+// CellList::bin_atoms: cell of every slot (current order) + histogram (one
+// atomic per run of equal cells in a wave).
 struct OpBin {
     using Args = SortArgs;
     static __device__ void run(const SortArgs& a, int bid, int nbk) {
         if (!*a.pred)
             return;
-        for (int64_t i = bid * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)nbk * blockDim.x) {
-            // wrap_orthorhombic (pbc.h:37-51) is applied here, at the rebuild,
-            // rather than every step: between rebuilds positions stay unwrapped so
-            // the periodic image of every listed pair is the one fixed at the build.
-            double xi = a.x[i], yi = a.y[i], zi = a.z[i];
-            xi -= a.lx * floor(xi / a.lx);
-            yi -= a.ly * floor(yi / a.ly);
-            zi -= a.lz * floor(zi / a.lz);
-            a.x[i] = xi;
-            a.y[i] = yi;
-            a.z[i] = zi;
-            // bin_atoms wraps its own copy once more (a no-op except at x == L)
-            xi -= a.lx * floor(xi / a.lx);
-            yi -= a.ly * floor(yi / a.ly);
-            zi -= a.lz * floor(zi / a.lz);
-            int cx = static_cast<int>(floor((xi - 0.0) / a.csx));
-            int cy = static_cast<int>(floor((yi - 0.0) / a.csy));
-            int cz = static_cast<int>(floor((zi - 0.0) / a.csz));
-            cx = max(0, min(cx, a.ncx - 1));
-            cy = max(0, min(cy, a.ncy - 1));
-            cz = max(0, min(cz, a.ncz - 1));
-            const int c = cx * a.ncy * a.ncz + cy * a.ncz + cz;
-            a.cell_of[i] = c;
-            atomicAdd(&a.count[a.cell_rank[c]], 1);
+        const int lane = threadIdx.x & 63;
+        for (int64_t w = bid * (int64_t)blockDim.x + (threadIdx.x & ~63); w < a.n;
+             w += (int64_t)nbk * blockDim.x) { // wave-uniform
+            const int64_t i = w + lane;
+            const bool act = i < a.n;
+            int key = -1 - lane;
+            if (act) {
+                // wrap_orthorhombic (pbc.h:37-51) is applied here, at the rebuild,
+                // rather than every step: between rebuilds positions stay unwrapped so
+                // the periodic image of every listed pair is the one fixed at the build.
+                double xi = a.x[i], yi = a.y[i], zi = a.z[i];
+                xi -= a.lx * floor(xi / a.lx);
+                yi -= a.ly * floor(yi / a.ly);
+                zi -= a.lz * floor(zi / a.lz);
+                a.x[i] = xi;
+                a.y[i] = yi;
+                a.z[i] = zi;
+                // bin_atoms wraps its own copy once more (a no-op except at x == L)
+                xi -= a.lx * floor(xi / a.lx);
+                yi -= a.ly * floor(yi / a.ly);
+                zi -= a.lz * floor(zi / a.lz);
+                int cx = static_cast<int>(floor((xi - 0.0) / a.csx));
+                int cy = static_cast<int>(floor((yi - 0.0) / a.csy));
+                int cz = static_cast<int>(floor((zi - 0.0) / a.csz));
+                cx = max(0, min(cx, a.ncx - 1));
+                cy = max(0, min(cy, a.ncy - 1));
+                cz = max(0, min(cz, a.ncz - 1));
+                const int c = cx * a.ncy * a.ncz + cy * a.ncz + cz;
+                a.cell_of[i] = c;
+                key = a.cell_rank[c];
+            }
+            const LaneRun r = lane_run(key);
+            if (act && r.head == lane)
+                atomicAdd(&a.count[key], r.length);
         }
     }
 };
@@ -179,11 +213,23 @@ struct OpScatter {
     static __device__ void run(const SortArgs& a, int bid, int nbk) {
         if (!*a.pred)
             return;
-        for (int64_t i = bid * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)nbk * blockDim.x) {
-            const int c = a.cell_rank[a.cell_of[i]];
-            const int slot = a.start[c] + atomicAdd(&a.fill[c], 1);
-            a.perm[slot] = static_cast<int>(i);
-            a.count[c] = 0;
+        const int lane = threadIdx.x & 63;
+        for (int64_t w = bid * (int64_t)blockDim.x + (threadIdx.x & ~63); w < a.n;
+             w += (int64_t)nbk * blockDim.x) { // wave-uniform
+            const int64_t i = w + lane;
+            const bool act = i < a.n;
+            const int c = act ? a.cell_rank[a.cell_of[i]] : -1 - lane;
+            // one atomic per run of equal cells; the run's lanes take
+            // consecutive places from its base
+            const LaneRun r = lane_run(c);
+            int base = 0;
+            if (act && r.head == lane)
+                base = atomicAdd(&a.fill[c], r.length);
+            base = __shfl(base, r.head, 64);
+            if (act) {
+                a.perm[a.start[c] + base + (lane - r.head)] = static_cast<int>(i);
+                a.count[c] = 0;
+            }
         }
     }
 };
@@ -221,6 +267,10 @@ struct PermArgs {
     double *ax, *ay, *az, *avx, *avy, *avz, *afx, *afy, *afz, *aq;
     int *atype, *atag, *acell_of;
     int with_forces;
+    // arrays whose permutation is a no-op or unread are skipped: charges of an
+    // uncharged frame (all zero), types of a one-type frame (all zero), and
+    // the cell index in brick mode (only the small-grid build reads it)
+    int with_q, with_type, with_cell_of;
 };
 
 struct OpPermute {
@@ -236,10 +286,13 @@ struct OpPermute {
         a.avx[s] = a.vx[p];
         a.avy[s] = a.vy[p];
         a.avz[s] = a.vz[p];
-        a.aq[s] = a.q[p];
-        a.atype[s] = a.type[p];
+        if (a.with_q)
+            a.aq[s] = a.q[p];
+        if (a.with_type)
+            a.atype[s] = a.type[p];
         a.atag[s] = a.tag[p];
-        a.acell_of[s] = a.cell_of[p];
+        if (a.with_cell_of)
+            a.acell_of[s] = a.cell_of[p];
         if (a.with_forces) {
             a.afx[s] = a.fx[p];
             a.afy[s] = a.fy[p];
@@ -257,6 +310,7 @@ struct CopyArgs {
     double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *q, *xs, *ys, *zs;
     int *type, *tag, *cell_of;
     int with_forces;
+    int with_q, with_type, with_cell_of; // as in PermArgs
 };
 
 // Copy the permuted arrays back and snapshot x_save (CellList::save_positions).
@@ -276,10 +330,13 @@ struct OpCopyback {
         a.vx[s] = a.avx[s];
         a.vy[s] = a.avy[s];
         a.vz[s] = a.avz[s];
-        a.q[s] = a.aq[s];
-        a.type[s] = a.atype[s];
+        if (a.with_q)
+            a.q[s] = a.aq[s];
+        if (a.with_type)
+            a.type[s] = a.atype[s];
         a.tag[s] = a.atag[s];
-        a.cell_of[s] = a.acell_of[s];
+        if (a.with_cell_of)
+            a.cell_of[s] = a.acell_of[s];
         if (a.with_forces) {
             a.fx[s] = a.afx[s];
             a.fy[s] = a.afy[s];
@@ -1222,6 +1279,9 @@ PermArgs nl_perm_args(atvh_system* s, const int* pred, bool permute_forces) {
     a.atag = s->atag;
     a.acell_of = s->acell_of;
     a.with_forces = permute_forces ? 1 : 0;
+    a.with_q = s->has_charge ? 1 : 0;
+    a.with_type = s->ntypes > 1 ? 1 : 0;
+    a.with_cell_of = s->brick_mode ? 0 : 1;
     return a;
 }
 
@@ -1259,6 +1319,9 @@ CopyArgs nl_copy_args(atvh_system* s, const int* pred, bool permute_forces) {
     a.tag = s->tag;
     a.cell_of = s->cell_of;
     a.with_forces = permute_forces ? 1 : 0;
+    a.with_q = s->has_charge ? 1 : 0;
+    a.with_type = s->ntypes > 1 ? 1 : 0;
+    a.with_cell_of = s->brick_mode ? 0 : 1;
     return a;
 }
 
