@@ -115,6 +115,7 @@ struct LjcArgs {
     const float4* erfc32;
     int nbin;
     double inv_dr2;
+    double alpha; // Ewald splitting parameter (the mixed kernels' closed-form erfc)
     double *fx, *fy, *fz;
     double* partials;
     // small-grid form (global slots, per-pair minimum image)
@@ -183,7 +184,7 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
         for (int g = h; g < ng; g += tpa) {
             const uint4 nx2 = grp[__umul24(min(g + 2 * tpa, gl), nat)];
             // the whole group's LDS reads issue before any arithmetic; padding
-            // rows are the atom itself (r^2 == 0, skipped like r >= r_c)
+            // rows are the far sentinel (skipped like r >= r_c)
             double jx[kGroup], jy[kGroup], jz[kGroup], jq[kGroup];
             int jt[kGroup];
 #pragma unroll
@@ -250,6 +251,23 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
     }
 }
 
+// erfc(x) / r and exp(-x^2) in FP32 for the mixed path: Abramowitz & Stegun,
+// Handbook of Mathematical Functions (1964), eq. 7.1.26,
+// erfc(x) = t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-x^2), t = 1/(1 + p x),
+// |error| <= 1.5e-7 for x >= 0 (below the FP32 table's own interpolation
+// error at the mixed path's force bar). exp via v_exp_f32 (2^y).
+__device__ __forceinline__ float erfc_over_r(float x, float rinv, float& ex) {
+    constexpr float p = 0.3275911f;
+    constexpr float a1 = 0.254829592f, a2 = -0.284496736f, a3 = 1.421413741f, a4 = -1.453152027f,
+                    a5 = 1.061405429f;
+    constexpr float log2e = 1.4426950408889634f;
+    const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(p, x, 1.0f));
+    const float poly =
+        t * __builtin_fmaf(t, __builtin_fmaf(t, __builtin_fmaf(t, __builtin_fmaf(t, a5, a4), a3), a2), a1);
+    ex = __builtin_amdgcn_exp2f(-(x * x) * log2e);
+    return poly * ex * rinv;
+}
+
 constexpr int kLjcMixedThreads = kBlock; // (512 measured slower here: 229 vs 221 us on the 512k-ion melt)
 
 template <bool EF, bool VF>
@@ -286,10 +304,11 @@ __global__ __launch_bounds__(kLjcMixedThreads) void k_ljc_brick_mixed(LjcArgs a)
         stype[kSentinelRow] = 0;
     }
     __syncthreads();
-    const float rc_sq = static_cast<float>(a.rc_sq), inv_dr2 = static_cast<float>(a.inv_dr2);
-    const int ntypes = a.ntypes, nbin1 = a.nbin - 1;
+    const float rc_sq = static_cast<float>(a.rc_sq);
+    const float alpha = static_cast<float>(a.alpha);
+    const float two_a_rtpi = static_cast<float>(kEwaldF * a.alpha);
+    const int ntypes = a.ntypes;
     const float* __restrict__ tab = a.tab32;
-    const float4* __restrict__ etab = a.erfc32;
     const uint4* __restrict__ base = reinterpret_cast<const uint4*>(a.nbr16 + (int64_t)a0 * a.cap);
     // two threads per atom when the brick has at most half a block of atoms
     const int tpa = (2 * nat <= static_cast<int>(blockDim.x)) ? 2 : 1;
@@ -311,7 +330,7 @@ __global__ __launch_bounds__(kLjcMixedThreads) void k_ljc_brick_mixed(LjcArgs a)
         for (int g = h; g < ng; g += tpa) {
             const uint4 nx2 = grp[__umul24(min(g + 2 * tpa, gl), nat)];
             // the whole group's LDS reads issue before any arithmetic; padding
-            // rows are the atom itself (r^2 == 0, skipped like r >= r_c)
+            // rows are the far sentinel (skipped like r >= r_c)
             float4 pjs[kGroup];
             int jt[kGroup];
 #pragma unroll
@@ -329,22 +348,23 @@ __global__ __launch_bounds__(kLjcMixedThreads) void k_ljc_brick_mixed(LjcArgs a)
                 const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                 if (r2 < rc_sq && r2 != 0.0f) {
                     const float4 t = *reinterpret_cast<const float4*>(trow + jt[u] * 8);
-                    const float r2inv = __builtin_amdgcn_rcpf(r2);
+                    // real-space Coulomb in closed form (no table load):
+                    // e = erfc(a r) / r, f = (e + 2/sqrt(pi) a exp(-a^2 r^2)) / r^2
+                    const float rinv = __builtin_amdgcn_rsqf(r2);
+                    const float r2inv = rinv * rinv;
                     const float r6inv = r2inv * r2inv * r2inv;
-                    const float xb = r2 * inv_dr2;
-                    const int bi = min(static_cast<int>(xb), nbin1);
-                    const float tt = xb - static_cast<float>(bi);
-                    const float4 ev = etab[bi];
+                    float ex;
+                    const float ecoul = erfc_over_r(r2 * rinv * alpha, rinv, ex);
+                    const float fcoul = __builtin_fmaf(two_a_rtpi, ex, ecoul) * r2inv;
                     const float qq = qi_scale * pj.w;
-                    const float fpair = r2inv * __builtin_fmaf(t.x * r6inv, r6inv, -t.y * r6inv) +
-                                        qq * __builtin_fmaf(tt, ev.y, ev.x);
+                    const float fpair = r2inv * __builtin_fmaf(t.x * r6inv, r6inv, -t.y * r6inv) + qq * fcoul;
                     fxi = __builtin_fmaf(fpair, dx, fxi);
                     fyi = __builtin_fmaf(fpair, dy, fyi);
                     fzi = __builtin_fmaf(fpair, dz, fzi);
                     if (EF) {
                         const float sh = trow[jt[u] * 8 + 4];
                         acc[0] += static_cast<double>(__builtin_fmaf(t.z * r6inv, r6inv, -t.w * r6inv) - sh) +
-                                  static_cast<double>(qq * __builtin_fmaf(tt, ev.w, ev.z));
+                                  static_cast<double>(qq * ecoul);
                     }
                     if (VF) {
                         const float fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
@@ -1098,6 +1118,7 @@ void ljc_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
     a.erfc32 = c->erfc32;
     a.nbin = c->nbin;
     a.inv_dr2 = c->inv_dr2;
+    a.alpha = c->alpha;
     a.fx = s->fx;
     a.fy = s->fy;
     a.fz = s->fz;
