@@ -270,10 +270,13 @@ __device__ __forceinline__ float erfc_over_r(float x, float rinv, float& ex) {
 
 constexpr int kLjcMixedThreads = kBlock; // (512 measured slower here: 229 vs 221 us on the 512k-ion melt)
 
-template <bool EF, bool VF>
+constexpr int kLjcLdsTypes = 4; // type tables up to this size are staged in LDS
+
+template <bool EF, bool VF, bool SMALLTAB>
 __global__ __launch_bounds__(kLjcMixedThreads) void k_ljc_brick_mixed(LjcArgs a) {
     __shared__ float4 sp[kHaloCap]; // x, y, z relative to the brick centre; w = charge
     __shared__ unsigned char stype[kHaloCap];
+    __shared__ float ljt[SMALLTAB ? kLjcLdsTypes * kLjcLdsTypes * 8 : 1]; // [ti][tj] lj1..lj4, shift
     const int b = xcd_brick(blockIdx.x, gridDim.x); // one launch block per brick
     const int nh = a.halo_n[b];
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -303,6 +306,9 @@ __global__ __launch_bounds__(kLjcMixedThreads) void k_ljc_brick_mixed(LjcArgs a)
         sp[kSentinelRow] = make_float4(kSentinelFarF, kSentinelFarF, kSentinelFarF, 0.0f);
         stype[kSentinelRow] = 0;
     }
+    if (SMALLTAB)
+        for (int k = threadIdx.x; k < a.ntypes * a.ntypes * 8; k += blockDim.x)
+            ljt[(k / (a.ntypes * 8)) * kLjcLdsTypes * 8 + k % (a.ntypes * 8)] = a.tab32[k];
     __syncthreads();
     const float rc_sq = static_cast<float>(a.rc_sq);
     const float alpha = static_cast<float>(a.alpha);
@@ -319,7 +325,7 @@ __global__ __launch_bounds__(kLjcMixedThreads) void k_ljc_brick_mixed(LjcArgs a)
         const int qs = a.self16[i];
         const float4 pi = sp[qs];
         const float qi_scale = static_cast<float>(coulomb) * pi.w;
-        const float* __restrict__ trow = tab + stype[qs] * ntypes * 8;
+        const float* __restrict__ trow = SMALLTAB ? ljt + stype[qs] * kLjcLdsTypes * 8 : tab + stype[qs] * ntypes * 8;
         const int nn = a.numneigh[i];
         const uint4* __restrict__ grp = base + li;
         const int ng = (nn + kGroup - 1) / kGroup;
@@ -396,6 +402,21 @@ __global__ __launch_bounds__(kLjcMixedThreads) void k_ljc_brick_mixed(LjcArgs a)
         for (int k = 0; k < 7; ++k)
             acc[k] *= 0.5;
         block_partials<kLjcMixedThreads>(acc, a.partials, blockIdx.x);
+    }
+}
+
+template <bool SMALLTAB>
+void launch_ljc_mixed(bool ef, bool vf, int grid, hipStream_t st, const LjcArgs& a) {
+    if (ef) {
+        if (vf)
+            k_ljc_brick_mixed<true, true, SMALLTAB><<<grid, kLjcMixedThreads, 0, st>>>(a);
+        else
+            k_ljc_brick_mixed<true, false, SMALLTAB><<<grid, kLjcMixedThreads, 0, st>>>(a);
+    } else {
+        if (vf)
+            k_ljc_brick_mixed<false, true, SMALLTAB><<<grid, kLjcMixedThreads, 0, st>>>(a);
+        else
+            k_ljc_brick_mixed<false, false, SMALLTAB><<<grid, kLjcMixedThreads, 0, st>>>(a);
     }
 }
 
@@ -1130,7 +1151,10 @@ void ljc_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
     const int g = pair_grid(s);
     if (s->brick_mode) {
         if (precision == ATVH_PREC_MIXED)
-            LJC_DISPATCH2(k_ljc_brick_mixed, eflag, vflag, g, kLjcMixedThreads, st, a);
+            if (c->ntypes <= kLjcLdsTypes)
+                launch_ljc_mixed<true>(eflag, vflag, g, st, a);
+            else
+                launch_ljc_mixed<false>(eflag, vflag, g, st, a);
         else
             LJC_DISPATCH2(k_ljc_brick_fp64, eflag, vflag, g, kBlock, st, a);
     } else {
