@@ -128,10 +128,13 @@ struct LjcArgs {
 // Force arithmetic is written with explicit fma and no automatic
 // contraction, so it rounds identically in every EF/VF instantiation: a
 // trajectory does not depend on which steps asked for energies.
-template <bool EF, bool VF>
+constexpr int kLjcLdsTypes = 4; // type tables up to this size are staged in LDS
+
+template <bool EF, bool VF, bool SMALLTAB>
 __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
     __shared__ double sx[kHaloCap], sy[kHaloCap], sz[kHaloCap], sq[kHaloCap];
     __shared__ int stype[kHaloCap];
+    __shared__ double ljt[SMALLTAB ? kLjcLdsTypes * kLjcLdsTypes * 8 : 1]; // [ti][tj] lj1..lj4, shift
     const int b = xcd_brick(blockIdx.x, gridDim.x); // one launch block per brick
     const int nh = a.halo_n[b];
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -156,6 +159,9 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
         sq[kSentinelRow] = 0.0;
         stype[kSentinelRow] = 0;
     }
+    if (SMALLTAB)
+        for (int k = threadIdx.x; k < a.ntypes * a.ntypes * 8; k += blockDim.x)
+            ljt[(k / (a.ntypes * 8)) * kLjcLdsTypes * 8 + k % (a.ntypes * 8)] = a.tab[k];
     __syncthreads();
     const int2 br = a.brick_range[b];
     const int a0 = br.x, nat = br.y;
@@ -173,7 +179,8 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
         const int qs = a.self16[i];
         const double xi = sx[qs], yi = sy[qs], zi = sz[qs];
         const double qi_scale = coulomb * sq[qs];
-        const double* __restrict__ trow = tab + stype[qs] * ntypes * 8;
+        const double* __restrict__ trow =
+            SMALLTAB ? ljt + stype[qs] * kLjcLdsTypes * 8 : tab + stype[qs] * ntypes * 8;
         const int nn = a.numneigh[i];
         const uint4* __restrict__ grp = base + li;
         const int ng = (nn + kGroup - 1) / kGroup;
@@ -269,8 +276,6 @@ __device__ __forceinline__ float erfc_over_r(float x, float rinv, float& ex) {
 }
 
 constexpr int kLjcMixedThreads = kBlock; // (512 measured slower here: 229 vs 221 us on the 512k-ion melt)
-
-constexpr int kLjcLdsTypes = 4; // type tables up to this size are staged in LDS
 
 template <bool EF, bool VF, bool SMALLTAB>
 __global__ __launch_bounds__(kLjcMixedThreads) void k_ljc_brick_mixed(LjcArgs a) {
@@ -402,6 +407,21 @@ __global__ __launch_bounds__(kLjcMixedThreads) void k_ljc_brick_mixed(LjcArgs a)
         for (int k = 0; k < 7; ++k)
             acc[k] *= 0.5;
         block_partials<kLjcMixedThreads>(acc, a.partials, blockIdx.x);
+    }
+}
+
+template <bool SMALLTAB>
+void launch_ljc_fp64(bool ef, bool vf, int grid, hipStream_t st, const LjcArgs& a) {
+    if (ef) {
+        if (vf)
+            k_ljc_brick_fp64<true, true, SMALLTAB><<<grid, kBlock, 0, st>>>(a);
+        else
+            k_ljc_brick_fp64<true, false, SMALLTAB><<<grid, kBlock, 0, st>>>(a);
+    } else {
+        if (vf)
+            k_ljc_brick_fp64<false, true, SMALLTAB><<<grid, kBlock, 0, st>>>(a);
+        else
+            k_ljc_brick_fp64<false, false, SMALLTAB><<<grid, kBlock, 0, st>>>(a);
     }
 }
 
@@ -910,17 +930,6 @@ __global__ __launch_bounds__(kBlock) void k_zero3(int64_t n, double* a, double* 
     }
 }
 
-#define LJC_DISPATCH2(KERNEL, EF, VF, GRID, BLOCK, ST, ARGS)                                    \
-    do {                                                                                       \
-        if (EF) {                                                                              \
-            if (VF) KERNEL<true, true><<<GRID, BLOCK, 0, ST>>>(ARGS);                          \
-            else KERNEL<true, false><<<GRID, BLOCK, 0, ST>>>(ARGS);                            \
-        } else {                                                                               \
-            if (VF) KERNEL<false, true><<<GRID, BLOCK, 0, ST>>>(ARGS);                         \
-            else KERNEL<false, false><<<GRID, BLOCK, 0, ST>>>(ARGS);                           \
-        }                                                                                      \
-    } while (0)
-
 #define LJC_DISPATCH3(KERNEL, EF, VF, M, GRID, ST, ARGS)                                        \
     do {                                                                                       \
         if (M) {                                                                               \
@@ -1156,7 +1165,10 @@ void ljc_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
             else
                 launch_ljc_mixed<false>(eflag, vflag, g, st, a);
         else
-            LJC_DISPATCH2(k_ljc_brick_fp64, eflag, vflag, g, kBlock, st, a);
+            if (c->ntypes <= kLjcLdsTypes)
+                launch_ljc_fp64<true>(eflag, vflag, g, st, a);
+            else
+                launch_ljc_fp64<false>(eflag, vflag, g, st, a);
     } else {
         LJC_DISPATCH3(k_ljc_global, eflag, vflag, precision == ATVH_PREC_MIXED, g, st, a);
     }
