@@ -10,10 +10,12 @@
 //   e, f linearly interpolated in r^2 over the ErfcTable (erfc_table.h:44-90,
 //   10000 bins, built on the host with libm erfc/exp exactly as the reference).
 //   Kernels: one workgroup per brick (same list and halo as the LJ kernels),
-//   full list, energy/virial halved. The table is stored per bin as
-//   {f_i, f_{i+1} - f_i, e_i, e_{i+1} - e_i}: one 32-byte (FP64) or 16-byte
-//   (FP32) load per pair, with the differences formed on the host exactly as
-//   the reference forms them per pair.
+//   full list, energy/virial halved, per-type coefficients in LDS for up to 4
+//   types. The FP64 kernel reads the table, stored per bin as
+//   {f_i, f_{i+1} - f_i, e_i, e_{i+1} - e_i} (one 32-byte load per pair, the
+//   differences formed on the host exactly as the reference forms them per
+//   pair); the mixed kernel evaluates erfc in closed form (Abramowitz &
+//   Stegun 7.1.26, |error| <= 1.5e-7) instead of loading a table row.
 //
 // KSpacePME (src/cpu/longrange/pme.cpp:147-253, pme_spline.h, pme_influence.h):
 //   1. spread   one workgroup per brick of the neighbour list (or per chunk
