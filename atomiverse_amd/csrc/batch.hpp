@@ -25,14 +25,26 @@ struct op_threads<Op, std::void_t<decltype(Op::kThreads)>> {
     static constexpr int value = Op::kThreads;
 };
 
+// Minimum waves per SIMD an op asks the register allocator to allow:
+// Op::kMinWaves where declared, else 1 (no constraint).
+template <class Op, class = void>
+struct op_waves {
+    static constexpr int value = 1;
+};
 template <class Op>
-__global__ __launch_bounds__(op_threads<Op>::value) void k_single(typename Op::Args a) {
+struct op_waves<Op, std::void_t<decltype(Op::kMinWaves)>> {
+    static constexpr int value = Op::kMinWaves;
+};
+
+template <class Op>
+__global__ __launch_bounds__(op_threads<Op>::value) __attribute__((amdgpu_waves_per_eu(op_waves<Op>::value)))
+void k_single(typename Op::Args a) {
     Op::run(a, blockIdx.x, gridDim.x);
 }
 
 template <class Op>
-__global__ __launch_bounds__(op_threads<Op>::value) void k_batched(const typename Op::Args* __restrict__ args,
-                                                    const int* __restrict__ base, int nsys) {
+__global__ __launch_bounds__(op_threads<Op>::value) __attribute__((amdgpu_waves_per_eu(op_waves<Op>::value)))
+void k_batched(const typename Op::Args* __restrict__ args, const int* __restrict__ base, int nsys) {
     const int blk = blockIdx.x;
     int lo = 0, hi = nsys - 1; // largest s with base[s] <= blk
     while (lo < hi) {

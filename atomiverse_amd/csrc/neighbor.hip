@@ -556,23 +556,27 @@ __device__ void OpHalo::run(const BrickBuildArgs& a, int bid, int nbk) {
 // Rebuild step 2 (one workgroup per brick, one wavefront per home cell). The
 // wave lays the cell's candidates out flat — every stencil column's z-run of
 // halo rows, in column order — and takes them 64 at a time, one per lane,
-// testing each against every atom of the cell (the atoms' coordinates and
-// running counts live one per lane and are read with readlane), so the
-// candidate loop does not diverge. Candidates are screened with FP32 image
-// coordinates (relative to the brick centre: |error of r^2| <= 1e-5 Å^2):
-// accepted below (rc+skin)^2 - kBand, rejected above (rc+skin)^2 + kBand, and
-// only inside the band decided by the exact reference predicate — raw FP64
-// positions, the shift form of the half-stencil offset in its canonical
-// orientation (refresh_half, cell_list.cpp:680-727). The pair set is
-// therefore exactly the reference's. Accepted rows are compacted with a
-// ballot in candidate order (column order, then halo row order:
-// deterministic) into 16-bit rows in 8-wide groups; the unused tail of the
-// last group holds the sentinel row. A cell with more than kCandCap
+// testing each against the cell's atoms two at a time (their coordinates
+// and running counts are wave-uniform scalars), so the candidate loop does
+// not diverge. Candidates are screened with FP32 image coordinates (relative
+// to the brick centre: |error of r^2| <= 1e-5 Å^2): accepted below
+// (rc+skin)^2 - kBand, rejected above (rc+skin)^2 + kBand, and only inside
+// the band decided by the exact reference predicate — raw FP64 positions,
+// the shift form of the half-stencil offset in its canonical orientation
+// (refresh_half, cell_list.cpp:680-727). The pair set is therefore exactly
+// the reference's. Accepted rows are compacted with a ballot in candidate
+// order (column order, then halo row order: deterministic) into 16-bit rows
+// in 8-wide groups; the unused tail of the last group holds the sentinel
+// row. While the list capacity fits kOutCap the rows of an atom collect in
+// LDS and leave as whole 16-byte groups (one store per lane); otherwise
+// each accepted row is stored directly. A cell with more than kCandCap
 // candidates walks its columns one at a time instead.
 constexpr float kBand = 1e-4f;
-constexpr int kCandCap = 768; // flat candidates per wavefront (LDS)
+constexpr int kCandCap = 768; // flat candidates per wavefront (LDS); a multiple of 64
+constexpr int kOutCap = 256;  // list entries per atom staged in LDS
+constexpr int kOutGroups = kOutCap / kGroup + 1; // + one group that absorbs rows beyond the capacity
 
-__device__ __noinline__ bool exact_pair(const double* __restrict__ x, const double* __restrict__ y,
+__device__ __forceinline__ bool exact_pair(const double* __restrict__ x, const double* __restrict__ y,
                                         const double* __restrict__ z, double rc_sq, int gi, int gj,
                                         int kind, double shx, double shy, double shz) {
     const double xi = x[gi], yi = y[gi], zi = z[gi];
@@ -591,16 +595,25 @@ __device__ __noinline__ bool exact_pair(const double* __restrict__ x, const doub
     return r2 < rc_sq;
 }
 
-// The reference predicate for a candidate inside the FP32 band: the stencil
-// offset (di, dj, dk) from the halo cell packed in hc, the periodic shift of
-// the neighbour cell, and the exact FP64 test in the canonical orientation.
-// (Arguments by value: a reference to the kernel's argument block would be
-// copied to scratch to take its address.)
-__device__ __noinline__ bool band_pair(const double* __restrict__ x, const double* __restrict__ y,
+// The reference predicate for a candidate inside the FP32 band: the halo
+// cell of row q (the last cell whose first row is <= q), hence the stencil
+// offset (di, dj, dk), the periodic shift of the neighbour cell, and the
+// exact FP64 test in the canonical orientation. (Arguments by value: a
+// reference to the kernel's argument block would be copied to scratch to
+// take its address.)
+__device__ __forceinline__ bool band_pair(const double* __restrict__ x, const double* __restrict__ y,
                                        const double* __restrict__ z, double rc_sq, double Lx, double Ly, double Lz,
-                                       int ncx, int ncy, int ncz, int hc, int hx0, int hy0, int hz0, int cx, int cy,
-                                       int cz, int gi, int gj) {
-    const int di = (hc >> 16) - hx0, dj = ((hc >> 8) & 0xff) - hy0, dk = (hc & 0xff) - hz0;
+                                       int ncx, int ncy, int ncz, const int* h_off, int ncell, int HY, int HZ, int q,
+                                       int hx0, int hy0, int hz0, int cx, int cy, int cz, int gi, int gj) {
+    int lo = 0, hi = ncell; // h_off[lo] <= q < h_off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (h_off[mid] <= q)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const int di = lo / (HY * HZ) - hx0, dj = (lo / HZ) % HY - hy0, dk = lo % HZ - hz0;
     const int ux = cx + di, uy = cy + dj, uz = cz + dk;
     const double shx = ux < 0 ? -Lx : (ux >= ncx ? Lx : 0.0);
     const double shy = uy < 0 ? -Ly : (uy >= ncy ? Ly : 0.0);
@@ -630,19 +643,26 @@ __device__ __forceinline__ int wave_inclusive_sum(int x) {
 
 struct OpPairs {
     using Args = BrickBuildArgs;
-    static constexpr int kThreads = 512;
+    static constexpr int kThreads = 1024;
+    static constexpr int kMinWaves = 8;
+    static constexpr int kWaves = kThreads / 64;
     static __device__ void run(const BrickBuildArgs& a, int bid, int nbk);
 };
 
-__device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
-    if (!*a.pred)
-        return;
-    constexpr int kWaves = kThreads / 64;
-    __shared__ float4 sp[kHaloCap]; // image x, y, z relative to the brick centre; w = halo cell hx<<16|hy<<8|hz
-    __shared__ int h_off[kHaloCellCap + 1];
-    __shared__ unsigned short cand[kWaves][kCandCap];
-    __shared__ long long sh_tot[kWaves];
-    __shared__ int sh_max[kWaves], sh_over[kWaves];
+// LDS of one OpPairs workgroup (48.5 KB: three workgroups per CU).
+struct PairsLds {
+    float sx[kHaloCap], sy[kHaloCap], sz[kHaloCap]; // image coordinates relative to the brick centre
+    int h_off[kHaloCellCap + 1];
+    unsigned short cand[OpPairs::kWaves][kCandCap];
+    uint4 out[OpPairs::kWaves][2][kOutGroups]; // staged rows of the wave's two atoms
+    long long tot[OpPairs::kWaves];
+    int max[OpPairs::kWaves], over[OpPairs::kWaves];
+    int next_cell; // next unclaimed cell of the brick
+};
+
+template <bool STAGED>
+__device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int nbk, PairsLds& L) {
+    constexpr int kWaves = OpPairs::kWaves;
     // wave-uniform values go through readfirstlane so their loops run on
     // scalar registers
     const int lane = threadIdx.x & 63;
@@ -654,8 +674,10 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
     const float rc2f = static_cast<float>(a.rc_sq);
     const float lo_acc = rc2f - kBand, hi_rej = rc2f + kBand;
     const int ncols = a.nrows;
+    const int cap = a.cap;
     // stencil column of this lane (lanes >= ncols have none)
     const int4 rw = lane < ncols ? a.rows[lane] : make_int4(0, 0, 0, 0);
+    unsigned short* const obuf = reinterpret_cast<unsigned short*>(&L.out[wid][0][0]);
     for (int t = bid; t < a.nbricks; t += nbk) {
         const int b = xcd_brick(t, a.nbricks);
         const int nh = a.halo_n[b];
@@ -676,47 +698,48 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
         const double bc2 = (o2 + 0.5 * d2) * a.g.cs[2];
         const int* __restrict__ hoff = a.halo_off + (int64_t)b * (kHaloCellCap + 1);
         for (int h = threadIdx.x; h <= ncell; h += blockDim.x)
-            h_off[h] = hoff[h];
+            L.h_off[h] = hoff[h];
         const int* __restrict__ rows = a.halo_rows + (int64_t)b * kHaloCap;
-        __syncthreads();
-        // halo cell of every row (per halo cell), then the coordinates (per
-        // row: independent loads, so many are in flight at once)
-        for (int h = threadIdx.x; h < ncell; h += blockDim.x) {
-            const int q0 = h_off[h], q1 = h_off[h + 1];
-            const int code = ((h / (HY * HZ)) << 16) | (((h / HZ) % HY) << 8) | (h % HZ);
-            for (int q = q0; q < q1; ++q)
-                sp[q].w = __int_as_float(code);
-        }
-        const int nh_rows = h_off[ncell];
-        for (int q = threadIdx.x; q < nh_rows; q += blockDim.x) {
+        // coordinates per row (independent loads, so many are in flight at once)
+        for (int q = threadIdx.x; q < nh; q += blockDim.x) {
             const int row = rows[q];
             const int gs = row & kRowSlotMask;
             const int code = static_cast<unsigned>(row) >> kRowSlotBits;
-            sp[q].x = static_cast<float>((a.x[gs] + shift_of(code, 0, Lx)) - bc0);
-            sp[q].y = static_cast<float>((a.y[gs] + shift_of(code, 1, Ly)) - bc1);
-            sp[q].z = static_cast<float>((a.z[gs] + shift_of(code, 2, Lz)) - bc2);
+            L.sx[q] = static_cast<float>((a.x[gs] + shift_of(code, 0, Lx)) - bc0);
+            L.sy[q] = static_cast<float>((a.y[gs] + shift_of(code, 1, Ly)) - bc1);
+            L.sz[q] = static_cast<float>((a.z[gs] + shift_of(code, 2, Lz)) - bc2);
         }
+        if (threadIdx.x == 0)
+            L.next_cell = kWaves;
         __syncthreads();
-        uint16_t* __restrict__ lst = a.nbr16 + (int64_t)a0 * a.cap;
+        uint16_t* __restrict__ lst = a.nbr16 + (int64_t)a0 * cap;
         const int nbc = d0 * d1 * d2;
-        for (int c = wid; c < nbc; c += kWaves) {
+        // cells are handed out one at a time (wave w starts with cell w and
+        // then takes the next unclaimed one), which evens out the waves' loads
+        // when cells hold different numbers of atoms
+        int c = wid;
+        while (c < nbc) {
+            int grab = 0;
+            if (lane == 0)
+                grab = atomicAdd(&L.next_cell, 1);
+            [&] {
             // cells of a brick are ranked in this order, so the cell's atoms are
             // the slots from cell_start of its rank on
             const int slot0 = __builtin_amdgcn_readfirstlane(a.start[cbase + c]);
             const int lz = c % d2, ly = (c / d2) % d1, lx = c / (d1 * d2);
             const int hz0 = lz + R2;
             const int hs = ((lx + R0) * HY + (ly + R1)) * HZ + hz0;
-            const int hq0 = __builtin_amdgcn_readfirstlane(h_off[hs]);
-            const int hq1 = __builtin_amdgcn_readfirstlane(h_off[hs + 1]);
+            const int hq0 = __builtin_amdgcn_readfirstlane(L.h_off[hs]);
+            const int hq1 = __builtin_amdgcn_readfirstlane(L.h_off[hs + 1]);
             if (hq0 == hq1)
-                continue;
+                return;
             const int cx = o0 + lx, cy = o1 + ly, cz = o2 + lz;
             // this cell's z-run of halo rows in every stencil column (lane = column)
             int qb = 0, len = 0;
             if (lane < ncols) {
                 const int hcol = ((lx + R0 + rw.x) * HY + (ly + R1 + rw.y)) * HZ + hz0;
-                qb = h_off[hcol - rw.z];
-                len = h_off[hcol + rw.z + 1] - qb;
+                qb = L.h_off[hcol - rw.z];
+                len = L.h_off[hcol + rw.z + 1] - qb;
             }
             const int inc = wave_inclusive_sum(len);
             const int total = __builtin_amdgcn_readlane(inc, 63);
@@ -724,7 +747,9 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
             if (flat) {
                 const int ex = inc - len;
                 for (int t = 0; t < len; ++t)
-                    cand[wid][ex + t] = static_cast<unsigned short>(qb + t);
+                    L.cand[wid][ex + t] = static_cast<unsigned short>(qb + t);
+                if (total + lane < ((total + 63) & ~63)) // kCandCap is a multiple of 64
+                    L.cand[wid][total + lane] = 0;
                 __builtin_amdgcn_wave_barrier();
             }
             const int li0 = slot0 - a0; // brick-local index of the cell's first atom
@@ -735,51 +760,60 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
             for (int qa = hq0; qa < hq1; qa += 2) {
                 const bool two = qa + 1 < hq1; // wave-uniform
                 const int qb2 = two ? qa + 1 : qa;
-                const float4 pva = sp[qa], pvb = sp[qb2];
-                const float ax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pva.x)));
-                const float ay = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pva.y)));
-                const float az = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pva.z)));
-                const float bx = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pvb.x)));
-                const float by = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pvb.y)));
-                const float bz = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pvb.z)));
+                const float ax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(L.sx[qa])));
+                const float ay = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(L.sy[qa])));
+                const float az = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(L.sz[qa])));
+                const float bx = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(L.sx[qb2])));
+                const float by = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(L.sy[qb2])));
+                const float bz = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(L.sz[qb2])));
                 const int lia = li0 + (qa - hq0), lib = li0 + (qb2 - hq0);
                 uint16_t* __restrict__ dsta = lst + static_cast<unsigned>(lia) * kGroup;
                 uint16_t* __restrict__ dstb = lst + static_cast<unsigned>(lib) * kGroup;
+                unsigned short* const oa = obuf;
+                unsigned short* const ob = obuf + kOutGroups * kGroup;
                 int ca = 0, cb = 0;
                 // candidate q (one per lane) against one atom: test, then append
-                // the accepted rows in lane order
-                auto test = [&](const float4& pj, int q, bool valid, float px, float py, float pz, int qs, int li,
-                                uint16_t* __restrict__ dst, int& count) {
-                    const float dx = px - pj.x, dy = py - pj.y, dz = pz - pj.z;
+                // the accepted rows in lane order. The lane masks are ballots of
+                // the compares themselves (one v_cmp each) combined on scalar
+                // registers; the band, where the FP32 screen cannot decide, is
+                // rare and takes the branch.
+                auto test = [&](float qx, float qy, float qz, int q, unsigned long long vmask, float px, float py,
+                                float pz, int qs, int li, uint16_t* __restrict__ dst, unsigned short* ost,
+                                int& count) {
+                    const float dx = px - qx, dy = py - qy, dz = pz - qz;
                     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                    const bool other = valid && q != qs;
-                    bool acc = other && r2 < lo_acc;
-                    const bool band = other && !acc && r2 <= hi_rej;
-                    if (__builtin_amdgcn_ballot_w64(band)) { // rare: the reference predicate
-                        if (band)
-                            acc = band_pair(a.x, a.y, a.z, a.rc_sq, Lx, Ly, Lz, a.g.nc[0], a.g.nc[1], a.g.nc[2],
-                                            __float_as_int(pj.w), lx + R0, ly + R1, hz0, cx, cy, cz, a0 + li,
-                                            rows[q] & kRowSlotMask);
+                    const unsigned long long other = vmask & ~__builtin_amdgcn_ballot_w64(q == qs);
+                    unsigned long long m = __builtin_amdgcn_ballot_w64(r2 < lo_acc) & other;
+                    const unsigned long long band = __builtin_amdgcn_ballot_w64(r2 <= hi_rej) & other & ~m;
+                    if (band) { // the reference predicate for the band lanes
+                        bool e = false;
+                        if (__builtin_amdgcn_inverse_ballot_w64(band))
+                            e = band_pair(a.x, a.y, a.z, a.rc_sq, Lx, Ly, Lz, a.g.nc[0], a.g.nc[1], a.g.nc[2],
+                                          L.h_off, ncell, HY, HZ, q, lx + R0, ly + R1, hz0, cx, cy, cz, a0 + li,
+                                          rows[q] & kRowSlotMask);
+                        m |= __builtin_amdgcn_ballot_w64(e);
                     }
-                    const unsigned long long m = __builtin_amdgcn_ballot_w64(acc);
-                    if (acc) {
+                    if (__builtin_amdgcn_inverse_ballot_w64(m)) {
                         const unsigned pos = static_cast<unsigned>(count + lanes_below(m));
-                        if (pos < static_cast<unsigned>(a.cap))
+                        if (STAGED) // rows beyond the capacity land in the last group, which is never stored
+                            ost[min(pos, static_cast<unsigned>(kOutCap))] = static_cast<unsigned short>(q);
+                        else if (pos < static_cast<unsigned>(cap))
                             dst[__umul24(pos >> 3, gstride) + (pos & 7)] = static_cast<uint16_t>(q);
                     }
                     count += static_cast<int>(__popcll(m));
                 };
-                auto step = [&](int q, bool valid) {
-                    const float4 pj = sp[q];
-                    test(pj, q, valid, ax, ay, az, qa, lia, dsta, ca);
+                auto step = [&](int q, unsigned long long vmask) {
+                    const float qx = L.sx[q], qy = L.sy[q], qz = L.sz[q];
+                    test(qx, qy, qz, q, vmask, ax, ay, az, qa, lia, dsta, oa, ca);
                     if (two)
-                        test(pj, q, valid, bx, by, bz, qb2, lib, dstb, cb);
+                        test(qx, qy, qz, q, vmask, bx, by, bz, qb2, lib, dstb, ob, cb);
                 };
                 if (flat) {
+                    // the candidate slots past `total` up to the next multiple of
+                    // 64 hold row 0 (a valid LDS row; masked out by `valid`)
                     for (int k0 = 0; k0 < total; k0 += 64) {
                         const int cc = k0 + lane;
-                        const bool valid = cc < total;
-                        step(valid ? static_cast<int>(cand[wid][cc]) : qa, valid);
+                        step(static_cast<int>(L.cand[wid][cc]), __builtin_amdgcn_ballot_w64(cc < total));
                     }
                 } else {
                     for (int col = 0; col < ncols; ++col) {
@@ -787,32 +821,55 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
                         const int clen = __builtin_amdgcn_readlane(len, col);
                         for (int k0 = 0; k0 < clen; k0 += 64) {
                             const bool valid = k0 + lane < clen;
-                            step(valid ? cqb + k0 + lane : qa, valid);
+                            step(valid ? cqb + k0 + lane : qa, __builtin_amdgcn_ballot_w64(valid));
                         }
                     }
                 }
                 // pad the last group with the sentinel row (staged out of reach by
                 // the pair kernels) so they can read whole groups; counts, stats
                 auto finish = [&](int qs, int li, uint16_t* __restrict__ dst, int count) {
-                    const int pos = count + lane;
-                    if (pos < ((count + kGroup - 1) & ~(kGroup - 1)) && pos < a.cap)
-                        dst[__umul24(static_cast<unsigned>(pos) >> 3, gstride) + (pos & 7)] =
-                            static_cast<uint16_t>(kSentinelRow);
+                    if (!STAGED) {
+                        const int pos = count + lane;
+                        if (pos < ((count + kGroup - 1) & ~(kGroup - 1)) && pos < cap)
+                            dst[__umul24(static_cast<unsigned>(pos) >> 3, gstride) + (pos & 7)] =
+                                static_cast<uint16_t>(kSentinelRow);
+                    }
                     if (lane == 0) {
-                        a.numneigh[a0 + li] = count < a.cap ? count : a.cap;
+                        a.numneigh[a0 + li] = count < cap ? count : cap;
                         a.self16[a0 + li] = static_cast<uint16_t>(qs);
                         my_cnt_sum += count;
                         my_max = max(my_max, count);
-                        if (count > a.cap)
+                        if (count > cap)
                             ++my_over;
                     }
                 };
                 finish(qa, lia, dsta, ca);
                 if (two)
                     finish(qb2, lib, dstb, cb);
+                if (STAGED) {
+                    // lanes 0-31 hold atom a's groups, lanes 32-63 atom b's (at
+                    // most kOutCap / kGroup = 32 each); the kept rows are the
+                    // first min(count, cap) (cap is a multiple of kGroup), the
+                    // tail of the last group is the sentinel
+                    const int hb = lane >> 5, g = lane & 31;
+                    const int cnt = hb ? cb : ca;
+                    const int kept = min(cnt, cap);
+                    const int pad = (kept + kGroup - 1) & ~(kGroup - 1);
+                    const bool mine = hb == 0 || two;
+                    if (mine && g < pad - kept)
+                        (hb ? ob : oa)[kept + g] = static_cast<unsigned short>(kSentinelRow);
+                    __builtin_amdgcn_wave_barrier();
+                    if (mine && g < (pad >> 3)) {
+                        const uint4 v = L.out[wid][hb][g];
+                        *reinterpret_cast<uint4*>((hb ? dstb : dsta) + __umul24(static_cast<unsigned>(g), gstride)) = v;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
             }
+            }();
+            c = __builtin_amdgcn_readfirstlane(grab);
         }
-        __syncthreads(); // sp and h_off are refilled for the next brick
+        __syncthreads(); // the coordinates and h_off are refilled for the next brick
     }
     for (int off = 32; off > 0; off >>= 1) {
         my_cnt_sum += __shfl_down(my_cnt_sum, off, 64);
@@ -820,24 +877,34 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
         my_over += __shfl_down(my_over, off, 64);
     }
     if (lane == 0) {
-        sh_tot[wid] = my_cnt_sum;
-        sh_max[wid] = my_max;
-        sh_over[wid] = my_over;
+        L.tot[wid] = my_cnt_sum;
+        L.max[wid] = my_max;
+        L.over[wid] = my_over;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         long long t = 0;
         int m = 0, ov = 0;
         for (int w = 0; w < kWaves; ++w) {
-            t += sh_tot[w];
-            m = max(m, sh_max[w]);
-            ov += sh_over[w];
+            t += L.tot[w];
+            m = max(m, L.max[w]);
+            ov += L.over[w];
         }
         atomicAdd(reinterpret_cast<unsigned long long*>(&a.ctl->nbr_total), static_cast<unsigned long long>(t));
         atomicMax(&a.ctl->max_nbr, m);
         if (ov)
             atomicAdd(&a.ctl->overflow, ov);
     }
+}
+
+__device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
+    if (!*a.pred)
+        return;
+    __shared__ PairsLds lds;
+    if (a.cap <= kOutCap)
+        pairs_run<true>(a, bid, nbk, lds);
+    else
+        pairs_run<false>(a, bid, nbk, lds);
 }
 
 // Brick-mode export: decode halo rows back to global slots, emit tag pairs a<b.
