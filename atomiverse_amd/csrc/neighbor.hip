@@ -558,10 +558,11 @@ __device__ void OpHalo::run(const BrickBuildArgs& a, int bid, int nbk) {
 // halo rows, in column order — and takes them 64 at a time, one per lane,
 // testing each against the cell's atoms two at a time (their coordinates
 // and running counts are wave-uniform scalars), so the candidate loop does
-// not diverge. Candidates are screened with FP32 image coordinates (relative
-// to the brick centre: |error of r^2| <= 1e-5 Å^2): accepted below
-// (rc+skin)^2 - kBand, rejected above (rc+skin)^2 + kBand, and only inside
-// the band decided by the exact reference predicate — raw FP64 positions,
+// not diverge. Candidates are screened in FP32 with image coordinates X
+// relative to the brick centre, in the expanded form
+//   r^2 - |X_i|^2 = |X_j|^2 - 2 X_i . X_j   (three fma per test)
+// accepted below (rc+skin)^2 - band, rejected above (rc+skin)^2 + band, and
+// only inside the band decided by the exact reference predicate — raw FP64 positions,
 // the shift form of the half-stencil offset in its canonical orientation
 // (refresh_half, cell_list.cpp:680-727). The pair set is therefore exactly
 // the reference's. Accepted rows are compacted with a ballot in candidate
@@ -571,7 +572,16 @@ __device__ void OpHalo::run(const BrickBuildArgs& a, int bid, int nbk) {
 // LDS and leave as whole 16-byte groups (one store per lane); otherwise
 // each accepted row is stored directly. A cell with more than kCandCap
 // candidates walks its columns one at a time instead.
-constexpr float kBand = 1e-4f;
+// Half-width of the undecided band for a brick whose halo reaches E Å from
+// its centre on any axis (|X| components <= E). With unit roundoff u = 2^-24
+// the screen's error in r^2 is at most 72 u E^2 + u rc^2: the FP32 rounding
+// of the coordinates (24 u E^2 for the pair distances in play), of |X|^2
+// (9 u E^2 each for i and j), of the three fma whose partial sums stay
+// below 9 E^2 (27 u E^2) and of the threshold (3 u E^2 + u rc^2). The band
+// is twice that, plus 1e-6 Å^2.
+__device__ __forceinline__ float screen_band(float E, float rc2) {
+    return 8.6e-6f * E * E + 1.2e-7f * rc2 + 1e-6f;
+}
 constexpr int kCandCap = 768; // flat candidates per wavefront (LDS); a multiple of 64
 constexpr int kOutCap = 256;  // list entries per atom staged in LDS
 constexpr int kOutGroups = kOutCap / kGroup + 1; // + one group that absorbs rows beyond the capacity
@@ -649,9 +659,10 @@ struct OpPairs {
     static __device__ void run(const BrickBuildArgs& a, int bid, int nbk);
 };
 
-// LDS of one OpPairs workgroup (48.5 KB: three workgroups per CU).
+// LDS of one OpPairs workgroup (77.6 KB: two workgroups per CU).
 struct PairsLds {
     float sx[kHaloCap], sy[kHaloCap], sz[kHaloCap]; // image coordinates relative to the brick centre
+    float ss[kHaloCap];                             // their squared norm
     int h_off[kHaloCellCap + 1];
     unsigned short cand[OpPairs::kWaves][kCandCap];
     uint4 out[OpPairs::kWaves][2][kOutGroups]; // staged rows of the wave's two atoms
@@ -672,7 +683,6 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
     const double Lx = a.g.L[0], Ly = a.g.L[1], Lz = a.g.L[2];
     const int R0 = a.g.R[0], R1 = a.g.R[1], R2 = a.g.R[2];
     const float rc2f = static_cast<float>(a.rc_sq);
-    const float lo_acc = rc2f - kBand, hi_rej = rc2f + kBand;
     const int ncols = a.nrows;
     const int cap = a.cap;
     // stencil column of this lane (lanes >= ncols have none)
@@ -705,10 +715,19 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
             const int row = rows[q];
             const int gs = row & kRowSlotMask;
             const int code = static_cast<unsigned>(row) >> kRowSlotBits;
-            L.sx[q] = static_cast<float>((a.x[gs] + shift_of(code, 0, Lx)) - bc0);
-            L.sy[q] = static_cast<float>((a.y[gs] + shift_of(code, 1, Ly)) - bc1);
-            L.sz[q] = static_cast<float>((a.z[gs] + shift_of(code, 2, Lz)) - bc2);
+            const float X = static_cast<float>((a.x[gs] + shift_of(code, 0, Lx)) - bc0);
+            const float Y = static_cast<float>((a.y[gs] + shift_of(code, 1, Ly)) - bc1);
+            const float Z = static_cast<float>((a.z[gs] + shift_of(code, 2, Lz)) - bc2);
+            L.sx[q] = X;
+            L.sy[q] = Y;
+            L.sz[q] = Z;
+            L.ss[q] = __builtin_fmaf(Z, Z, __builtin_fmaf(Y, Y, X * X));
         }
+        // the halo reaches (d/2 + R) cells from the centre on each axis
+        const float ext = static_cast<float>(fmax(fmax((0.5 * d0 + R0) * a.g.cs[0], (0.5 * d1 + R1) * a.g.cs[1]),
+                                                  (0.5 * d2 + R2) * a.g.cs[2]));
+        const float band_w = screen_band(ext, rc2f);
+        const float lo_acc = rc2f - band_w, hi_rej = rc2f + band_w;
         if (threadIdx.x == 0)
             L.next_cell = kWaves;
         __syncthreads();
@@ -760,12 +779,13 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
             for (int qa = hq0; qa < hq1; qa += 2) {
                 const bool two = qa + 1 < hq1; // wave-uniform
                 const int qb2 = two ? qa + 1 : qa;
-                const float ax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(L.sx[qa])));
-                const float ay = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(L.sy[qa])));
-                const float az = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(L.sz[qa])));
-                const float bx = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(L.sx[qb2])));
-                const float by = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(L.sy[qb2])));
-                const float bz = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(L.sz[qb2])));
+                // -2 X of the two atoms and their thresholds (rc+skin)^2 -/+ band - |X|^2
+                auto uni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
+                const float ax = uni(-2.0f * L.sx[qa]), ay = uni(-2.0f * L.sy[qa]), az = uni(-2.0f * L.sz[qa]);
+                const float bx = uni(-2.0f * L.sx[qb2]), by = uni(-2.0f * L.sy[qb2]), bz = uni(-2.0f * L.sz[qb2]);
+                const float sa = L.ss[qa], sb = L.ss[qb2];
+                const float tlo_a = uni(lo_acc - sa), thi_a = uni(hi_rej - sa);
+                const float tlo_b = uni(lo_acc - sb), thi_b = uni(hi_rej - sb);
                 const int lia = li0 + (qa - hq0), lib = li0 + (qb2 - hq0);
                 uint16_t* __restrict__ dsta = lst + static_cast<unsigned>(lia) * kGroup;
                 uint16_t* __restrict__ dstb = lst + static_cast<unsigned>(lib) * kGroup;
@@ -777,14 +797,13 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
                 // the compares themselves (one v_cmp each) combined on scalar
                 // registers; the band, where the FP32 screen cannot decide, is
                 // rare and takes the branch.
-                auto test = [&](float qx, float qy, float qz, int q, unsigned long long vmask, float px, float py,
-                                float pz, int qs, int li, uint16_t* __restrict__ dst, unsigned short* ost,
-                                int& count) {
-                    const float dx = px - qx, dy = py - qy, dz = pz - qz;
-                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                auto test = [&](float qx, float qy, float qz, float qss, int q, unsigned long long vmask, float px,
+                                float py, float pz, float tlo, float thi, int qs, int li, uint16_t* __restrict__ dst,
+                                unsigned short* ost, int& count) {
+                    const float v = __builtin_fmaf(px, qx, __builtin_fmaf(py, qy, __builtin_fmaf(pz, qz, qss)));
                     const unsigned long long other = vmask & ~__builtin_amdgcn_ballot_w64(q == qs);
-                    unsigned long long m = __builtin_amdgcn_ballot_w64(r2 < lo_acc) & other;
-                    const unsigned long long band = __builtin_amdgcn_ballot_w64(r2 <= hi_rej) & other & ~m;
+                    unsigned long long m = __builtin_amdgcn_ballot_w64(v < tlo) & other;
+                    const unsigned long long band = __builtin_amdgcn_ballot_w64(v <= thi) & other & ~m;
                     if (band) { // the reference predicate for the band lanes
                         bool e = false;
                         if (__builtin_amdgcn_inverse_ballot_w64(band))
@@ -803,10 +822,10 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
                     count += static_cast<int>(__popcll(m));
                 };
                 auto step = [&](int q, unsigned long long vmask) {
-                    const float qx = L.sx[q], qy = L.sy[q], qz = L.sz[q];
-                    test(qx, qy, qz, q, vmask, ax, ay, az, qa, lia, dsta, oa, ca);
+                    const float qx = L.sx[q], qy = L.sy[q], qz = L.sz[q], qss = L.ss[q];
+                    test(qx, qy, qz, qss, q, vmask, ax, ay, az, tlo_a, thi_a, qa, lia, dsta, oa, ca);
                     if (two)
-                        test(qx, qy, qz, q, vmask, bx, by, bz, qb2, lib, dstb, ob, cb);
+                        test(qx, qy, qz, qss, q, vmask, bx, by, bz, tlo_b, thi_b, qb2, lib, dstb, ob, cb);
                 };
                 if (flat) {
                     // the candidate slots past `total` up to the next multiple of
