@@ -33,6 +33,7 @@ SIGNATURES = {
     "atvh_ctx_stream": (_vp, [_vp]),
     "atvh_ctx_synchronize": (C.c_int, [_vp]),
     "atvh_ctx_enable_timing": (C.c_int, [_vp, C.c_int]),
+    "atvh_ctx_timing_only": (C.c_int, [_vp, C.c_char_p]),
     "atvh_ctx_kernel_time": (C.c_int, [_vp, C.c_char_p, _dp, _lp]),
     "atvh_ctx_reset_timing": (C.c_int, [_vp]),
     "atvh_ctx_mem_info": (C.c_int, [_vp, _lp, _lp]),

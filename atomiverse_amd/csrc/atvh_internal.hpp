@@ -79,6 +79,7 @@ struct atvh_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     bool timing = false;
+    std::string timing_only; // empty: every kernel family
     std::map<std::string, atvh::TimerSlot> timers;
     std::vector<hipEvent_t> event_pool;
 };

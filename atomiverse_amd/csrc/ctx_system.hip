@@ -25,7 +25,7 @@ static hipEvent_t take_event(atvh_ctx* c) {
 }
 
 ScopedTimer::ScopedTimer(atvh_ctx* c, const char* nm) : ctx(c), name(nm) {
-    if (ctx->timing) {
+    if (ctx->timing && (ctx->timing_only.empty() || ctx->timing_only == nm)) {
         start = take_event(ctx);
         ATVH_HIP(hipEventRecord(start, ctx->stream));
     }
@@ -191,6 +191,15 @@ int atvh_ctx_enable_timing(atvh_ctx* c, int enable) {
         return ATVH_E_ARG;
     c->timing = enable != 0;
     return ATVH_OK;
+}
+
+int atvh_ctx_timing_only(atvh_ctx* c, const char* name) {
+    if (!c)
+        return ATVH_E_ARG;
+    API_TRY
+    c->timing_only = name ? name : "";
+    return ATVH_OK;
+    API_CATCH(c)
 }
 
 int atvh_ctx_kernel_time(atvh_ctx* c, const char* name, double* total_ms, int64_t* launches) {

@@ -64,7 +64,9 @@ class Context:
     def synchronize(self):
         _check(self.lib.atvh_ctx_synchronize(self.h), self.h)
 
-    def enable_timing(self, on=True):
+    def enable_timing(self, on=True, only=None):
+        """Per-kernel HIP-event timing; `only` restricts it to one kernel family."""
+        _check(self.lib.atvh_ctx_timing_only(self.h, None if only is None else only.encode()), self.h)
         _check(self.lib.atvh_ctx_enable_timing(self.h, 1 if on else 0), self.h)
 
     def reset_timing(self):

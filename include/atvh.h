@@ -57,6 +57,10 @@ int atvh_ctx_synchronize(atvh_ctx* ctx);
 /* Per-kernel timing on the context stream with HIP events (off by default).
  * name = kernel family ("pair", "integrate", "nlist", "pme", ...). */
 int atvh_ctx_enable_timing(atvh_ctx* ctx, int enable);
+/* Restrict the timing events to one kernel family (NULL: every family).
+ * Every event pair records two markers into the stream, so a throughput
+ * measurement times only the kernel it needs. */
+int atvh_ctx_timing_only(atvh_ctx* ctx, const char* name);
 int atvh_ctx_kernel_time(atvh_ctx* ctx, const char* name, double* total_ms,
                          int64_t* launches);
 int atvh_ctx_reset_timing(atvh_ctx* ctx);
