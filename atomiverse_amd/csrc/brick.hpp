@@ -19,14 +19,17 @@ namespace atvh {
 constexpr int kHaloCap = 2048;      // LDS rows of a staged halo
 constexpr int kSentinelRow = kHaloCap - 1; // staged far away; pads the last list group
 constexpr int kHaloRows = kHaloCap - 1;    // halo atoms a brick may hold
+constexpr int kHaloFillMax = kHaloRows * 15 / 16; // fullest halo a list build accepts for its shape
 // Sentinel coordinates: every pair with the sentinel is far beyond any cutoff
 // (r^2 ~ 3e30 Å^2 in FP32 relative coordinates, ~3e36 in FP64 absolute ones)
 // and its arithmetic stays finite.
 constexpr double kSentinelFar = 1.0e18;
 constexpr float kSentinelFarF = 1.0e15f;
-constexpr int kHaloCellCap = 768;   // halo cells per brick ((4+4)*(4+4)*(5+4)=576)
+constexpr int kHaloCellCap = 768;   // halo cells per brick ((4+4)*(5+4)*(5+4)=648)
 constexpr int kRowSlotBits = 26;    // halo row = global slot | image code << 26
 constexpr int kRowSlotMask = (1 << kRowSlotBits) - 1;
+constexpr int kNumBrickCands = 12;                 // brick shapes, largest first
+constexpr int kLastBrickCand = kNumBrickCands - 1; // the one-cell brick
 constexpr int kGroup = 8;           // neighbour indices per 16-byte load
 
 struct BrickGeom {

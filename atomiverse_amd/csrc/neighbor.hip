@@ -508,6 +508,7 @@ struct BrickBuildArgs {
 
 constexpr int kMaxCols = 25;     // stencil columns: (2*reach+1)^2 with reach <= 2
 constexpr int kMaxBrickAtoms = 320;
+constexpr int kBrickAtomsFillMax = kMaxBrickAtoms * 15 / 16; // fullest brick a list build accepts for its shape
 
 // Rebuild step 1 (one workgroup per brick, small LDS, high occupancy): the
 // brick's halo table from the static halo-cell table and the new cell_start —
@@ -534,6 +535,12 @@ __device__ void OpHalo::run(const BrickBuildArgs& a, int bid, int nbk) {
         if (threadIdx.x == 0) {
             a.halo_n[b] = ht.natoms;
             a.brick_range[b] = make_int2(a0, nat);
+            // fill statistics: an atomic only where this brick raises the
+            // maximum seen so far (every brick on one address would queue)
+            if (ht.natoms > __hip_atomic_load(&a.ctl->max_halo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMax(&a.ctl->max_halo, ht.natoms);
+            if (nat > __hip_atomic_load(&a.ctl->max_brick_atoms, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMax(&a.ctl->max_brick_atoms, nat);
             if (ht.natoms > kHaloRows || nat > kMaxBrickAtoms)
                 atomicAdd(&a.ctl->halo_overflow, 1);
         }
@@ -910,7 +917,8 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
             ov += L.over[w];
         }
         atomicAdd(reinterpret_cast<unsigned long long*>(&a.ctl->nbr_total), static_cast<unsigned long long>(t));
-        atomicMax(&a.ctl->max_nbr, m);
+        if (m > __hip_atomic_load(&a.ctl->max_nbr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&a.ctl->max_nbr, m);
         if (ov)
             atomicAdd(&a.ctl->overflow, ov);
     }
@@ -996,6 +1004,8 @@ __global__ void k_nl_stats_reset(DevCtl* ctl) {
         ctl->max_nbr = 0;
         ctl->overflow = 0;
         ctl->halo_overflow = 0;
+        ctl->max_halo = 0;
+        ctl->max_brick_atoms = 0;
     }
 }
 
@@ -1039,21 +1049,30 @@ __global__ void k_nl_export_count(int64_t n, const int* __restrict__ tag,
 namespace atvh {
 
 // Brick size: the largest candidate whose expected halo fills at most 70% of
-// the LDS rows (uniform density), never larger than the grid.
+// the LDS rows (uniform density) and whose expected atoms fit a 256-thread
+// block, never larger than the grid. The largest shape, 4x5x5, may fill 78%:
+// at about 2.4 atoms per cell (dense LJ liquids at the usual cutoff) it keeps
+// two threads per atom busy in the 512-thread mixed pair kernel (4x5x5 vs
+// 4x4x5 at 2.048M atoms: 141 vs 158 us); the list build then measures the
+// real fill and steps down when the input is denser than uniform.
+static const int kBrickCand[kNumBrickCands][3] = {{4, 5, 5}, {4, 4, 5}, {4, 4, 4}, {3, 4, 4},
+                                                  {3, 3, 4}, {3, 3, 3}, {2, 3, 3}, {2, 2, 3},
+                                                  {2, 2, 2}, {1, 2, 2}, {1, 1, 2}, {1, 1, 1}};
+
 static void choose_bricks(atvh_system* s, int first) {
     const double vol = s->box[0] * s->box[1] * s->box[2];
     const double per_cell = double(s->n) / vol * s->cs[0] * s->cs[1] * s->cs[2];
-    static const int cand[][3] = {{4, 4, 5}, {4, 4, 4}, {3, 4, 4}, {3, 3, 4}, {3, 3, 3},
-                                  {2, 3, 3}, {2, 2, 3}, {2, 2, 2}, {1, 2, 2}, {1, 1, 2}, {1, 1, 1}};
-    int pick = 10;
-    for (int k = first < 10 ? first : 10; k < 11; ++k) {
+    const auto& cand = kBrickCand;
+    int pick = kLastBrickCand;
+    for (int k = first < kLastBrickCand ? first : kLastBrickCand; k < kNumBrickCands; ++k) {
         int d[3], hc = 1, bc = 1;
         for (int a = 0; a < 3; ++a) {
             d[a] = std::min(cand[k][a], s->nc[a]);
             hc *= d[a] + 2 * s->reach[a];
             bc *= d[a];
         }
-        if (hc <= kHaloCellCap && per_cell * hc <= 0.7 * kHaloCap && per_cell * bc <= kBlock) {
+        const double fill = k == 0 ? 0.78 : 0.7;
+        if (hc <= kHaloCellCap && per_cell * hc <= fill * kHaloCap && per_cell * bc <= kBlock) {
             pick = k;
             break;
         }
@@ -1607,7 +1626,14 @@ int atvh_nlist_build(atvh_system* s) {
         ATVH_HIP(hipStreamSynchronize(c->stream));
         DevCtl h{};
         ATVH_HIP(hipMemcpy(&h, s->ctl, sizeof(h), hipMemcpyDeviceToHost));
-        if (h.overflow == 0 && h.halo_overflow == 0) {
+        // a brick shape whose fullest halo or brick (measured on these
+        // positions) leaves less than a sixteenth of the LDS rows or of the
+        // brick's atom slots free is given up for the next smaller one:
+        // denser-than-uniform inputs keep some headroom for the rebuilds of
+        // the run
+        const bool tight = s->brick_mode && s->brick_pick < kLastBrickCand &&
+                           (h.max_halo > kHaloFillMax || h.max_brick_atoms > kBrickAtomsFillMax);
+        if (h.overflow == 0 && h.halo_overflow == 0 && !tight) {
             h.ago = 0;
             h.nrebuilds = 1;
             h.disp_flag = 0;
@@ -1616,9 +1642,9 @@ int atvh_nlist_build(atvh_system* s) {
             s->nl_built = true;
             return ATVH_OK;
         }
-        if (h.halo_overflow) {
+        if (h.halo_overflow || tight) {
             // a denser-than-uniform region: fall back to the next smaller brick
-            if (s->brick_pick >= 10)
+            if (s->brick_pick >= kLastBrickCand)
                 break;
             s->brick_pick_min = s->brick_pick + 1;
             nl_setup_grid(s);
@@ -1649,8 +1675,9 @@ int atvh_nlist_set_brick_shape(atvh_system* s, int candidate) {
     API_TRY
     if (!s->nl_configured)
         throw Error(ATVH_E_STATE, "atvh_nlist_set_brick_shape: neighbor list not configured");
-    if (candidate < 0 || candidate > 10)
-        throw Error(ATVH_E_ARG, "atvh_nlist_set_brick_shape: candidate out of range [0, 10]");
+    if (candidate < 0 || candidate > kLastBrickCand)
+        throw Error(ATVH_E_ARG, "atvh_nlist_set_brick_shape: candidate out of range [0, " +
+                                    std::to_string(kLastBrickCand) + "]");
     ATVH_HIP(hipSetDevice(c->device));
     ATVH_HIP(hipStreamSynchronize(c->stream));
     if (s->brick_mode) {
@@ -1694,7 +1721,7 @@ int atvh_nlist_autotune(atvh_system* s, int precision, int trials, int* chosen, 
     double best_t = 1e300;
     std::vector<int> tried;
     try {
-        for (int k = k0; k <= std::min(k0 + 2, 10); ++k) {
+        for (int k = k0; k <= std::min(k0 + 2, kLastBrickCand); ++k) {
             s->brick_pick_min = k;
             nl_setup_grid(s);
             if (std::find(tried.begin(), tried.end(), s->brick_pick) != tried.end())

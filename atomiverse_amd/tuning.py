@@ -61,7 +61,7 @@ def autotune_batch(ctx, systems, precision=_abi.PREC_MIXED, nclass=8, steps=10, 
         chosen = None  # None: the members' own defaults
         for off in span:
             k = k0 + off
-            if k < 0 or k > 10:
+            if k < 0 or k > 11:
                 continue
             for i in idx:
                 systems[i].set_brick_shape(k)

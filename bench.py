@@ -401,7 +401,8 @@ def main():
         "data": "synthetic (seeded FCC lattice + Maxwell-Boltzmann velocities)",
         "config": {"workload": "lj_fcc_2048000_nvt", "atoms": n, "precision": "mixed", "cutoff": 2.5,
                    "skin": 0.3, "thermostat": "nose-hoover tau*=0.5", "dt_fs": w.dt,
-                   "rebuilds_in_timed_steps": res["mixed"]["st1"]["nrebuilds"] - res["mixed"]["st0"]["nrebuilds"]},
+                   "rebuilds_in_timed_steps": res["mixed"]["st1"]["nrebuilds"] - res["mixed"]["st0"]["nrebuilds"],
+                   "brick_cells": list(res["mixed"]["s"].brick_shape()[1])},
         "fp64_value": fp64_value,
         "fp64_ms_per_step": res["fp64"]["dt"] * 1e3 / steps,
         "mixed_over_fp64": value / fp64_value,

@@ -108,7 +108,7 @@ int atvh_nlist_build(atvh_system* sys);
  * by the rebuild it decides; no host synchronisation. */
 int atvh_nlist_decide_and_rebuild(atvh_system* sys);
 /* Brick layout of the list (this backend's tiling of the reference cell
- * grid): the candidate index in use (0 = largest brick, 10 = one cell; -1
+ * grid): the candidate index in use (0 = largest brick, 11 = one cell; -1
  * for the small-grid layout) and its size in cells per axis. */
 int atvh_nlist_brick_shape(atvh_system* sys, int* candidate, int* bdim);
 /* Use candidate k or the next smaller valid one; the list must be built

@@ -21,7 +21,7 @@ def test_every_brick_shape_gives_the_reference_pair_set(hip_ctx, orc):
     want = orc.pair_keys(orc.half_pairs(*orc.build_half(d["x"], d["y"], d["z"], w.box, 2.8)), w.n)
     ref = None
     seen = set()
-    for k in range(11):
+    for k in range(12):
         s.set_brick_shape(k)
         s.upload(d["x"], d["y"], d["z"], w.mass)
         s.nlist_build()
@@ -48,7 +48,7 @@ def test_autotune_keeps_forces_and_list(hip_ctx, orc):
     s.run(20, av.PREC_MIXED)
     before = s.download()
     k, us = s.autotune(av.PREC_MIXED, trials=3)
-    assert 0 <= k <= 10 and us > 0
+    assert 0 <= k <= 11 and us > 0
     after = s.download()
     for c in before:
         np.testing.assert_array_equal(after[c], before[c])
