@@ -59,8 +59,6 @@ struct DevCtl {
     double nh_Q;         // Nose-Hoover mass Q = Ndof kB T tau^2
     double nh_ndof_kT;   // Ndof kB T
     double T_target;     // thermostat temperature (moves with replica exchanges)
-    int max_halo;        // largest brick halo (rows) since the statistics were reset
-    int max_brick_atoms; // most atoms in one brick since the statistics were reset
 };
 
 struct CoulState; // PairLJCutCoulLong tables (coul_pme.hip)

@@ -535,12 +535,6 @@ __device__ void OpHalo::run(const BrickBuildArgs& a, int bid, int nbk) {
         if (threadIdx.x == 0) {
             a.halo_n[b] = ht.natoms;
             a.brick_range[b] = make_int2(a0, nat);
-            // fill statistics: an atomic only where this brick raises the
-            // maximum seen so far (every brick on one address would queue)
-            if (ht.natoms > __hip_atomic_load(&a.ctl->max_halo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                atomicMax(&a.ctl->max_halo, ht.natoms);
-            if (nat > __hip_atomic_load(&a.ctl->max_brick_atoms, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                atomicMax(&a.ctl->max_brick_atoms, nat);
             if (ht.natoms > kHaloRows || nat > kMaxBrickAtoms)
                 atomicAdd(&a.ctl->halo_overflow, 1);
         }
@@ -917,8 +911,7 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
             ov += L.over[w];
         }
         atomicAdd(reinterpret_cast<unsigned long long*>(&a.ctl->nbr_total), static_cast<unsigned long long>(t));
-        if (m > __hip_atomic_load(&a.ctl->max_nbr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            atomicMax(&a.ctl->max_nbr, m);
+        atomicMax(&a.ctl->max_nbr, m);
         if (ov)
             atomicAdd(&a.ctl->overflow, ov);
     }
@@ -1004,8 +997,6 @@ __global__ void k_nl_stats_reset(DevCtl* ctl) {
         ctl->max_nbr = 0;
         ctl->overflow = 0;
         ctl->halo_overflow = 0;
-        ctl->max_halo = 0;
-        ctl->max_brick_atoms = 0;
     }
 }
 
@@ -1611,6 +1602,21 @@ int atvh_nlist_configure(atvh_system* s, double cutoff, double skin, int every, 
     API_CATCH(c)
 }
 
+// Fullest halo and brick of the last build (per-brick counts read back; a
+// list build only).
+static void brick_fill(atvh_system* s, int& max_halo, int& max_atoms) {
+    std::vector<int> hn(s->nbricks);
+    std::vector<int2> br(s->nbricks);
+    ATVH_HIP(hipMemcpy(hn.data(), s->halo_n, hn.size() * sizeof(int), hipMemcpyDeviceToHost));
+    ATVH_HIP(hipMemcpy(br.data(), s->brick_range, br.size() * sizeof(int2), hipMemcpyDeviceToHost));
+    max_halo = 0;
+    max_atoms = 0;
+    for (int b = 0; b < s->nbricks; ++b) {
+        max_halo = std::max(max_halo, hn[b]);
+        max_atoms = std::max(max_atoms, br[b].y);
+    }
+}
+
 int atvh_nlist_build(atvh_system* s) {
     if (!s)
         return ATVH_E_ARG;
@@ -1631,8 +1637,12 @@ int atvh_nlist_build(atvh_system* s) {
         // brick's atom slots free is given up for the next smaller one:
         // denser-than-uniform inputs keep some headroom for the rebuilds of
         // the run
-        const bool tight = s->brick_mode && s->brick_pick < kLastBrickCand &&
-                           (h.max_halo > kHaloFillMax || h.max_brick_atoms > kBrickAtomsFillMax);
+        bool tight = false;
+        if (s->brick_mode && s->brick_pick < kLastBrickCand && h.halo_overflow == 0) {
+            int max_halo = 0, max_atoms = 0;
+            brick_fill(s, max_halo, max_atoms);
+            tight = max_halo > kHaloFillMax || max_atoms > kBrickAtomsFillMax;
+        }
         if (h.overflow == 0 && h.halo_overflow == 0 && !tight) {
             h.ago = 0;
             h.nrebuilds = 1;
