@@ -58,6 +58,8 @@ SIGNATURES = {
     "atvh_pme_compute": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "atvh_pme_info": (C.c_int, [_vp, _ip, _dp]),
     "atvh_energy": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "atvh_kinetic_stress": (C.c_int, [_vp]),
+    "atvh_stress": (C.c_int, [_vp, _dp, _dp, _dp]),
     "atvh_integrator_configure": (C.c_int, [_vp, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int64]),
     "atvh_step_vvnh": (C.c_int, [_vp, C.c_int]),
     "atvh_thermostat_state": (C.c_int, [_vp, _dp, _dp]),

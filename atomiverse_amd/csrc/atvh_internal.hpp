@@ -59,6 +59,7 @@ struct DevCtl {
     double nh_Q;         // Nose-Hoover mass Q = Ndof kB T tau^2
     double nh_ndof_kT;   // Ndof kB T
     double T_target;     // thermostat temperature (moves with replica exchanges)
+    double stress_kin[6]; // last kinetic stress sum m v_a v_b (eV, Voigt xx yy zz xy xz yz)
 };
 
 struct CoulState; // PairLJCutCoulLong tables (coul_pme.hip)

@@ -22,6 +22,7 @@
 
 #include "atvh_internal.hpp"
 #include "batch.hpp"
+#include "pair_common.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -142,6 +143,30 @@ struct OpFinal {
     }
 };
 
+// The Virial observer's kinetic part (cpu::md::Virial::launch,
+// src/cpu/observe/virial_stress.cpp:14-56): sum_i m_i v_ia v_ib over the
+// current velocities (the pending Nose-Hoover scale applied), in the
+// reference's Voigt order xx yy zz xy xz yz, as fixed-order per-block
+// partials (slots 1..6 of the block's partial row).
+struct OpKinStress {
+    using Args = IntArgs;
+    static __device__ void run(const IntArgs& a, int bid, int nbk) {
+        const double s = a.thermostat ? a.ctl->nh_scale : 1.0;
+        double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int64_t i = bid * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)nbk * blockDim.x) {
+            const double m = a.mass[a.ntypes > 1 ? a.type[i] : 0];
+            const double vx = a.vx[i] * s, vy = a.vy[i] * s, vz = a.vz[i] * s;
+            acc[1] += m * vx * vx;
+            acc[2] += m * vy * vy;
+            acc[3] += m * vz * vz;
+            acc[4] += m * vx * vy;
+            acc[5] += m * vx * vz;
+            acc[6] += m * vy * vz;
+        }
+        block_partials<kBlock>(acc, a.ke_partials, bid);
+    }
+};
+
 // Fixed-order sum of nb partials (stride `stride`) over one 256-thread block.
 __device__ double block_fixed_sum(const double* p, int nb, int stride) {
     __shared__ double sh[kBlock];
@@ -213,6 +238,25 @@ struct OpPost {
                 c->max_nbr = 0;
             }
         }
+    }
+};
+
+struct KinStressArgs {
+    DevCtl* ctl;
+    const double* partials;
+    int nb;
+};
+
+// Totals of the kinetic-stress partials, converted to eV (mass_v2_to_energy).
+struct OpKinStressSum {
+    using Args = KinStressArgs;
+    static __device__ void run(const KinStressArgs& a, int, int) {
+        double v[6];
+        for (int c = 0; c < 6; ++c)
+            v[c] = block_fixed_sum(a.partials + 1 + c, a.nb, kPartialStride);
+        if (threadIdx.x == 0)
+            for (int c = 0; c < 6; ++c)
+                a.ctl->stress_kin[c] = mass_v2_to_energy * v[c];
     }
 };
 
@@ -547,6 +591,49 @@ int atvh_md_run(atvh_system* s, int nsteps, int precision, int energy_every) {
     if (h.overflow || h.halo_overflow) {
         s->nl_built = false;
         nl_check_ctl(h);
+    }
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+int atvh_kinetic_stress(atvh_system* s) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    check_integ(s);
+    ATVH_HIP(hipSetDevice(c->device));
+    const IntArgs ia = make_int_args(s);
+    const int g = integ_grid(s);
+    k_single<OpKinStress><<<g, kBlock, 0, c->stream>>>(ia);
+    KinStressArgs ka{s->ctl, ia.ke_partials, g};
+    k_single<OpKinStressSum><<<1, kBlock, 0, c->stream>>>(ka);
+    ATVH_HIP(hipGetLastError());
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+int atvh_stress(atvh_system* s, double* kinetic6, double* virial6, double* pressure) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    ATVH_HIP(hipSetDevice(c->device));
+    ATVH_HIP(hipStreamSynchronize(c->stream));
+    DevCtl h{};
+    ATVH_HIP(hipMemcpy(&h, s->ctl, sizeof(h), hipMemcpyDeviceToHost));
+    if (kinetic6)
+        for (int k = 0; k < 6; ++k)
+            kinetic6[k] = h.stress_kin[k];
+    if (virial6)
+        for (int k = 0; k < 6; ++k)
+            virial6[k] = h.vir[k];
+    if (pressure) {
+        // compute_pressure convention (virial_stress.h:30-45): P = (tr K + tr W) / (3V),
+        // positive is compressive, in eV/Å^3 (the reference's pressure unit)
+        const double vol = s->box[0] * s->box[1] * s->box[2];
+        *pressure = (h.stress_kin[0] + h.stress_kin[1] + h.stress_kin[2] + h.vir[0] + h.vir[1] + h.vir[2]) /
+                    (3.0 * vol);
     }
     return ATVH_OK;
     API_CATCH(c)

@@ -223,6 +223,18 @@ class System:
         self._ck(self.lib.atvh_energy(self.h, C.byref(pe), dptr(vir), C.byref(ke)))
         return pe.value, vir, ke.value
 
+    def kinetic_stress(self):
+        """Queue the Virial observer's kinetic stress of the current velocities."""
+        self._ck(self.lib.atvh_kinetic_stress(self.h))
+
+    def stress(self):
+        """(kinetic stress [6], virial [6], pressure) in eV, eV and eV/Å^3."""
+        kin = np.zeros(6)
+        vir = np.zeros(6)
+        p = C.c_double()
+        self._ck(self.lib.atvh_stress(self.h, dptr(kin), dptr(vir), C.byref(p)))
+        return kin, vir, p.value
+
     # integration
     def integrator(self, dt, thermostat=0, T=0.0, tau=0.0, ndof=0):
         self._ck(self.lib.atvh_integrator_configure(self.h, dt, thermostat, T, tau, ndof))

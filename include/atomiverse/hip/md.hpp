@@ -405,6 +405,26 @@ class NoseHoover : public Component {
     double T_, tau_, dt_;
 };
 
+/// cpu::md::Virial (src/cpu/observe/virial_stress.cpp:14-56; cuda::Virial,
+/// src/cuda/observe/stress.cu): at post_force (-50) the kinetic stress
+/// sum m v_a v_b of the current velocities, queued on the device; mounting it
+/// makes the force phase compute the virial (set_stress_required). Mount with
+/// sim.on(Phase::post_force).add_ref(virial, -50) to read it afterwards.
+class Virial : public Component {
+  public:
+    std::string name() const override { return "virial"; }
+    bool provides_system_stress() const { return true; }
+    std::vector<Binding> default_bindings() const override { return {{ATV::md::Phase::post_force, -50}}; }
+    void launch(const LaunchContext& c) override {
+        check(atvh_kinetic_stress(c.runtime.sys()), c.runtime.ctx(), "Virial::launch");
+    }
+    /// Kinetic stress and virial (eV, extensive, Voigt xx yy zz xy xz yz) and
+    /// the pressure (tr K + tr W) / (3V) in eV/Å^3; synchronises.
+    void read(const Context& rt, double kinetic[6], double virial[6], double* pressure) const {
+        check(atvh_stress(rt.sys(), kinetic, virial, pressure), rt.ctx(), "Virial::read");
+    }
+};
+
 /// KineticEnergy at end_of_step (-100): reduced by the final-integrate kernel.
 class KineticEnergy : public Component {
   public:
@@ -511,6 +531,11 @@ class MD {
         for (auto& op : ops_)
             if (auto* k = dynamic_cast<KSpacePME*>(op.comp))
                 k->set_standalone(!has_pair);
+        // a mounted Virial observer consumes the force phase's virial
+        // (the reference's install_defaults virial gating)
+        for (auto& op : ops_)
+            if (dynamic_cast<Virial*>(op.comp))
+                rt_->stress_required = true;
         for (auto& op : ops_)
             op.comp->set_stress_required(rt_->stress_required);
         fused_ = fused_allowed_;

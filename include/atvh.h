@@ -172,6 +172,21 @@ int atvh_pme_info(atvh_system* sys, int* K, double* e_parts);
 /* Last computed potential energy (eV) and Voigt virial (eV); synchronises. */
 int atvh_energy(atvh_system* sys, double* pe, double* virial6, double* ke);
 
+/* ---- stress -------------------------------------------------------------------------
+ * The Virial observer: replaces cpu::md::Virial::launch
+ * (src/cpu/observe/virial_stress.cpp:14-56, contract virial_stress.h:16-54) and
+ * the CUDA kinetic-stress producer with cuda::Virial
+ * (src/cuda/observe/stress.cu:1-56). Queue the kinetic stress
+ * sum_i m_i v_ia v_ib * mass_v2_to_energy (eV, extensive, Voigt xx yy zz xy
+ * xz yz) of the current velocities; no synchronisation. */
+int atvh_kinetic_stress(atvh_system* sys);
+/* The last queued kinetic stress, the virial of the last force evaluation
+ * that requested it (pair + k-space, eV, extensive) and the pressure
+ * P = (tr K + tr W) / (3V) in eV/Å^3 (compute_pressure convention, positive
+ * is compressive; virial_stress.h:30-45). Any pointer may be NULL;
+ * synchronises. */
+int atvh_stress(atvh_system* sys, double* kinetic6, double* virial6, double* pressure);
+
 /* ---- integration ------------------------------------------------------------------
  * VelocityVerlet (src/cpu/dynamics/velocity_verlet.cpp:24-68) and single-chain
  * NoseHoover (src/cpu/dynamics/nose_hoover.cpp:31-170); CUDA counterparts

@@ -531,6 +531,29 @@ double orc_kinetic_energy(int64_t n, const double* mass, const double* vx, const
     return kinetic(n, mass, vx, vy, vz);
 }
 
+// cpu::md::Virial::launch (src/cpu/observe/virial_stress.cpp:14-56): the
+// kinetic stress, serial sums in atom order, each term m * v_a * v_b as the
+// reference writes it, scaled by mass_v2_to_energy.
+void orc_kinetic_stress(int64_t n, const double* mass, const double* vx, const double* vy, const double* vz,
+                        double out[6]) {
+    double v_xx = 0, v_yy = 0, v_zz = 0, v_xy = 0, v_xz = 0, v_yz = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double m = mass[i];
+        v_xx += m * vx[i] * vx[i];
+        v_yy += m * vy[i] * vy[i];
+        v_zz += m * vz[i] * vz[i];
+        v_xy += m * vx[i] * vy[i];
+        v_xz += m * vx[i] * vz[i];
+        v_yz += m * vy[i] * vz[i];
+    }
+    out[0] = mass_v2_to_energy * v_xx;
+    out[1] = mass_v2_to_energy * v_yy;
+    out[2] = mass_v2_to_energy * v_zz;
+    out[3] = mass_v2_to_energy * v_xy;
+    out[4] = mass_v2_to_energy * v_xz;
+    out[5] = mass_v2_to_energy * v_yz;
+}
+
 // cpu::md::MD::run driving Runtime::step_verlet_ (src/cpu/driver/runtime.cpp:452-502):
 //   initial_integrate: VelocityVerlet (velocity_verlet.cpp:41-55) then
 //                      NoseHoover (nose_hoover.cpp:104-170)

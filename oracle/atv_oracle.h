@@ -191,6 +191,10 @@ int orc_md_run_ff(int64_t n, double* x, double* y, double* z, double* vx,
 double orc_kinetic_energy(int64_t n, const double* mass, const double* vx,
                           const double* vy, const double* vz);
 
+/* cpu::md::Virial::launch: mvv2e * sum m v_a v_b, Voigt xx yy zz xy xz yz */
+void orc_kinetic_stress(int64_t n, const double* mass, const double* vx,
+                        const double* vy, const double* vz, double out[6]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -97,6 +97,8 @@ def lib():
         L.orc_lj_mix.argtypes = [C.c_int, _dp, _dp]
         L.orc_kinetic_energy.restype = C.c_double
         L.orc_kinetic_energy.argtypes = [C.c_int64, _dp, _dp, _dp, _dp]
+        L.orc_kinetic_stress.restype = None
+        L.orc_kinetic_stress.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp]
         L.orc_md_run.restype = C.c_int
         L.orc_md_run.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                  C.POINTER(MDParams), C.POINTER(MDState), C.c_int, _dp, _dp, _dp]
@@ -256,6 +258,13 @@ def lj_mix(eps_tab, sigma_tab):
 
 def kinetic_energy(mass, vx, vy, vz):
     return lib().orc_kinetic_energy(mass.size, _d(mass), _d(vx), _d(vy), _d(vz))
+
+
+def kinetic_stress(mass, vx, vy, vz):
+    """cpu::md::Virial's kinetic stress (eV, Voigt xx yy zz xy xz yz)."""
+    out = np.zeros(6)
+    lib().orc_kinetic_stress(mass.size, _d(mass), _d(vx), _d(vy), _d(vz), _d(out))
+    return out
 
 
 def md_run(state_arrays, box, params: MDParams, nsteps, md_state: MDState | None = None):

@@ -99,6 +99,33 @@ int main() {
     std::printf("hip::md::MD: %zu atoms, E0/N %.6f, |dE|/N %.3e, per-phase == fused\n", n, a0 / n, drift);
     if (!(drift < 1e-2) || !std::isfinite(a1))
         return 1;
+    {
+        // the Virial observer on the per-phase walk: pressure of the LJ fluid
+        hip::HostFrame v = make_frame();
+        const double tau = std::sqrt(103.6427);
+        hip::md::MD sim;
+        hip::md::Virial virial;
+        sim.set_neighbor(hip::NeighborList{2.5, 0.3, 1, 0, true});
+        sim.set_evaluator(hip::PairLJCut{1.0, 1.0, 2.5});
+        sim.set_integrator(hip::md::VelocityVerlet{0.005 * tau});
+        sim.on(ATV::md::Phase::post_force).add_ref(virial, -50);
+        sim.build();
+        if (sim.fused()) {
+            std::fprintf(stderr, "virial: the observer needs the per-phase walk\n");
+            return 1;
+        }
+        sim.run(v, 20);
+        double kin[6], vir[6], p = 0.0;
+        virial.read(sim.context(), kin, vir, &p);
+        const double vol = v.box[0] * v.box[1] * v.box[2];
+        const double want = (kin[0] + kin[1] + kin[2] + vir[0] + vir[1] + vir[2]) / (3.0 * vol);
+        if (!std::isfinite(p) || std::fabs(p - want) > 1e-12 * std::fabs(want) || !(kin[0] > 0.0) || vir[0] == 0.0) {
+            std::fprintf(stderr, "virial: bad pressure %g (want %g)\n", p, want);
+            return 1;
+        }
+        std::printf("hip::md::Virial: P %.6f eV/A^3, tr K %.3f eV, tr W %.3f eV\n", p, kin[0] + kin[1] + kin[2],
+                    vir[0] + vir[1] + vir[2]);
+    }
     hip::HostFrame c = make_melt(), d = make_melt();
     run_melt(c, true, 60);
     run_melt(d, false, 60);

@@ -20,3 +20,4 @@ def test_hip_md_cpp_consumer():
     assert "per-phase == fused" in out.stdout
     assert "per-phase == fused (PME)" in out.stdout
     assert "hip::path::Remd: 12 attempts" in out.stdout
+    assert "hip::md::Virial: P " in out.stdout
