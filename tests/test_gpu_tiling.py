@@ -56,3 +56,26 @@ def test_autotune_keeps_forces_and_list(hip_ctx, orc):
     p = s.export_pairs()
     np.testing.assert_array_equal(np.sort(p[:, 0] * w.n + p[:, 1]), want)
     s.run(20, av.PREC_MIXED)  # the tuned layout steps on
+
+
+def test_dense_input_steps_down_and_runs(hip_ctx, orc):
+    """A trimmed FCC lattice packs its atoms at ~1.46x the nominal density in
+    part of the box: the largest brick shapes, chosen for uniform density,
+    would leave almost no halo or brick headroom there. The list build reads
+    back the fullest halo and brick and steps down; the hot ladder run that
+    follows keeps every rebuild within capacity, and the list is still the
+    reference's pair set."""
+    w = W.lj_trimmed(1406, T_star=1.5, seed=9)
+    s = av.System(hip_ctx, w.n, w.box)
+    W.upload(s, w)
+    W.configure(s, w)
+    s.set_brick_shape(0)
+    s.nlist_build()
+    k, bdim = s.brick_shape()
+    assert k > 0, bdim  # the 4x5x5 shape does not keep its headroom here
+    s.run(200, av.PREC_MIXED)  # raises if any rebuild overflowed a halo or brick
+    d = s.download()
+    s.nlist_build()
+    want = orc.pair_keys(orc.half_pairs(*orc.build_half(d["x"], d["y"], d["z"], w.box, 2.8)), w.n)
+    p = s.export_pairs()
+    np.testing.assert_array_equal(np.sort(p[:, 0] * w.n + p[:, 1]), want)
