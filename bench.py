@@ -27,6 +27,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # FETCH_SIZE / WRITE_SIZE passes scaled by a plain copy's counters (the file
 # records the commands and numbers); absent -> traffic null.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_lj_brick_mixed_lj_fcc_2048000.txt")
+VALU_FILE = os.path.join(ROOT, "profiles", "pmc_sq_k_lj_brick_mixed_lj_fcc_2048000.txt")
+CU_COUNT, CLOCK_HZ = 256, 2.4e9  # MI355X: one wave64 VALU instruction per clock per CU
 
 
 def measured_traffic():
@@ -35,6 +37,29 @@ def measured_traffic():
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def valu_issue(avg_s):
+    """The pair kernel's VALU issue time per launch against its measured duration:
+    the bound that the HBM fraction leaves unexplained."""
+    insts = measured_valu_insts()
+    if insts is None:
+        return None
+    issue_s = insts / (CU_COUNT * CLOCK_HZ)
+    return {"insts_per_launch": insts, "issue_us": issue_s * 1e6, "frac": issue_s / avg_s,
+            "source": os.path.relpath(VALU_FILE, ROOT)}
+
+
+def measured_valu_insts():
+    """VALU wave-instructions per launch of the pair kernel, from the committed
+    SQ_INSTS_VALU summary (None if absent)."""
+    try:
+        for line in open(VALU_FILE):
+            if line.startswith("valu_insts_per_launch:"):
+                return float(line.split(":")[1])
     except OSError:
         pass
     return None
@@ -409,7 +434,8 @@ def main():
         "force_rms_rel_err_mixed_vs_fp64": rms,
         "roofline": {"bound": "hbm", "kernel": "k_single<OpLjBrickMixed<false, false, false>>", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "bytes_alg_per_launch": bytes_alg, "avg_launch_us": avg_s * 1e6},
+                     "bytes_alg_per_launch": bytes_alg, "avg_launch_us": avg_s * 1e6,
+                     "valu_issue": valu_issue(avg_s)},
         "kernel_avg_us": {name: {k: v[0] * 1e3 / max(v[1], 1) for k, v in r["kt"].items()}
                           for name, r in res.items()},
     }
