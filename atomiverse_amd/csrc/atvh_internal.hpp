@@ -198,6 +198,9 @@ struct atvh_batch {
     std::vector<atvh_system*> sys;
     std::map<std::string, BatchStage> stages;
     atvh::DevCtl** d_ctls = nullptr; // member control blocks
+    int* nl_nb = nullptr;            // [kNlStages][nsys] block counts of the rebuild stages
+    int* nl_act = nullptr;           // [kNlStages][nsys+1] this step's prefix over rebuilding members
+    unsigned char* nl_flags = nullptr; // [nsys] this step's rebuild flags
     bool typed = false;
     bool use_graph = false;
     hipGraphExec_t graph[2] = {nullptr, nullptr}; // [energy step?]

@@ -129,6 +129,9 @@ void batch_free_stages(atvh_batch* b) {
         dev_free(kv.second.base);
     }
     b->stages.clear();
+    dev_free(b->nl_nb);
+    dev_free(b->nl_act);
+    dev_free(b->nl_flags);
 }
 
 } // namespace atvh

@@ -59,4 +59,29 @@ void k_batched(const typename Op::Args* __restrict__ args, const int* __restrict
     Op::run(args[lo], blk - base[lo], base[lo + 1] - base[lo]);
 }
 
+// A predicated stage over the members whose flag is set: `act` (nsys+1
+// entries, computed on the device for this step) is the prefix of the active
+// members' block counts (0 for an inactive member). A fixed grid strides over
+// the active blocks only, so members that skip the stage cost nothing beyond
+// their zero-length range. Every member still runs its own single-system
+// code, block for block.
+template <class Op>
+__global__ __launch_bounds__(op_threads<Op>::value) __attribute__((amdgpu_waves_per_eu(op_waves<Op>::value)))
+void k_batched_active(const typename Op::Args* __restrict__ args, const int* __restrict__ act, int nsys) {
+    const int total = act[nsys];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        if (t != static_cast<int>(blockIdx.x))
+            __syncthreads(); // the previous block's shared memory is reused
+        int lo = 0, hi = nsys - 1; // largest s with act[s] <= t: an active member
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (act[mid] <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        Op::run(args[lo], t - act[lo], act[lo + 1] - act[lo]);
+    }
+}
+
 } // namespace atvh

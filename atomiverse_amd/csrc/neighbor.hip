@@ -1524,26 +1524,71 @@ void nl_batch_build(atvh_batch* b) {
     batch_stage(b, "nl_copyback", ca, g[7]);
     batch_stage(b, "nl_halo", ba, g[8]);
     batch_stage(b, "nl_pairs", ba, g[9]);
+    const size_t nsys = b->sys.size();
+    std::vector<int> nb;
+    for (int k = 0; k < kNlStages; ++k)
+        nb.insert(nb.end(), g[k].begin(), g[k].end());
+    dev_free(b->nl_nb);
+    dev_free(b->nl_act);
+    dev_free(b->nl_flags);
+    dev_alloc(&b->nl_nb, nb.size());
+    dev_alloc(&b->nl_act, size_t(kNlStages) * (nsys + 1));
+    dev_alloc(&b->nl_flags, nsys);
+    ATVH_HIP(hipMemcpy(b->nl_nb, nb.data(), nb.size() * sizeof(int), hipMemcpyHostToDevice));
 }
 
+// This step's prefixes, per rebuild stage, of the block counts of the members
+// whose rebuild flag is set: the flags are read once, then wavefront k scans
+// stage k in member order, 64 members at a time (DPP prefix sums).
+constexpr int kActiveThreads = 64 * kNlStages;
+
+__global__ __launch_bounds__(kActiveThreads) void k_nl_active_prefix(DevCtl* const* __restrict__ ctls, int nsys,
+                                                                      const int* __restrict__ nb,
+                                                                      int* __restrict__ act,
+                                                                      unsigned char* __restrict__ flags) {
+    for (int m = threadIdx.x; m < nsys; m += blockDim.x)
+        flags[m] = ctls[m]->rebuild ? 1 : 0;
+    __syncthreads();
+    const int k = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int* nbk = nb + (int64_t)k * nsys;
+    int* ak = act + (int64_t)k * (nsys + 1);
+    if (lane == 0)
+        ak[0] = 0;
+    int carry = 0;
+    for (int c0 = 0; c0 < nsys; c0 += 64) {
+        const int m = c0 + lane;
+        const int v = (m < nsys && flags[m]) ? nbk[m] : 0;
+        const int inc = wave_inclusive_sum(v);
+        if (m < nsys)
+            ak[m + 1] = carry + inc;
+        carry += __builtin_amdgcn_readlane(inc, 63);
+    }
+}
+
+// One rebuild stage over the rebuilding members only: a grid of about eight
+// 256-thread blocks' worth per CU strides over their blocks.
 template <class Op>
-static void nl_batch_one(atvh_batch* b, const char* name) {
+static void nl_batch_one(atvh_batch* b, const char* name, int stage) {
     const BatchStage& st = b->stages.at(name);
-    k_batched<Op><<<st.nblocks, op_threads<Op>::value, 0, b->ctx->stream>>>(static_cast<const typename Op::Args*>(st.args),
-                                                              st.base, static_cast<int>(b->sys.size()));
+    const int nsys = static_cast<int>(b->sys.size());
+    const int grid = std::max(1, std::min(st.nblocks, 2048 * kBlock / op_threads<Op>::value));
+    k_batched_active<Op><<<grid, op_threads<Op>::value, 0, b->ctx->stream>>>(
+        static_cast<const typename Op::Args*>(st.args), b->nl_act + (int64_t)stage * (nsys + 1), nsys);
 }
 
 void nl_batch_launch_rebuild(atvh_batch* b) {
-    nl_batch_one<OpBin>(b, "nl_bin");
-    nl_batch_one<OpScanTiles>(b, "nl_scan_tiles");
-    nl_batch_one<OpScanSums>(b, "nl_scan_sums");
-    nl_batch_one<OpScanAdd>(b, "nl_scan_add");
-    nl_batch_one<OpScatter>(b, "nl_scatter");
-    nl_batch_one<OpCellOrder>(b, "nl_cell_order");
-    nl_batch_one<OpPermute>(b, "nl_permute");
-    nl_batch_one<OpCopyback>(b, "nl_copyback");
-    nl_batch_one<OpHalo>(b, "nl_halo");
-    nl_batch_one<OpPairs>(b, "nl_pairs");
+    k_nl_active_prefix<<<1, kActiveThreads, 0, b->ctx->stream>>>(b->d_ctls, static_cast<int>(b->sys.size()),
+                                                                 b->nl_nb, b->nl_act, b->nl_flags);
+    nl_batch_one<OpBin>(b, "nl_bin", 0);
+    nl_batch_one<OpScanTiles>(b, "nl_scan_tiles", 1);
+    nl_batch_one<OpScanSums>(b, "nl_scan_sums", 2);
+    nl_batch_one<OpScanAdd>(b, "nl_scan_add", 3);
+    nl_batch_one<OpScatter>(b, "nl_scatter", 4);
+    nl_batch_one<OpCellOrder>(b, "nl_cell_order", 5);
+    nl_batch_one<OpPermute>(b, "nl_permute", 6);
+    nl_batch_one<OpCopyback>(b, "nl_copyback", 7);
+    nl_batch_one<OpHalo>(b, "nl_halo", 8);
+    nl_batch_one<OpPairs>(b, "nl_pairs", 9);
     ATVH_HIP(hipGetLastError());
 }
 
