@@ -79,3 +79,33 @@ def test_dense_input_steps_down_and_runs(hip_ctx, orc):
     want = orc.pair_keys(orc.half_pairs(*orc.build_half(d["x"], d["y"], d["z"], w.box, 2.8)), w.n)
     p = s.export_pairs()
     np.testing.assert_array_equal(np.sort(p[:, 0] * w.n + p[:, 1]), want)
+
+
+def test_long_cutoff_list_paths(hip_ctx, orc):
+    """A cutoff of 4.0 + 0.4 at liquid density: more than 256 neighbours per
+    atom (list rows stored directly instead of staged per atom) and more than
+    768 candidates per cell (the build walks the stencil columns one at a
+    time): the pair set is still the reference's and the FP64 forces its
+    forces."""
+    w = W.lj_trimmed(6000, seed=13)
+    s = av.System(hip_ctx, w.n, w.box)
+    W.upload(s, w)
+    s.nlist_configure(4.0, 0.4, 1, 0, True)
+    s.pair_lj(1.0, 1.0, 4.0)
+    s.integrator(w.dt, 0, 0.0, 0.0, 0)
+    s.nlist_build()
+    s.run(30, av.PREC_FP64)  # off the lattice
+    d = s.download()
+    s.nlist_build()
+    st = s.nlist_stats()
+    assert st["capacity"] > 256 and st["max_neighbors"] > 256, st
+    half = orc.build_half(d["x"], d["y"], d["z"], w.box, 4.4, nthreads=8)
+    want = orc.pair_keys(orc.half_pairs(*half), w.n)
+    p = s.export_pairs()
+    np.testing.assert_array_equal(np.sort(p[:, 0] * w.n + p[:, 1]), want)
+    s.compute(av.PREC_FP64, True, True)
+    g = s.download()
+    fx, fy, fz, e, _ = orc.pair_lj(d["x"], d["y"], d["z"], w.box, half, 1.0, 1.0, 4.0, nthreads=8)
+    F = np.stack([fx, fy, fz])
+    G = np.stack([g["fx"], g["fy"], g["fz"]])
+    assert np.max(np.abs(G - F)) <= 1e-12 * np.max(np.abs(F))
