@@ -106,12 +106,12 @@ def lj_fcc(ncell, density=0.8442, T_star=0.72, seed=87287, thermostat=0, name=No
                       thermostat=thermostat, meta=dict(ncell=ncell, density=density, T_star=T_star, seed=seed))
 
 
-def lj_c0():
+def lj_fcc_4000():
     """4000-atom FCC LJ fluid, NVE (10^3 cells, rho*=0.8442, T*=0.72, seed 87287)."""
     return lj_fcc(10, seed=87287, thermostat=0, name="lj_fcc_4000_nve")
 
 
-def lj_c1(thermostat=1):
+def lj_fcc_2048000(thermostat=1):
     """2,048,000-atom FCC LJ fluid (80^3 cells), Nose-Hoover tau*=0.5, seed 87288."""
     return lj_fcc(80, seed=87288, thermostat=thermostat, name="lj_fcc_2048000_nvt")
 
@@ -141,7 +141,7 @@ def charged_melt(nside, density=0.9, T_star=1.2, seed=87289, thermostat=1, cutof
                       meta=dict(nside=nside, density=density, T_star=T_star, seed=seed, accuracy=accuracy))
 
 
-def lj_c2():
+def charged_melt_512000():
     """512,000-ion charged LJ melt (80^3 rock salt, rho*=0.9, T*=1.2, seed 87289), NVT
     tau*=0.5, r_c 3.0 + skin 0.3, Ewald tolerance 1e-5, PME 128^3 order 4."""
     return charged_melt(80, name="charged_melt_512000_nvt")

@@ -83,7 +83,7 @@ def stage_breakdown(ctx, s, precision, steps, names):
 
 
 def run_gpu(av, W, ctx, precision, steps, warmup):
-    w = W.lj_c1(thermostat=1)
+    w = W.lj_fcc_2048000(thermostat=1)
     s = av.System(ctx, w.n, w.box)
     W.upload(s, w)
     W.configure(s, w)
@@ -107,7 +107,7 @@ def run_gpu(av, W, ctx, precision, steps, warmup):
 
 def run_charged(av, W, ctx, precision, steps, warmup):
     """The 512,000-ion charged melt (LJ + real-space Coulomb + PME 128^3, order 4, NVT)."""
-    w = W.lj_c2()
+    w = W.charged_melt_512000()
     s = av.System(ctx, w.n, w.box)
     W.upload(s, w)
     W.configure(s, w)
@@ -150,13 +150,13 @@ def charged_section(av, W, ctx, steps, warmup):
     return out
 
 
-def nve_section(av, W, ctx, c1_steps, c2_steps, every=100):
-    """NVE side runs from the C1 and C2 starting states (thermostat off), both
+def nve_section(av, W, ctx, lj_steps, melt_steps, every=100):
+    """NVE side runs from the 2.048M-atom LJ and 512k-ion melt starting states (thermostat off), both
     precisions: total energy per atom recorded every `every` steps, its linear
     drift in eV/atom/ns, and the largest deviation from the first record."""
     out = {}
-    for label, make, steps in (("lj_fcc_2048000", lambda: W.lj_c1(thermostat=0), c1_steps),
-                               ("charged_melt_512000", lambda: W.charged_melt(80, thermostat=0), c2_steps)):
+    for label, make, steps in (("lj_fcc_2048000", lambda: W.lj_fcc_2048000(thermostat=0), lj_steps),
+                               ("charged_melt_512000", lambda: W.charged_melt(80, thermostat=0), melt_steps)):
         if steps <= 0:
             continue
         out[label] = {"steps": steps, "record_every": every}
@@ -329,7 +329,7 @@ def cpu_baseline(W, nsteps):
     except AttributeError:
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
-    w = W.lj_c1(thermostat=1)
+    w = W.lj_fcc_2048000(thermostat=1)
     n = w.n
     st = dict(x=w.x.copy(), y=w.y.copy(), z=w.z.copy(), vx=w.vx.copy(), vy=w.vy.copy(), vz=w.vz.copy(),
               fx=np.zeros(n), fy=np.zeros(n), fz=np.zeros(n), mass=w.mass.copy())
@@ -365,7 +365,7 @@ def main():
                     help="also run a plain 1 GiB device copy (the reference for byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one MI355X; multi-GPU runs are not part of it "
+        print("bench.py: measures one MI355X; multi-GPU runs are not part of it "
               "(the single-GPU throughput is the reported value).", file=sys.stderr)
         sys.exit(2)
 

@@ -73,7 +73,7 @@ def test_pair_set_bit_exact_small(hip_ctx, orc, case):
 def test_pair_set_and_forces_c0_snapshot(hip_ctx, orc):
     """4000-atom FCC fluid: lattice pair set, then 300 GPU steps (FP64), then
     the pair set and FP64 forces/energy/virial on the GPU's own positions."""
-    w = W.lj_c0()
+    w = W.lj_fcc_4000()
     s = make_system(hip_ctx, w)
     s.nlist_build()
     np.testing.assert_array_equal(gpu_pairs(s), oracle_pairs(orc, w.x, w.y, w.z, w.box, 2.8))
@@ -162,7 +162,7 @@ def _oracle_params(orc, w, nh):
 def test_trajectory_matches_oracle_short_horizon(hip_ctx, orc, nh):
     """40 steps of the full loop (VV [+NH], decide/rebuild, LJ) from identical
     inputs: positions within 1e-9 Å, velocities, energies and xi to rounding."""
-    w = W.lj_c0()
+    w = W.lj_fcc_4000()
     w.thermostat = nh
     s = make_system(hip_ctx, w)
     nsteps = 40
@@ -188,7 +188,7 @@ def test_nve_energy_gates_c0(hip_ctx):
     """The reference LJ regression gates on the 4000-atom NVE config, 1000 steps,
     both precisions: max|E-E0|/(N eps) <= 1e-2; late drift <= 4x early drift."""
     for prec in (av.PREC_FP64, av.PREC_MIXED):
-        w = W.lj_c0()
+        w = W.lj_fcc_4000()
         s = make_system(hip_ctx, w)
         s.run(1000, prec, energy_every=1)
         pe, ke = s.thermo(1000)
@@ -218,7 +218,7 @@ def test_decide_cadence_delay_ten_check_no(hip_ctx):
 def test_deterministic_bitwise(hip_ctx):
     outs = []
     for _ in range(2):
-        w = W.lj_c0()
+        w = W.lj_fcc_4000()
         w.thermostat = 1
         s = make_system(hip_ctx, w)
         s.run(120, av.PREC_MIXED, energy_every=10)

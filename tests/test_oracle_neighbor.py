@@ -20,7 +20,7 @@ def c0_geometry(orc):
 
 
 def test_grid_and_stencil_closed_forms(orc):
-    # C0: L = 16.79596, rc+skin = 2.8 -> 11 bins of 1.52691, reach 2; every
+    # 4000-atom FCC: L = 16.79596, rc+skin = 2.8 -> 11 bins of 1.52691, reach 2; every
     # offset of the 5^3 block has min cell distance^2 = 3*cs^2 = 6.99 < 7.84.
     a = (4 / 0.8442) ** (1 / 3)
     nc, cs, reach, nhalf, nfull = orc.cell_grid(np.full(3, 10 * a), 2.8)
@@ -28,7 +28,7 @@ def test_grid_and_stencil_closed_forms(orc):
     assert np.allclose(cs, 10 * a / 11, rtol=0, atol=1e-15)
     assert list(reach) == [2, 2, 2]
     assert nfull == 125 and nhalf == 63
-    # C1: 80 cells of FCC -> 95 bins per axis (floor(134.3677/1.4))
+    # 2,048,000-atom FCC: 80 cells -> 95 bins per axis (floor(134.3677/1.4))
     nc, cs, reach, nhalf, nfull = orc.cell_grid(np.full(3, 80 * a), 2.8)
     assert list(nc) == [95, 95, 95] and list(reach) == [2, 2, 2]
     # jittered lattice fixture: L = 7.6, rc+skin = 3.0 -> 5 bins (the half path's minimum)
