@@ -396,12 +396,11 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
         const int ng = (nn + kGroup - 1) / kGroup;
         float fxi = 0.f, fyi = 0.f, fzi = 0.f;
         // group indices clamp to the last group (to group 0 when nn == 0: the
-        // brick's rows always exist); loads run two groups ahead
+        // brick's rows always exist); loads run two groups ahead in three
+        // rotating slots, so no group is moved between registers
         const int gl = max(ng - 1, 0);
-        uint4 cur = grp[__umul24(min(h, gl), nat)];
-        uint4 nx1 = grp[__umul24(min(h + tpa, gl), nat)];
-        for (int g = h; g < ng; g += tpa) {
-            const uint4 nx2 = grp[__umul24(min(g + 2 * tpa, gl), nat)];
+        auto load = [&](int g) { return grp[__umul24(min(g, gl), nat)]; };
+        auto group = [&](const uint4 cur) {
             // the whole group's LDS reads issue before any arithmetic; padding
             // rows (the atom itself, r^2 == 0) and pairs beyond r_c add zero
             const unsigned w[4] = {cur.x, cur.y, cur.z, cur.w};
@@ -447,8 +446,24 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
                     acc[6] += static_cast<double>(dy * fzp);
                 }
             }
-            cur = nx1;
-            nx1 = nx2;
+        };
+        uint4 s0 = load(h), s1 = load(h + tpa), s2;
+        for (int g = h;;) {
+            if (g >= ng)
+                break;
+            s2 = load(g + 2 * tpa);
+            group(s0);
+            g += tpa;
+            if (g >= ng)
+                break;
+            s0 = load(g + 2 * tpa);
+            group(s1);
+            g += tpa;
+            if (g >= ng)
+                break;
+            s1 = load(g + 2 * tpa);
+            group(s2);
+            g += tpa;
         }
         if (tpa == 2) {
             fxi += __shfl_xor(fxi, 1, 64);
