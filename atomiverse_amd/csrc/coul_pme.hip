@@ -662,7 +662,7 @@ __device__ __forceinline__ int wave_max_i(int v) {
 }
 
 constexpr int kSpreadChunk = kBlock; // atoms per workgroup without brick ranges
-constexpr int kSpreadCap = 6144;     // LDS grid points per workgroup (48 KiB of int64)
+constexpr int kSpreadCap = 6144;     // most LDS grid points per workgroup (48 KiB of int64)
 constexpr int kSpreadGridCap = 8192; // workgroups of one spread launch
 
 // The box of grid points a range of atoms touches. Base indices are taken as
@@ -748,14 +748,16 @@ __device__ __forceinline__ int wrap_any(int g, int K) {
 // order, so a brick's atoms (or a chunk of consecutive slots) touch a small
 // box of grid points: the workgroup accumulates all its contributions to that
 // box in LDS with 64-bit integer atomics and adds every touched point to the
-// global grid once. Ranges whose box exceeds kSpreadCap points go straight
-// to the global grid. Integer addition makes both paths give the same grid
+// global grid once. The LDS box holds `cap` points (dynamic shared memory
+// sized by the host from the brick extent, so small bricks leave room for
+// more resident workgroups); ranges whose box exceeds it go straight to the
+// global grid. Integer addition makes both paths give the same grid
 // bit for bit. Work items are (atom, first-axis offset p) pairs so a range of
 // up to n * count items spreads evenly over the workgroup.
 template <class T, int N>
 __global__ __launch_bounds__(kBlock) void k_pme_spread_lds(PmeArgs a, const int2* __restrict__ ranges,
-                                                           int nranges) {
-    __shared__ unsigned long long acc[kSpreadCap];
+                                                           int nranges, int cap) {
+    extern __shared__ unsigned long long acc[]; // cap points
     __shared__ int red[kBlock / 64][6];
     for (int t = blockIdx.x; t < nranges; t += gridDim.x) {
         const int r = xcd_brick(t, nranges);
@@ -763,7 +765,7 @@ __global__ __launch_bounds__(kBlock) void k_pme_spread_lds(PmeArgs a, const int2
         if (!range_of(a, ranges, r, rb))
             continue;
         range_box<N>(a, rb, red);
-        if (rb.total > kSpreadCap) {
+        if (rb.total > cap) {
             for (int t = threadIdx.x; t < rb.count; t += blockDim.x)
                 spread_atom_global<T, N>(a, rb.first + t);
             continue;
@@ -966,12 +968,28 @@ __global__ __launch_bounds__(kBlock) void k_zero3(int64_t n, double* a, double* 
 struct SpreadRanges {
     const int2* ranges;
     int n;
+    int cap; // LDS box points per workgroup
 };
+
+// Grid points of the box a brick's atoms touch: per axis the brick's extent
+// plus the skin (atoms move at most skin/2 either way between checked
+// rebuilds) in grid spacings, rounded up, plus the order's n points and one
+// for the floor of the lowest atom. A range whose box is larger at run time
+// (unchecked rebuilds) takes the global path, so the bound only sizes the LDS.
+int spread_box_cap(const atvh_system* s, const PmeState* p) {
+    int64_t tot = 1;
+    for (int d = 0; d < 3; ++d) {
+        const double span = (s->bdim[d] * s->cs[d] + s->skin) * p->K[d] * p->ic[d];
+        tot *= static_cast<int64_t>(std::ceil(span)) + p->order + 1;
+    }
+    return static_cast<int>(std::min<int64_t>(tot, kSpreadCap));
+}
 
 template <class T, int N>
 void launch_spread_gather(bool spread, hipStream_t st, const PmeArgs& a, const T* real, const SpreadRanges& sr) {
     if (spread)
-        k_pme_spread_lds<T, N><<<std::min(sr.n, kSpreadGridCap), kBlock, 0, st>>>(a, sr.ranges, sr.n);
+        k_pme_spread_lds<T, N><<<std::min(sr.n, kSpreadGridCap), kBlock, sr.cap * sizeof(unsigned long long), st>>>(
+            a, sr.ranges, sr.n, sr.cap);
     else
         k_pme_gather<T, N><<<grid_capped(a.n), kBlock, 0, st>>>(a, real);
 }
@@ -1217,9 +1235,11 @@ void pme_enqueue(atvh_system* s, int precision, bool ev) {
     if (s->brick_mode && s->nl_built && s->brick_range) {
         sr.ranges = s->brick_range;
         sr.n = s->nbricks;
+        sr.cap = spread_box_cap(s, p);
     } else {
         sr.ranges = nullptr;
         sr.n = static_cast<int>((s->n + kSpreadChunk - 1) / kSpreadChunk);
+        sr.cap = kSpreadCap;
     }
     const int gg = grid_capped(ntot);
     const int gm = std::min(grid_for(nmodes), kPmeGridCap);
