@@ -803,16 +803,38 @@ __global__ __launch_bounds__(kBlock) void k_pme_spread_lds(PmeArgs a, const int2
             }
         }
         __syncthreads();
-        for (int k = threadIdx.x; k < nbox; k += blockDim.x) {
+        // box point k = (i0 e1 + i1) e2 + i2 is walked with the block's stride:
+        // the indices advance by the stride's own digits (at most one carry
+        // each), so the walk divides once per range, not per point
+        const int st = static_cast<int>(blockDim.x);
+        const int s2 = st % e2, s1 = (st / e2) % e1, s0 = st / (e2 * e1);
+        int i2 = static_cast<int>(threadIdx.x) % e2;
+        int i1 = (static_cast<int>(threadIdx.x) / e2) % e1;
+        int i0 = static_cast<int>(threadIdx.x) / (e2 * e1);
+        const int w0 = wrap_any(rb.o[0], a.K0), w1 = wrap_any(rb.o[1], a.K1), w2 = wrap_any(rb.o[2], a.K2);
+        for (int k = threadIdx.x; k < nbox; k += st) {
             const unsigned long long v = acc[k];
-            if (v == 0ull)
-                continue;
-            const int i2 = k % e2, rest = k / e2;
-            const int i1 = rest % e1, i0 = rest / e1;
-            const int g0 = wrap_any(rb.o[0] + i0, a.K0);
-            const int g1 = wrap_any(rb.o[1] + i1, a.K1);
-            const int g2 = wrap_any(rb.o[2] + i2, a.K2);
-            atomicAdd(a.fix + (int64_t)a.K2 * (g1 + (int64_t)a.K1 * g0) + g2, v);
+            if (v != 0ull) {
+                int g0 = w0 + i0, g1 = w1 + i1, g2 = w2 + i2;
+                while (g0 >= a.K0)
+                    g0 -= a.K0;
+                while (g1 >= a.K1)
+                    g1 -= a.K1;
+                while (g2 >= a.K2)
+                    g2 -= a.K2;
+                atomicAdd(a.fix + (int64_t)a.K2 * (g1 + (int64_t)a.K1 * g0) + g2, v);
+            }
+            i2 += s2;
+            if (i2 >= e2) {
+                i2 -= e2;
+                ++i1;
+            }
+            i1 += s1;
+            if (i1 >= e1) {
+                i1 -= e1;
+                ++i0;
+            }
+            i0 += s0;
         }
         __syncthreads(); // acc[] is cleared for the next range
     }
