@@ -161,7 +161,7 @@ struct atvh_system {
     // k-space (KSpacePME), evaluated after the pair style in the force phase
     atvh::PmeState* pme = nullptr;
     // charge sums of the uploaded frame (ascending atom order)
-    double q_sum = 0, q_sq = 0, q_abs = 0;
+    double q_sum = 0, q_sq = 0, q_abs = 0, q_max = 0;
     double lj_eps = 1, lj_sigma = 1, lj_cutoff = 0;
     int lj_ntypes = 0;
     std::vector<double> lj_eps_tab, lj_sigma_tab;

@@ -601,10 +601,19 @@ __device__ __forceinline__ int axis_spline(double xv, double ic, int K, T (&th)[
 }
 
 // Fixed-point value of one weight: the same rounding on every path, so the
-// grid does not depend on which path (or order) added it.
+// grid does not depend on which path (or order) added it. The scale keeps
+// |w| scale below 2^50, so w scale + 1.5 2^52 lies in the binade whose
+// doubles are the integers: the FMA's one rounding gives
+// rint(w scale) + 1.5 2^52 (ties to even, the offset being even), and
+// subtracting the offset's bit pattern leaves rint(w scale) in two's
+// complement (three instructions instead of rint and a double -> int64
+// conversion).
 template <class T>
 __device__ __forceinline__ unsigned long long fix_of(T w, double scale) {
-    return static_cast<unsigned long long>(static_cast<long long>(rint(static_cast<double>(w) * scale)));
+    constexpr double kRound = 0x1.8p52;
+    const double d = __builtin_fma(static_cast<double>(w), scale, kRound);
+    return static_cast<unsigned long long>(__double_as_longlong(d)) -
+           static_cast<unsigned long long>(__double_as_longlong(kRound));
 }
 
 // All n^3 contributions of atom i added straight to the global grid.
@@ -1147,9 +1156,14 @@ PmeArgs make_pme_args(atvh_system* s) {
     a.ic0 = p->ic[0];
     a.ic1 = p->ic[1];
     a.ic2 = p->ic[2];
-    // fixed-point scale: a grid point holds at most sum|q_i| (weights <= 1)
+    // fixed-point scale: a grid point holds at most sum|q_i| (weights <= 1),
+    // kept below 2^61; one weight at most max|q_i|, kept below 2^50 (the
+    // range of the spread's single-FMA rounding)
     const double qabs = std::max(s->q_abs, 1.0);
-    p->fix_scale = std::ldexp(1.0, 61 - static_cast<int>(std::ceil(std::log2(qabs))));
+    int fix_exp = 61 - static_cast<int>(std::ceil(std::log2(qabs)));
+    if (s->q_max > 0.0)
+        fix_exp = std::min(fix_exp, 50 - static_cast<int>(std::ceil(std::log2(s->q_max))));
+    p->fix_scale = std::ldexp(1.0, fix_exp);
     a.fix_scale = p->fix_scale;
     a.inv_fix = 1.0 / p->fix_scale;
     a.fix = p->fix;

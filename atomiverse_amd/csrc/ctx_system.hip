@@ -375,12 +375,13 @@ int atvh_frame_upload(atvh_system* s, const double* x, const double* y, const do
     s->has_charge = charge != nullptr;
     // charge sums in ascending atom order (KSpacePME self / net-charge terms,
     // pme.cpp:222-226) and the bound of any spread grid point
-    s->q_sum = s->q_sq = s->q_abs = 0.0;
+    s->q_sum = s->q_sq = s->q_abs = s->q_max = 0.0;
     if (charge)
         for (int64_t i = 0; i < n; ++i) {
             s->q_sq += charge[i] * charge[i];
             s->q_sum += charge[i];
             s->q_abs += std::fabs(charge[i]);
+            s->q_max = std::max(s->q_max, std::fabs(charge[i]));
         }
     if (type)
         ATVH_HIP(hipMemcpyAsync(s->type, type, n * sizeof(int), hipMemcpyHostToDevice, st));
