@@ -1450,18 +1450,24 @@ BrickBuildArgs nl_brick_args(atvh_system* s, const int* pred) {
 
 // Launch grids of the rebuild stages for one system (the batched rebuild
 // gives every system the same block counts).
+// Every stage strides over its units, so the grids are capped at
+// kStageGridCap blocks (a multiple of the 8 XCDs, which keeps the XCD-aware
+// brick order): on the steps without a rebuild each stage is launched and
+// leaves at its first instruction, and a smaller grid costs less to drain.
+constexpr int kStageGridCap = 2048;
+
 void nl_stage_grids(const atvh_system* s, int g[kNlStages]) {
-    const int ga = grid_capped(s->n);
-    g[0] = ga;                            // bin
-    g[1] = s->nscan_blocks;               // scan tiles
-    g[2] = 1;                             // scan sums
-    g[3] = s->nscan_blocks;               // scan add
-    g[4] = ga;                            // scatter
-    g[5] = grid_capped(s->ncells);        // in-cell order
-    g[6] = ga;                            // permute
-    g[7] = ga;                            // copy back
-    g[8] = std::min(s->nbricks, 4096);    // halo rows
-    g[9] = s->nbricks;                    // pair build
+    const int ga = std::min(grid_capped(s->n), kStageGridCap);
+    g[0] = ga;                                                   // bin
+    g[1] = s->nscan_blocks;                                      // scan tiles
+    g[2] = 1;                                                    // scan sums
+    g[3] = s->nscan_blocks;                                      // scan add
+    g[4] = ga;                                                   // scatter
+    g[5] = std::min(grid_capped(s->ncells), kStageGridCap);      // in-cell order
+    g[6] = ga;                                                   // permute
+    g[7] = ga;                                                   // copy back
+    g[8] = std::min(s->nbricks, kStageGridCap);                  // halo rows
+    g[9] = std::min(s->nbricks, kStageGridCap);                  // pair build
 }
 
 void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
