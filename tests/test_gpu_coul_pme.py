@@ -125,6 +125,34 @@ def test_pme_is_bitwise_reproducible(hip_ctx):
             assert e == outs[0][1]
 
 
+@pytest.mark.parametrize("nside", [12, 24])
+def test_pme_spread_groupings_agree_bitwise(hip_ctx, nside):
+    """Each spread workgroup sums its atoms' weights in an LDS box of integer
+    points and adds the box to the integer grid. Before any list build a
+    workgroup takes 256 consecutive slots (the largest box, 6144 points);
+    after the build it takes one brick, with a box sized from the brick's
+    extent plus the skin. Integer sums do not depend on the grouping, so the
+    two give the same forces and energies bit for bit, in both precisions."""
+    w = snapshot(nside, seed=5)
+    s = make(hip_ctx, w)
+    before = {}
+    for prec in (av.PREC_FP64, av.PREC_MIXED):
+        s.pme_compute(prec, True, True)
+        before[prec] = (forces(s.download()), s.energy()[0], s.pme_info()[1])
+    s.nlist_build()
+    cand, bdim = s.brick_shape()
+    assert cand >= 0, "the brick layout is the case under test"
+    d = s.download()
+    for k in ("x", "y", "z"):  # the build's wrap leaves wrapped positions as they are
+        np.testing.assert_array_equal(d[k], getattr(w, k))
+    for prec in (av.PREC_FP64, av.PREC_MIXED):
+        s.pme_compute(prec, True, True)
+        F, e, parts = forces(s.download()), s.energy()[0], s.pme_info()[1]
+        np.testing.assert_array_equal(F, before[prec][0])
+        assert e == before[prec][1]
+        assert parts == before[prec][2]
+
+
 def test_madelung_nacl_on_device(hip_ctx):
     """test_pme_coul.cpp:157-175 with the device pair style (LJ off) + PME."""
     d, n = 2.82, 4
