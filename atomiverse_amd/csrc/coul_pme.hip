@@ -912,10 +912,11 @@ __global__ __launch_bounds__(kBlock) void k_pme_influence(PmeArgs a, T2* __restr
         block_partials(acc, a.partials, blockIdx.x);
 }
 
-// Force on atom i from the potential grid, read through at(g0, g1, g2) with
-// the base indices b (grid index of weight 0 on each axis, before wrapping).
-template <class T, int N, class At>
-__device__ __forceinline__ void gather_atom(const PmeArgs& a, int64_t i, At at) {
+// Force on atom i from the potential grid, read one row of n points along
+// the last axis at a time through row(b0, b1, b2, p, r, v) with the base
+// indices b (grid index of weight 0 on each axis, before wrapping).
+template <class T, int N, class Row>
+__device__ __forceinline__ void gather_atom(const PmeArgs& a, int64_t i, Row row) {
     const double qi = a.q[i];
     if (qi == 0.0)
         return;
@@ -928,9 +929,11 @@ __device__ __forceinline__ void gather_atom(const PmeArgs& a, int64_t i, At at) 
     for (int p = 0; p < N; ++p) {
 #pragma unroll
         for (int r = 0; r < N; ++r) {
+            T v[N];
+            row(b0, b1, b2, p, r, v);
 #pragma unroll
             for (int c = 0; c < N; ++c) {
-                const T cval = at(b0, b1, b2, p, r, c);
+                const T cval = v[c];
                 gx += cval * dx[p] * ty[r] * tz[c];
                 gy += cval * tx[p] * dy[r] * tz[c];
                 gz += cval * tx[p] * ty[r] * dz[c];
@@ -946,11 +949,16 @@ __device__ __forceinline__ void gather_atom(const PmeArgs& a, int64_t i, At at) 
     a.fz[i] += pre * fzi;
 }
 
+// The row's first two indices are wrapped once per row, its last index per
+// point (the row may cross the periodic edge).
 template <class T, int N>
 __device__ __forceinline__ void gather_atom_global(const PmeArgs& a, int64_t i, const T* __restrict__ real) {
-    gather_atom<T, N>(a, i, [&](int b0, int b1, int b2, int p, int r, int c) {
-        const int g0 = pme_wrap(b0 + p, a.K0), g1 = pme_wrap(b1 + r, a.K1), g2 = pme_wrap(b2 + c, a.K2);
-        return real[(int64_t)a.K2 * (g1 + (int64_t)a.K1 * g0) + g2];
+    gather_atom<T, N>(a, i, [&](int b0, int b1, int b2, int p, int r, T (&v)[N]) {
+        const int g0 = pme_wrap(b0 + p, a.K0), g1 = pme_wrap(b1 + r, a.K1);
+        const T* __restrict__ rp = real + (int64_t)a.K2 * (g1 + (int64_t)a.K1 * g0);
+#pragma unroll
+        for (int c = 0; c < N; ++c)
+            v[c] = rp[pme_wrap(b2 + c, a.K2)];
     });
 }
 
