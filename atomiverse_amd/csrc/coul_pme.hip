@@ -927,13 +927,15 @@ __device__ __forceinline__ void gather_atom(const PmeArgs& a, int64_t i, Row row
     T gx = T(0), gy = T(0), gz = T(0);
 #pragma unroll
     for (int p = 0; p < N; ++p) {
+        T v[N][N]; // the plane's n rows, all read before its arithmetic
+#pragma unroll
+        for (int r = 0; r < N; ++r)
+            row(b0, b1, b2, p, r, v[r]);
 #pragma unroll
         for (int r = 0; r < N; ++r) {
-            T v[N];
-            row(b0, b1, b2, p, r, v);
 #pragma unroll
             for (int c = 0; c < N; ++c) {
-                const T cval = v[c];
+                const T cval = v[r][c];
                 gx += cval * dx[p] * ty[r] * tz[c];
                 gy += cval * tx[p] * dy[r] * tz[c];
                 gz += cval * tx[p] * ty[r] * dz[c];
