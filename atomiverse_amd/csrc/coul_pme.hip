@@ -205,6 +205,22 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
                 jq[u] = sq[q];
                 jt[u] = stype[q];
             }
+            // every pair's table row is read before any pair's arithmetic, so
+            // the group's eight table loads overlap (a pair beyond the cutoff
+            // reads the row of r_c and discards it; within it fmin is r^2)
+            double4 evs[kGroup];
+            double tts[kGroup];
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) {
+                const double dx = xi - jx[u];
+                const double dy = yi - jy[u];
+                const double dz = zi - jz[u];
+                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                const double xb = fmin(r2, rc_sq) * inv_dr2;
+                const int bi = min(static_cast<int>(xb), nbin1);
+                tts[u] = xb - static_cast<double>(bi);
+                evs[u] = etab[bi];
+            }
 #pragma unroll
             for (int u = 0; u < kGroup; ++u) {
                 const double dx = xi - jx[u];
@@ -215,10 +231,8 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
                     const double* __restrict__ t = trow + jt[u] * 8;
                     const double r2inv = 1.0 / r2;
                     const double r6inv = r2inv * r2inv * r2inv;
-                    const double xb = r2 * inv_dr2;
-                    const int bi = min(static_cast<int>(xb), nbin1);
-                    const double tt = xb - static_cast<double>(bi);
-                    const double4 ev = etab[bi];
+                    const double tt = tts[u];
+                    const double4 ev = evs[u];
                     const double qq = qi_scale * jq[u];
                     const double flj = r2inv * fma(t[0] * r6inv, r6inv, -(t[1] * r6inv));
                     const double fpair = fma(qq, fma(tt, ev.y, ev.x), flj);
