@@ -49,6 +49,7 @@ SIGNATURES = {
     "atvh_nlist_brick_shape": (C.c_int, [_vp, _ip, _ip]),
     "atvh_nlist_set_brick_shape": (C.c_int, [_vp, C.c_int]),
     "atvh_nlist_autotune": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _dp]),
+    "atvh_nlist_set_capacity": (C.c_int, [_vp, C.c_int]),
     "atvh_nlist_stats": (C.c_int, [_vp, _lp, _lp, _ip, _ip]),
     "atvh_nlist_export_pairs": (C.c_int, [_vp, _lp, C.c_int64, _lp]),
     "atvh_pair_lj_configure": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp]),

@@ -60,7 +60,14 @@ struct DevCtl {
     double nh_ndof_kT;   // Ndof kB T
     double T_target;     // thermostat temperature (moves with replica exchanges)
     double stress_kin[6]; // last kinetic stress sum m v_a v_b (eV, Voigt xx yy zz xy xz yz)
+    int steps_done;       // steps of the current run completed (advanced by the end-of-step kernel)
 };
+
+// A rebuild that overflowed the neighbor capacity or a brick halo leaves the
+// system halted: the integration kernels of every later phase leave at
+// their first instruction, so no step integrates forces from a truncated
+// list. The host regrows the list at the halted step and resumes there.
+__device__ __forceinline__ bool halted(const DevCtl* c) { return (c->overflow | c->halo_overflow) != 0; }
 
 struct CoulState; // PairLJCutCoulLong tables (coul_pme.hip)
 struct PmeState;  // KSpacePME mesh, plans and tables (coul_pme.hip)
@@ -257,6 +264,10 @@ void nl_enqueue_decide(atvh_system* s);
 void nl_free(atvh_system* s);
 void nl_realloc_capacity(atvh_system* s, int cap);
 void nl_check_ctl(const DevCtl& h);
+// List build at the current positions, growing the capacity or stepping to a
+// smaller brick shape until it fits. keep_counts: a regrow inside a run (the
+// rebuild counter keeps the rebuild the device already counted).
+void nl_build(atvh_system* s, bool keep_counts);
 
 // pair_lj.hip
 int pair_grid(const atvh_system* s);
@@ -287,6 +298,12 @@ void pair_batch_launch(atvh_batch* b, int precision, bool ef);
 
 // integrate.hip
 void integ_enqueue_initial(atvh_system* s);
+// Queue steps [first, nsteps) of a fused run; resume: the first step starts
+// at its force phase (its initial integrate, decide and rebuild already ran).
+void md_enqueue_steps(atvh_system* s, int first, int nsteps, int precision, int energy_every, bool resume);
+// Regrow the list of a system halted by an overflow and finish its run alone
+// (bit-identical to the batched step); returns after the stream drains.
+void md_recover_and_finish(atvh_system* s, int nsteps, int precision, int energy_every);
 void integ_enqueue_final(atvh_system* s);
 void energy_enqueue_finalize(atvh_system* s, double* record_pe, int step, bool with_pair = true);
 

@@ -25,8 +25,10 @@ namespace {
 
 __global__ void k_batch_reset(DevCtl** ctls, int nsys) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nsys)
+    if (k < nsys) {
         ctls[k]->rec_step = 0;
+        ctls[k]->steps_done = 0;
+    }
 }
 
 __device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
@@ -274,13 +276,12 @@ int atvh_batch_run(atvh_batch* b, int nsteps, int precision, int energy_every) {
             enqueue_step(b, precision, ef);
         }
     }
-    // capacity and halo overflows of any member surface here
+    // a member whose rebuild overflowed its list capacity or a brick halo
+    // halted at that step; it regrows its list and finishes the run alone
+    // (bit-identical to the batched step), the others are done
     ATVH_HIP(hipStreamSynchronize(st));
-    for (atvh_system* s : b->sys) {
-        DevCtl h{};
-        ATVH_HIP(hipMemcpy(&h, s->ctl, sizeof(h), hipMemcpyDeviceToHost));
-        nl_check_ctl(h);
-    }
+    for (atvh_system* s : b->sys)
+        md_recover_and_finish(s, nsteps, precision, energy_every);
     return ATVH_OK;
     API_CATCH(c)
 }

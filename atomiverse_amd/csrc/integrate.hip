@@ -75,6 +75,8 @@ __device__ __forceinline__ void block_sum_to(double v, double* out) {
 struct OpInitial {
     using Args = IntArgs;
     static __device__ void run(const IntArgs& a, int bid, int nbk) {
+        if (halted(a.ctl))
+            return;
         const double s = a.thermostat ? a.ctl->nh_scale : 1.0;
         double mv2 = 0.0;
         int moved = 0;
@@ -120,6 +122,8 @@ struct OpInitial {
 struct OpFinal {
     using Args = IntArgs;
     static __device__ void run(const IntArgs& a, int bid, int nbk) {
+        if (halted(a.ctl))
+            return;
         const double s = a.thermostat ? a.ctl->nh_scale : 1.0;
         double mv2 = 0.0;
         for (int64_t i = bid * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)nbk * blockDim.x) {
@@ -202,6 +206,12 @@ struct PostArgs {
 struct OpPost {
     using Args = PostArgs;
     static __device__ void run(const PostArgs& a, int, int) {
+        if (halted(a.ctl)) {
+            // nothing of this step is integrated, and no rebuild is decided
+            if (threadIdx.x == 0 && a.decide)
+                a.ctl->rebuild = 0;
+            return;
+        }
         const double mv2 = block_fixed_sum(a.ke_partials, a.nb, 1);
         if (threadIdx.x != 0)
             return;
@@ -221,6 +231,7 @@ struct OpPost {
                 a.record_ke[step] = c->ke;
             if (a.step < 0)
                 c->rec_step = step + 1;
+            c->steps_done += 1;
         } else {
             c->ke_half = ke;
         }
@@ -302,6 +313,12 @@ struct OpFinalize {
         }
     }
 };
+
+// Start of a fused run: no step of it completed yet.
+__global__ void k_run_reset(DevCtl* ctl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        ctl->steps_done = 0;
+}
 
 IntArgs make_int_args(atvh_system* s) {
     IntArgs a;
@@ -434,6 +451,62 @@ void integ_batch_launch(atvh_batch* b, const char* stage) {
     ATVH_HIP(hipGetLastError());
 }
 
+void md_enqueue_steps(atvh_system* s, int first, int nsteps, int precision, int energy_every, bool resume) {
+    atvh_ctx* c = s->ctx;
+    hipStream_t st = c->stream;
+    double* rec_pe = s->d_thermo;
+    double* rec_ke = s->d_thermo + s->thermo_cap;
+    const IntArgs ia = make_int_args(s);
+    PostArgs p_init = make_post_args(s, false, true);
+    PostArgs p_fin = make_post_args(s, true, false);
+    p_fin.record_ke = rec_ke;
+    const int g = integ_grid(s);
+    for (int step = first; step < nsteps; ++step) {
+        if (!(resume && step == first)) {
+            {
+                ScopedTimer tm(c, "integrate");
+                k_single<OpInitial><<<g, kBlock, 0, st>>>(ia);
+            }
+            k_single<OpPost><<<1, kBlock, 0, st>>>(p_init);
+            nl_enqueue_rebuild(s, &s->ctl->rebuild, false);
+        }
+        const bool ef = (step == nsteps - 1) || (energy_every > 0 && (step % energy_every) == 0);
+        pair_enqueue(s, precision, ef, ef);
+        if (ef)
+            energy_enqueue_finalize(s, rec_pe, step);
+        {
+            ScopedTimer tm(c, "integrate");
+            k_single<OpFinal><<<g, kBlock, 0, st>>>(ia);
+        }
+        p_fin.step = step;
+        k_single<OpPost><<<1, kBlock, 0, st>>>(p_fin);
+        ATVH_HIP(hipGetLastError());
+    }
+}
+
+// A rebuild that overflowed halted the system at the start of that step's
+// force phase (every later kernel of the run left at once). The list is
+// regrown at the positions the step reached (the reference regrows inside
+// the rebuild, src/cuda/neighbor/neighbor_list.cu:544-553) and the run
+// resumes at that force phase, so the trajectory equals an uninterrupted
+// run with a larger capacity.
+void md_recover_and_finish(atvh_system* s, int nsteps, int precision, int energy_every) {
+    hipStream_t st = s->ctx->stream;
+    for (int attempt = 0;; ++attempt) {
+        DevCtl h{};
+        ATVH_HIP(hipMemcpy(&h, s->ctl, sizeof(h), hipMemcpyDeviceToHost));
+        if (!(h.overflow || h.halo_overflow))
+            return;
+        if (attempt >= 8 || h.steps_done >= nsteps) {
+            s->nl_built = false;
+            nl_check_ctl(h);
+        }
+        nl_build(s, true);
+        md_enqueue_steps(s, h.steps_done, nsteps, precision, energy_every, true);
+        ATVH_HIP(hipStreamSynchronize(st));
+    }
+}
+
 void energy_enqueue_finalize(atvh_system* s, double* record_pe, int step, bool with_pair) {
     FinalizeArgs a;
     a.ctl = s->ctl;
@@ -557,41 +630,10 @@ int atvh_md_run(atvh_system* s, int nsteps, int precision, int energy_every) {
                                 hipMemcpyHostToDevice, st));
         ATVH_HIP(hipStreamSynchronize(st));
     }
-    double* rec_pe = s->d_thermo;
-    double* rec_ke = s->d_thermo + s->thermo_cap;
-    const IntArgs ia = make_int_args(s);
-    PostArgs p_init = make_post_args(s, false, true);
-    PostArgs p_fin = make_post_args(s, true, false);
-    p_fin.record_ke = rec_ke;
-    const int g = std::min(grid_for(s->n), kIntGrid);
-    for (int step = 0; step < nsteps; ++step) {
-        {
-            ScopedTimer tm(c, "integrate");
-            k_single<OpInitial><<<g, kBlock, 0, st>>>(ia);
-        }
-        k_single<OpPost><<<1, kBlock, 0, st>>>(p_init);
-        nl_enqueue_rebuild(s, &s->ctl->rebuild, false);
-        const bool ef = (step == nsteps - 1) || (energy_every > 0 && (step % energy_every) == 0);
-        pair_enqueue(s, precision, ef, ef);
-        if (ef)
-            energy_enqueue_finalize(s, rec_pe, step);
-        {
-            ScopedTimer tm(c, "integrate");
-            k_single<OpFinal><<<g, kBlock, 0, st>>>(ia);
-        }
-        p_fin.step = step;
-        k_single<OpPost><<<1, kBlock, 0, st>>>(p_fin);
-        ATVH_HIP(hipGetLastError());
-    }
-    // a neighbor capacity or halo overflow during the run means skipped work:
-    // report it (the list build that follows a failed call resizes)
+    k_run_reset<<<1, 64, 0, st>>>(s->ctl);
+    md_enqueue_steps(s, 0, nsteps, precision, energy_every, false);
     ATVH_HIP(hipStreamSynchronize(st));
-    DevCtl h{};
-    ATVH_HIP(hipMemcpy(&h, s->ctl, sizeof(h), hipMemcpyDeviceToHost));
-    if (h.overflow || h.halo_overflow) {
-        s->nl_built = false;
-        nl_check_ctl(h);
-    }
+    md_recover_and_finish(s, nsteps, precision, energy_every);
     return ATVH_OK;
     API_CATCH(c)
 }

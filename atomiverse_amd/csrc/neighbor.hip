@@ -1610,6 +1610,65 @@ void nl_check_ctl(const DevCtl& h) {
         throw Error(ATVH_E_CAPACITY, "neighbor capacity exceeded during a rebuild");
 }
 
+// Fullest halo and brick of the last build (per-brick counts read back; a
+// list build only).
+static void brick_fill(atvh_system* s, int& max_halo, int& max_atoms) {
+    std::vector<int> hn(s->nbricks);
+    std::vector<int2> br(s->nbricks);
+    ATVH_HIP(hipMemcpy(hn.data(), s->halo_n, hn.size() * sizeof(int), hipMemcpyDeviceToHost));
+    ATVH_HIP(hipMemcpy(br.data(), s->brick_range, br.size() * sizeof(int2), hipMemcpyDeviceToHost));
+    max_halo = 0;
+    max_atoms = 0;
+    for (int b = 0; b < s->nbricks; ++b) {
+        max_halo = std::max(max_halo, hn[b]);
+        max_atoms = std::max(max_atoms, br[b].y);
+    }
+}
+
+void nl_build(atvh_system* s, bool keep_counts) {
+    atvh_ctx* c = s->ctx;
+    for (int attempt = 0; attempt < 12; ++attempt) {
+        k_nl_stats_reset<<<1, 64, 0, c->stream>>>(s->ctl);
+        // the initial build also orders the frame's forces (first kick uses them)
+        nl_enqueue_rebuild(s, &s->ctl->always, true);
+        ATVH_HIP(hipStreamSynchronize(c->stream));
+        DevCtl h{};
+        ATVH_HIP(hipMemcpy(&h, s->ctl, sizeof(h), hipMemcpyDeviceToHost));
+        // a brick shape whose fullest halo or brick (measured on these
+        // positions) leaves less than a sixteenth of the LDS rows or of the
+        // brick's atom slots free is given up for the next smaller one:
+        // denser-than-uniform inputs keep some headroom for the rebuilds of
+        // the run
+        bool tight = false;
+        if (s->brick_mode && s->brick_pick < kLastBrickCand && h.halo_overflow == 0) {
+            int max_halo = 0, max_atoms = 0;
+            brick_fill(s, max_halo, max_atoms);
+            tight = max_halo > kHaloFillMax || max_atoms > kBrickAtomsFillMax;
+        }
+        if (h.overflow == 0 && h.halo_overflow == 0 && !tight) {
+            h.ago = 0;
+            if (!keep_counts)
+                h.nrebuilds = 1;
+            h.disp_flag = 0;
+            h.rebuild = 0;
+            ATVH_HIP(hipMemcpy(s->ctl, &h, sizeof(h), hipMemcpyHostToDevice));
+            s->nl_built = true;
+            return;
+        }
+        if (h.halo_overflow || tight) {
+            // a denser-than-uniform region: fall back to the next smaller brick
+            if (s->brick_pick >= kLastBrickCand)
+                break;
+            s->brick_pick_min = s->brick_pick + 1;
+            nl_setup_grid(s);
+            continue;
+        }
+        // grow and rebuild from the (now sorted) positions
+        nl_realloc_capacity(s, static_cast<int>(h.max_nbr * 1.25) + 16);
+    }
+    throw Error(ATVH_E_CAPACITY, "neighbor list build: the capacity could not be satisfied");
+}
+
 } // namespace atvh
 
 #define API_TRY try {
@@ -1653,21 +1712,6 @@ int atvh_nlist_configure(atvh_system* s, double cutoff, double skin, int every, 
     API_CATCH(c)
 }
 
-// Fullest halo and brick of the last build (per-brick counts read back; a
-// list build only).
-static void brick_fill(atvh_system* s, int& max_halo, int& max_atoms) {
-    std::vector<int> hn(s->nbricks);
-    std::vector<int2> br(s->nbricks);
-    ATVH_HIP(hipMemcpy(hn.data(), s->halo_n, hn.size() * sizeof(int), hipMemcpyDeviceToHost));
-    ATVH_HIP(hipMemcpy(br.data(), s->brick_range, br.size() * sizeof(int2), hipMemcpyDeviceToHost));
-    max_halo = 0;
-    max_atoms = 0;
-    for (int b = 0; b < s->nbricks; ++b) {
-        max_halo = std::max(max_halo, hn[b]);
-        max_atoms = std::max(max_atoms, br[b].y);
-    }
-}
-
 int atvh_nlist_build(atvh_system* s) {
     if (!s)
         return ATVH_E_ARG;
@@ -1676,45 +1720,25 @@ int atvh_nlist_build(atvh_system* s) {
     if (!s->nl_configured)
         throw Error(ATVH_E_STATE, "atvh_nlist_build: neighbor list not configured");
     ATVH_HIP(hipSetDevice(c->device));
-    for (int attempt = 0; attempt < 12; ++attempt) {
-        k_nl_stats_reset<<<1, 64, 0, c->stream>>>(s->ctl);
-        // the initial build also orders the frame's forces (first kick uses them)
-        nl_enqueue_rebuild(s, &s->ctl->always, true);
-        ATVH_HIP(hipStreamSynchronize(c->stream));
-        DevCtl h{};
-        ATVH_HIP(hipMemcpy(&h, s->ctl, sizeof(h), hipMemcpyDeviceToHost));
-        // a brick shape whose fullest halo or brick (measured on these
-        // positions) leaves less than a sixteenth of the LDS rows or of the
-        // brick's atom slots free is given up for the next smaller one:
-        // denser-than-uniform inputs keep some headroom for the rebuilds of
-        // the run
-        bool tight = false;
-        if (s->brick_mode && s->brick_pick < kLastBrickCand && h.halo_overflow == 0) {
-            int max_halo = 0, max_atoms = 0;
-            brick_fill(s, max_halo, max_atoms);
-            tight = max_halo > kHaloFillMax || max_atoms > kBrickAtomsFillMax;
-        }
-        if (h.overflow == 0 && h.halo_overflow == 0 && !tight) {
-            h.ago = 0;
-            h.nrebuilds = 1;
-            h.disp_flag = 0;
-            h.rebuild = 0;
-            ATVH_HIP(hipMemcpy(s->ctl, &h, sizeof(h), hipMemcpyHostToDevice));
-            s->nl_built = true;
-            return ATVH_OK;
-        }
-        if (h.halo_overflow || tight) {
-            // a denser-than-uniform region: fall back to the next smaller brick
-            if (s->brick_pick >= kLastBrickCand)
-                break;
-            s->brick_pick_min = s->brick_pick + 1;
-            nl_setup_grid(s);
-            continue;
-        }
-        // grow and rebuild from the (now sorted) positions
-        nl_realloc_capacity(s, static_cast<int>(h.max_nbr * 1.25) + 16);
-    }
-    throw Error(ATVH_E_CAPACITY, "atvh_nlist_build: neighbor capacity could not be satisfied");
+    nl_build(s, false);
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+int atvh_nlist_set_capacity(atvh_system* s, int capacity) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    if (!s->nl_configured)
+        throw Error(ATVH_E_STATE, "atvh_nlist_set_capacity: neighbor list not configured");
+    if (capacity < 1)
+        throw Error(ATVH_E_ARG, "atvh_nlist_set_capacity: capacity must be positive");
+    ATVH_HIP(hipSetDevice(c->device));
+    ATVH_HIP(hipStreamSynchronize(c->stream));
+    nl_realloc_capacity(s, capacity);
+    s->nl_built = false;
+    return ATVH_OK;
     API_CATCH(c)
 }
 
@@ -1847,6 +1871,14 @@ int atvh_nlist_decide_and_rebuild(atvh_system* s) {
         throw Error(ATVH_E_STATE, "atvh_nlist_decide_and_rebuild: call atvh_nlist_build first");
     nl_enqueue_decide(s);
     nl_enqueue_rebuild(s, &s->ctl->rebuild, false);
+    // the overflow probe is read back here, as the reference's list rebuild
+    // reads its pair count (src/cuda/neighbor/neighbor_list.cu:544-553): an
+    // overflowing rebuild is redone at once with a grown capacity
+    ATVH_HIP(hipStreamSynchronize(c->stream));
+    DevCtl h{};
+    ATVH_HIP(hipMemcpy(&h, s->ctl, sizeof(h), hipMemcpyDeviceToHost));
+    if (h.overflow || h.halo_overflow)
+        nl_build(s, true);
     return ATVH_OK;
     API_CATCH(c)
 }

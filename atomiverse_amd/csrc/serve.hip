@@ -42,6 +42,16 @@ struct Segment {
     // results
     std::vector<double> rx, ry, rz, rvx, rvy, rvz;
     double pe = 0, ke = 0;
+    // a segment whose admission or advance failed: the error it ended with
+    int fail_code = 0;
+    std::string fail_msg;
+    Segment() = default;
+    Segment(const Segment&) = delete;
+    Segment& operator=(const Segment&) = delete;
+    ~Segment() {
+        if (sys)
+            atvh_system_destroy(sys);
+    }
 };
 
 template <class T>
@@ -143,6 +153,21 @@ void retire(atvh_server* sv, Segment& sg) {
         std::vector<double>().swap(*v);
 }
 
+// A segment that failed is retired without results: its device memory is
+// released and poll reports it like a finished one; fetch returns its error.
+void fail_segment(atvh_server* sv, std::unique_ptr<Segment> sg, bool counted, int code, const std::string& msg) {
+    if (sg->sys) {
+        atvh_system_destroy(sg->sys);
+        sg->sys = nullptr;
+    }
+    if (counted)
+        sv->in_use -= sg->bytes;
+    sg->fail_code = code != ATVH_OK ? code : ATVH_E_STATE;
+    sg->fail_msg = msg;
+    sv->newly_done.push_back(sg->id);
+    sv->done.push_back(std::move(sg));
+}
+
 void drop_batch(atvh_server* sv) {
     if (sv->batch)
         atvh_batch_destroy(sv->batch);
@@ -197,9 +222,7 @@ int atvh_server_destroy(atvh_server* sv) {
     if (!sv)
         return ATVH_E_ARG;
     drop_batch(sv);
-    for (auto& r : sv->resident)
-        if (r->sys)
-            atvh_system_destroy(r->sys);
+    sv->resident.clear(); // each segment releases its device system
     sv->ctx->live_systems -= 1;
     delete sv;
     return ATVH_OK;
@@ -260,7 +283,15 @@ int atvh_server_advance(atvh_server* sv, int* active, int* queued) {
     while (!sv->queue.empty() && sv->in_use + sv->queue.front()->bytes <= sv->budget) {
         std::unique_ptr<Segment> sg = std::move(sv->queue.front());
         sv->queue.pop_front();
-        admit(sv, *sg);
+        try {
+            admit(sv, *sg);
+        } catch (const Error& e) {
+            fail_segment(sv, std::move(sg), false, e.code, e.what());
+            continue;
+        } catch (const std::exception& e) {
+            fail_segment(sv, std::move(sg), false, ATVH_E_ARG, e.what());
+            continue;
+        }
         sv->in_use += sg->bytes;
         sv->resident.push_back(std::move(sg));
         changed = true;
@@ -284,18 +315,39 @@ int atvh_server_advance(atvh_server* sv, int* active, int* queued) {
             }
         }
         if (step > 0) {
-            if (sv->batch)
-                check_rc(atvh_batch_run(sv->batch, static_cast<int>(step), sv->precision, 0), c, "server step");
+            // a run that fails ends its segments (marked failed, retired
+            // below); only the segments whose run succeeded count the chunk
+            if (sv->batch) {
+                const int rc = atvh_batch_run(sv->batch, static_cast<int>(step), sv->precision, 0);
+                if (rc != ATVH_OK) {
+                    for (auto& r : sv->resident)
+                        if (r->batched && r->left > 0) {
+                            r->fail_code = rc;
+                            r->fail_msg = std::string("server step: ") + c->err;
+                        }
+                    drop_batch(sv);
+                }
+            }
             for (auto& r : sv->resident)
-                if (!r->batched && r->left > 0)
-                    check_rc(atvh_md_run(r->sys, static_cast<int>(step), sv->precision, 0), c, "server step");
+                if (!r->batched && r->left > 0) {
+                    const int rc = atvh_md_run(r->sys, static_cast<int>(step), sv->precision, 0);
+                    if (rc != ATVH_OK) {
+                        r->fail_code = rc;
+                        r->fail_msg = std::string("server step: ") + c->err;
+                    }
+                }
             for (auto& r : sv->resident)
-                r->left -= step;
+                if (r->fail_code == ATVH_OK)
+                    r->left -= step;
         }
-        // retire finished segments (their device memory is released)
+        // retire finished and failed segments (their device memory is released)
         std::vector<std::unique_ptr<Segment>> keep;
         for (auto& r : sv->resident) {
-            if (r->left <= 0) {
+            if (r->fail_code != ATVH_OK) {
+                const int code = r->fail_code;
+                const std::string msg = r->fail_msg;
+                fail_segment(sv, std::move(r), true, code, msg);
+            } else if (r->left <= 0) {
                 retire(sv, *r);
                 sv->newly_done.push_back(r->id);
                 sv->done.push_back(std::move(r));
@@ -336,6 +388,12 @@ int atvh_server_fetch(atvh_server* sv, int64_t id, double* x, double* y, double*
     if (it == sv->done.end())
         throw Error(ATVH_E_STATE, "atvh_server_fetch: no finished segment with that id");
     Segment& sg = **it;
+    if (sg.fail_code != ATVH_OK) {
+        const int code = sg.fail_code;
+        const std::string msg = "segment " + std::to_string(id) + " failed: " + sg.fail_msg;
+        sv->done.erase(it);
+        throw Error(code, msg);
+    }
     auto put = [&](double* dst, const std::vector<double>& src) {
         if (dst)
             std::copy(src.begin(), src.end(), dst);

@@ -164,6 +164,10 @@ class System:
         self._ck(self.lib.atvh_nlist_autotune(self.h, precision, int(trials), C.byref(k), C.byref(us)))
         return k.value, us.value
 
+    def set_capacity(self, capacity):
+        """Per-atom list capacity for the next build (grown on overflow)."""
+        self._ck(self.lib.atvh_nlist_set_capacity(self.h, int(capacity)))
+
     def nlist_stats(self):
         nr = C.c_int64()
         npairs = C.c_int64()

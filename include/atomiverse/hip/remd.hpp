@@ -69,9 +69,14 @@ class Remd {
         const int n = static_cast<int>(sys_.size());
         for (int r = 0; r < n_exchanges; ++r) {
             check(atvh_batch_run(batch_, steps_per_exchange, static_cast<int>(prec_), 0), ctx_, "Remd::propagate");
+            // Remd::run alternates even and odd edges from r = 0 on every call
+            // (exchange_round_(r % 2 == 0)); the round index handed to the
+            // device is 2 * (rounds so far) + parity, so its parity is this
+            // call's and its uniform stream never repeats across calls
             int acc = 0;
-            check(atvh_batch_exchange(batch_, round_, seed, &acc), ctx_, "Remd::exchange");
-            attempts_ += (n - static_cast<int>(round_ & 1)) / 2;
+            const int64_t parity = r & 1;
+            check(atvh_batch_exchange(batch_, 2 * round_ + parity, seed, &acc), ctx_, "Remd::exchange");
+            attempts_ += (n - static_cast<int>(parity)) / 2;
             accepts_ += acc;
             ++round_;
         }

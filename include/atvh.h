@@ -105,7 +105,8 @@ int atvh_nlist_configure(atvh_system* sys, double cutoff, double skin,
 /* Unconditional build (NeighborList::build at component build time). */
 int atvh_nlist_build(atvh_system* sys);
 /* LAMMPS Neighbor::decide on the device (++ago, displacement check) followed
- * by the rebuild it decides; no host synchronisation. */
+ * by the rebuild it decides; synchronises once to read the overflow probe
+ * (see atvh_nlist_set_capacity). */
 int atvh_nlist_decide_and_rebuild(atvh_system* sys);
 /* Brick layout of the list (this backend's tiling of the reference cell
  * grid): the candidate index in use (0 = largest brick, 11 = one cell; -1
@@ -121,6 +122,18 @@ int atvh_nlist_set_brick_shape(atvh_system* sys, int candidate);
  * *best_us = its pair + rebuild/8 time in microseconds. */
 int atvh_nlist_autotune(atvh_system* sys, int precision, int trials,
                         int* chosen, double* best_us);
+/* Per-atom list capacity (entries, rounded up to a multiple of 8) for the
+ * next build; the list must be built again. Builds and runs grow it when a
+ * rebuild needs more: the reference sizes its pair arrays from the density
+ * and regrows them inside the rebuild (include/atomiverse/cuda/neighbor/
+ * neighbor_list.cuh:119, src/cuda/neighbor/neighbor_list.cu:544-553). A
+ * rebuild inside atvh_md_run or atvh_batch_run that overflows halts that
+ * system at the step's force phase (no later phase integrates anything);
+ * the call then regrows the list at those positions and resumes, so the
+ * trajectory equals an uninterrupted run with a larger capacity.
+ * atvh_nlist_decide_and_rebuild reads the overflow back (one stream
+ * synchronisation) and regrows at once. */
+int atvh_nlist_set_capacity(atvh_system* sys, int capacity);
 /* Host read of the rebuild count (synchronises the stream). */
 int atvh_nlist_stats(atvh_system* sys, int64_t* nrebuilds, int64_t* npairs_half,
                      int* max_neighbors, int* capacity);
@@ -250,8 +263,9 @@ int atvh_batch_create(atvh_ctx* ctx, atvh_system* const* systems, int nsys,
 int atvh_batch_destroy(atvh_batch* batch);
 /* Capture the batched step once into a hipGraph and replay it (default off). */
 int atvh_batch_set_graph(atvh_batch* batch, int enable);
-/* energy_every as in atvh_md_run. Synchronises at the end and reports any
- * member's capacity overflow as ATVH_E_CAPACITY. */
+/* energy_every as in atvh_md_run. Synchronises at the end. A member whose
+ * rebuild overflows halts at that step, regrows its list and finishes the
+ * run alone (bit-identical to the batched step). */
 int atvh_batch_run(atvh_batch* batch, int nsteps, int precision,
                    int energy_every);
 /* One replica-exchange round on the device (Remd::exchange_round_,
@@ -317,13 +331,16 @@ int atvh_server_submit(atvh_server* server, const atvh_segment* seg,
                        int64_t* id);
 /* Admit what fits, advance every resident segment by one chunk (never past
  * its own end), retire the finished ones (device memory released).
- * Synchronises. */
+ * Synchronises. A segment whose admission or run fails is retired at once
+ * (device memory released) and reported by poll like a finished one; the
+ * other segments go on. */
 int atvh_server_advance(atvh_server* server, int* active, int* queued);
 /* Ids finished since the last poll (up to cap). */
 int atvh_server_poll(atvh_server* server, int64_t* ids, int cap, int* n);
 /* Final positions (wrapped) and velocities of a finished segment in frame
  * order (any pointer may be NULL) and its last potential and kinetic energy;
- * forgets the segment. */
+ * forgets the segment. For a failed segment: returns the error code it
+ * failed with (atvh_last_error names the cause) and forgets it. */
 int atvh_server_fetch(atvh_server* server, int64_t id, double* x, double* y,
                       double* z, double* vx, double* vy, double* vz,
                       double* pe, double* ke);

@@ -35,8 +35,11 @@ def gpu_pairs(s):
     return np.sort(p[:, 0] * s.n + p[:, 1])
 
 
+CASE_SEEDS = {"jittered": 1100, "small_grid": 1101, "random_dense": 1102, "anisotropic": 1103}
+
+
 def random_case(name):
-    rng = np.random.default_rng(abs(hash(name)) % 2 ** 32)
+    rng = np.random.default_rng(CASE_SEEDS[name])
     if name == "jittered":
         x, y, z, box = jittered_lattice(4, 1.9)
         return x, y, z, box, 2.6, 0.4

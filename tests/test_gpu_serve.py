@@ -115,3 +115,37 @@ def test_footprint_estimate_bounds_the_allocation(hip_ctx):
         used = free0 - free1
         assert used <= av.segment_bytes(w) + (32 << 20), (w.name, used, av.segment_bytes(w))
         s.close()
+
+
+def test_failed_admission_is_reported_and_released(hip_ctx):
+    """A segment whose set-up fails (here a coefficient table of 17 types,
+    beyond the 16 the pair kernels hold) is retired as failed: poll reports
+    it, fetch returns its error code, its memory is released and the other
+    segments finish with the results they have alone."""
+    good = W.lj_trimmed(1500, seed=21)
+    bad = W.charged_melt(6, pme_spacing=0.83)
+    nt = 17
+    bad.eps_tab = np.eye(nt)
+    bad.sigma_tab = np.eye(nt)
+    bad.set_tab = np.eye(nt, dtype=np.int32)
+    srv = av.Server(hip_ctx, budget_bytes=4 << 30, chunk_steps=40, precision=av.PREC_MIXED)
+    ids = [srv.submit(good, 120), srv.submit(bad, 50), srv.submit(good, 80)]
+    finished = {}
+    failed = {}
+    rounds = 0
+    while len(finished) + len(failed) < len(ids):
+        srv.advance()
+        for sid in srv.poll():
+            try:
+                finished[sid] = srv.fetch(sid)
+            except av.HipError as e:
+                failed[sid] = e.code
+        rounds += 1
+        assert rounds < 100
+    assert list(failed) == [ids[1]] and failed[ids[1]] == av._abi.E_ARG
+    assert srv.usage()[0] == 0
+    for sid, n in ((ids[0], 120), (ids[2], 80)):
+        d, pe, ke = run_alone(hip_ctx, good, n, av.PREC_MIXED)
+        for k in ("x", "y", "z", "vx", "vy", "vz"):
+            np.testing.assert_array_equal(finished[sid][k], d[k])
+    srv.close()
