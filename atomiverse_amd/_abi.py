@@ -66,6 +66,7 @@ SIGNATURES = {
     "atvh_thermostat_state": (C.c_int, [_vp, _dp, _dp]),
     "atvh_md_run": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "atvh_md_thermo": (C.c_int, [_vp, _dp, _dp, C.c_int]),
+    "atvh_md_run_noalloc": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp]),
     "atvh_batch_create": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.POINTER(_vp)]),
     "atvh_batch_destroy": (C.c_int, [_vp]),
     "atvh_batch_set_graph": (C.c_int, [_vp, C.c_int]),

@@ -278,6 +278,9 @@ void pair_free(atvh_system* s);
 void ljc_enqueue(atvh_system* s, int precision, bool eflag, bool vflag);
 void pme_enqueue(atvh_system* s, int precision, bool ev);
 int pme_partial_blocks(const atvh_system* s);
+// true when the PME buffers and transform plans of this precision exist (or
+// no PME is configured), so a step allocates nothing
+bool pme_ready(const atvh_system* s, int precision);
 const double* pme_partials(const atvh_system* s);
 void pme_energy_consts(const atvh_system* s, double& escale, double& e_self, double& e_net);
 // device bytes one FFT plan pair took (largest seen; 64 MiB before the first)
@@ -300,10 +303,20 @@ void pair_batch_launch(atvh_batch* b, int precision, bool ef);
 void integ_enqueue_initial(atvh_system* s);
 // Queue steps [first, nsteps) of a fused run; resume: the first step starts
 // at its force phase (its initial integrate, decide and rebuild already ran).
-void md_enqueue_steps(atvh_system* s, int first, int nsteps, int precision, int energy_every, bool resume);
+// Step k's potential and kinetic energy go to pe[k] and ke[k].
+struct ThermoRec {
+    double* pe;
+    double* ke;
+};
+ThermoRec thermo_rec(atvh_system* s); // the system's own per-step record
+void md_enqueue_steps(atvh_system* s, int first, int nsteps, int precision, int energy_every, bool resume,
+                      ThermoRec rec);
 // Regrow the list of a system halted by an overflow and finish its run alone
 // (bit-identical to the batched step); returns after the stream drains.
 void md_recover_and_finish(atvh_system* s, int nsteps, int precision, int energy_every);
+// Looks up this device's copy of the allocation-free entry point's static
+// scratch array (once per device, with the device current).
+void noalloc_bind(int device);
 void integ_enqueue_final(atvh_system* s);
 void energy_enqueue_finalize(atvh_system* s, double* record_pe, int step, bool with_pair = true);
 

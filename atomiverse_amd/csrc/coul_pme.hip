@@ -1339,6 +1339,11 @@ void pme_enqueue(atvh_system* s, int precision, bool ev) {
 }
 
 int pme_partial_blocks(const atvh_system* s) { return s->pme ? s->pme->nb : 0; }
+bool pme_ready(const atvh_system* s, int precision) {
+    if (!s->pme)
+        return true;
+    return precision == ATVH_PREC_MIXED ? s->pme->real32 != nullptr : s->pme->real64 != nullptr;
+}
 const double* pme_partials(const atvh_system* s) { return s->pme ? s->pme->partials : nullptr; }
 
 // escale = k_e / (2 pi V) (pme_influence.h:215); self and net-charge energies

@@ -140,6 +140,13 @@ int atvh_ctx_create(int device, atvh_ctx** out) {
         return ATVH_E_HIP;
     }
     c->stream = c->own_stream;
+    try {
+        noalloc_bind(device);
+    } catch (const Error&) {
+        (void)hipStreamDestroy(c->own_stream);
+        delete c;
+        return ATVH_E_HIP;
+    }
     *out = c;
     return ATVH_OK;
 }

@@ -27,6 +27,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 using namespace atvh;
 
@@ -451,11 +453,14 @@ void integ_batch_launch(atvh_batch* b, const char* stage) {
     ATVH_HIP(hipGetLastError());
 }
 
-void md_enqueue_steps(atvh_system* s, int first, int nsteps, int precision, int energy_every, bool resume) {
+ThermoRec thermo_rec(atvh_system* s) { return ThermoRec{s->d_thermo, s->d_thermo + s->thermo_cap}; }
+
+void md_enqueue_steps(atvh_system* s, int first, int nsteps, int precision, int energy_every, bool resume,
+                      ThermoRec rec) {
     atvh_ctx* c = s->ctx;
     hipStream_t st = c->stream;
-    double* rec_pe = s->d_thermo;
-    double* rec_ke = s->d_thermo + s->thermo_cap;
+    double* rec_pe = rec.pe;
+    double* rec_ke = rec.ke;
     const IntArgs ia = make_int_args(s);
     PostArgs p_init = make_post_args(s, false, true);
     PostArgs p_fin = make_post_args(s, true, false);
@@ -502,7 +507,7 @@ void md_recover_and_finish(atvh_system* s, int nsteps, int precision, int energy
             nl_check_ctl(h);
         }
         nl_build(s, true);
-        md_enqueue_steps(s, h.steps_done, nsteps, precision, energy_every, true);
+        md_enqueue_steps(s, h.steps_done, nsteps, precision, energy_every, true, thermo_rec(s));
         ATVH_HIP(hipStreamSynchronize(st));
     }
 }
@@ -521,6 +526,55 @@ void energy_enqueue_finalize(atvh_system* s, double* record_pe, int step, bool w
     a.step = step;
     k_single<OpFinalize><<<1, kBlock, 0, s->ctx->stream>>>(a);
     ATVH_HIP(hipGetLastError());
+}
+
+} // namespace atvh
+
+namespace atvh {
+
+// Allocation-free MD run: the per-step energy record lives in this file-scope
+// array instead of a per-system buffer grown with hipMalloc. Layout: potential
+// energies in [0, kNoAllocSteps), kinetic energies in [kNoAllocSteps,
+// 2*kNoAllocSteps). Each device holds its own copy (1 MiB) until the process
+// ends. Host code never passes the array's name to a kernel or a copy (on the
+// host it is only a stand-in for the symbol): it uses the device address that
+// noalloc_bind looks up once per device.
+constexpr int kNoAllocSteps = 1 << 16;
+static __device__ __align__(256) double scratch[2 * kNoAllocSteps];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+__global__ void k_fill_records(double* rec, int nsteps, int stride) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nsteps) {
+        const double nan = __builtin_nan("");
+        rec[i] = nan;
+        rec[stride + i] = nan;
+    }
+}
+
+double* noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices || !g_scratch_ptr[device])
+        throw Error(ATVH_E_STATE, "atvh_md_run_noalloc: the device's static record was not looked up");
+    return g_scratch_ptr[device];
+}
+} // namespace
+
+void noalloc_bind(int device) {
+    if (device < 0 || device >= kMaxDevices)
+        throw Error(ATVH_E_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device])
+        return;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        throw Error(ATVH_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
 }
 
 } // namespace atvh
@@ -631,9 +685,63 @@ int atvh_md_run(atvh_system* s, int nsteps, int precision, int energy_every) {
         ATVH_HIP(hipStreamSynchronize(st));
     }
     k_run_reset<<<1, 64, 0, st>>>(s->ctl);
-    md_enqueue_steps(s, 0, nsteps, precision, energy_every, false);
+    md_enqueue_steps(s, 0, nsteps, precision, energy_every, false, thermo_rec(s));
     ATVH_HIP(hipStreamSynchronize(st));
     md_recover_and_finish(s, nsteps, precision, energy_every);
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+int atvh_md_run_noalloc(atvh_system* s, int nsteps, int precision, int energy_every, double* pe, double* ke) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    if (nsteps < 0)
+        throw Error(ATVH_E_ARG, "atvh_md_run_noalloc: nsteps < 0");
+    if (nsteps > kNoAllocSteps)
+        throw Error(ATVH_E_ARG, "atvh_md_run_noalloc: nsteps exceeds the static record of " +
+                                    std::to_string(kNoAllocSteps) + " steps");
+    if (precision != ATVH_PREC_FP64 && precision != ATVH_PREC_MIXED)
+        throw Error(ATVH_E_ARG, "atvh_md_run_noalloc: unknown precision");
+    check_integ(s);
+    if (!s->nl_configured)
+        throw Error(ATVH_E_STATE, "atvh_md_run_noalloc: no neighbor list configured");
+    if (!s->pair_configured)
+        throw Error(ATVH_E_STATE, "atvh_md_run_noalloc: no force evaluator configured");
+    // every buffer a step touches must already exist: the first list build
+    // and the first PME evaluation of a precision allocate theirs
+    if (!s->nl_built)
+        throw Error(ATVH_E_STATE, "atvh_md_run_noalloc: the neighbor list is not built (atvh_nlist_build first)");
+    if (!pme_ready(s, precision))
+        throw Error(ATVH_E_STATE, "atvh_md_run_noalloc: no PME evaluation of this precision has run yet "
+                                  "(its mesh buffers and transform plans are made then)");
+    if (c->timing)
+        throw Error(ATVH_E_STATE, "atvh_md_run_noalloc: kernel timing is on (it takes events from a pool)");
+    ATVH_HIP(hipSetDevice(c->device));
+    double* rec = noalloc_scratch(c->device);
+    hipStream_t st = c->stream;
+    if (nsteps > 0) {
+        // NaN marks steps without an energy request (a fill kernel, no memset)
+        k_fill_records<<<(nsteps + kBlock - 1) / kBlock, kBlock, 0, st>>>(rec, nsteps, kNoAllocSteps);
+        ATVH_HIP(hipGetLastError());
+    }
+    s->thermo_n = 0; // the system's own record no longer describes the last run
+    k_run_reset<<<1, 64, 0, st>>>(s->ctl);
+    md_enqueue_steps(s, 0, nsteps, precision, energy_every, false, ThermoRec{rec, rec + kNoAllocSteps});
+    ATVH_HIP(hipStreamSynchronize(st));
+    DevCtl h{};
+    ATVH_HIP(hipMemcpy(&h, s->ctl, sizeof(h), hipMemcpyDeviceToHost));
+    if (h.overflow || h.halo_overflow) {
+        // regrowing the list would allocate: the system stays halted at the
+        // step whose rebuild overflowed and the caller re-uploads its frame
+        s->nl_built = false;
+        nl_check_ctl(h);
+    }
+    if (nsteps > 0 && pe)
+        ATVH_HIP(hipMemcpy(pe, rec, size_t(nsteps) * sizeof(double), hipMemcpyDeviceToHost));
+    if (nsteps > 0 && ke)
+        ATVH_HIP(hipMemcpy(ke, rec + kNoAllocSteps, size_t(nsteps) * sizeof(double), hipMemcpyDeviceToHost));
     return ATVH_OK;
     API_CATCH(c)
 }

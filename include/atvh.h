@@ -226,6 +226,27 @@ int atvh_md_run(atvh_system* sys, int nsteps, int precision, int energy_every);
  * (entries for steps without an energy request hold NaN). Synchronises. */
 int atvh_md_thermo(atvh_system* sys, double* pe, double* ke, int nsteps);
 
+/* Allocation-free variant of atvh_md_run. It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same steps, kernels and results as atvh_md_run; the per-step
+ * energy record lives in a static device array of 65,536 steps per GPU
+ * (1 MiB, held until the process ends) instead of a buffer grown per call,
+ * and is copied to pe[0..nsteps) / ke[0..nsteps) (host arrays, either may be
+ * NULL; NaN on steps without an energy request).
+ * - nsteps above 65,536: ATVH_E_ARG.
+ * - The list must be built (atvh_nlist_build or an earlier run) and, with
+ *   PME, one evaluation of this precision must have run, since those make
+ *   their buffers; kernel timing must be off. Otherwise ATVH_E_STATE.
+ * - A rebuild that overflows the list capacity is not regrown (that would
+ *   allocate): ATVH_E_CAPACITY, the system is left halted mid-step and its
+ *   frame must be uploaded again.
+ * - Every call on one GPU shares that GPU's array: calls that overlap on one
+ *   GPU (two contexts, two host threads) race. atvh_md_thermo does not see
+ *   this variant's records. */
+int atvh_md_run_noalloc(atvh_system* sys, int nsteps, int precision, int energy_every, double* pe,
+                        double* ke);
+
 /* Measurement aid: a plain 16-byte-vector copy of `bytes` between two
  * device buffers, launched iters times on the context stream; *avg_ms =
  * mean kernel time. Its byte counters calibrate the HBM traffic counters. */
