@@ -64,6 +64,11 @@ SIGNATURES = {
     "atvh_integrator_configure": (C.c_int, [_vp, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int64]),
     "atvh_step_vvnh": (C.c_int, [_vp, C.c_int]),
     "atvh_thermostat_state": (C.c_int, [_vp, _dp, _dp]),
+    "atvh_npt_configure": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.c_double, C.c_int, C.c_int, _ip, C.c_double]),
+    "atvh_npt_set_run_span": (C.c_int, [_vp, C.c_int64, C.c_int64]),
+    "atvh_npt_thermo": (C.c_int, [_vp, _dp, _dp, C.c_int]),
+    "atvh_npt_state": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
     "atvh_md_run": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "atvh_md_thermo": (C.c_int, [_vp, _dp, _dp, C.c_int]),
     "atvh_md_run_noalloc": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp]),

@@ -61,6 +61,7 @@ struct DevCtl {
     double T_target;     // thermostat temperature (moves with replica exchanges)
     double stress_kin[6]; // last kinetic stress sum m v_a v_b (eV, Voigt xx yy zz xy xz yz)
     int steps_done;       // steps of the current run completed (advanced by the end-of-step kernel)
+    double red[6];        // fixed-order sums of the last NoseHooverNPT reduction
 };
 
 // A rebuild that overflowed the neighbor capacity or a brick halo leaves the
@@ -71,6 +72,7 @@ __device__ __forceinline__ bool halted(const DevCtl* c) { return (c->overflow | 
 
 struct CoulState; // PairLJCutCoulLong tables (coul_pme.hip)
 struct PmeState;  // KSpacePME mesh, plans and tables (coul_pme.hip)
+struct NptState;  // NoseHooverNPT chains, barostat and records (npt.hip)
 
 struct TimerSlot {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
@@ -123,6 +125,7 @@ struct atvh_system {
     bool nl_configured = false;
     bool nl_built = false;
     double cutoff = 0, skin = 0, cutoff_skin = 0;
+    double nl_box[3] = {0, 0, 0}; // box of the last list build (a different box makes it stale)
     int every = 1, delay = 10, check = 1;
     int nc[3] = {0, 0, 0};
     double cs[3] = {0, 0, 0};
@@ -185,6 +188,9 @@ struct atvh_system {
     double* partials = nullptr;
     int npartials = 0;
     atvh::DevCtl* ctl = nullptr;
+
+    // MTK NoseHooverNPT in place of VelocityVerlet + NoseHoover (npt.hip)
+    atvh::NptState* npt = nullptr;
 
     // thermo record of the last md_run
     double* d_thermo = nullptr;
@@ -298,6 +304,12 @@ void nl_batch_build(atvh_batch* b);
 void nl_batch_launch_rebuild(atvh_batch* b);
 void pair_batch_build(atvh_batch* b);
 void pair_batch_launch(atvh_batch* b, int precision, bool ef);
+
+// npt.hip
+// nsteps of the fused loop with NoseHooverNPT as the integrator; per-step
+// potential/kinetic energy go to pe/ke (host arrays, may be null)
+void npt_run(atvh_system* s, int nsteps, int precision, double* pe, double* ke);
+void npt_free(atvh_system* s);
 
 // integrate.hip
 void integ_enqueue_initial(atvh_system* s);

@@ -232,6 +232,8 @@ int atvh_batch_run(atvh_batch* b, int nsteps, int precision, int energy_every) {
         if (!s->integ_configured || !s->nl_configured || !s->pair_configured)
             throw Error(ATVH_E_STATE, "atvh_batch_run: every member needs an integrator, a neighbor list and "
                                       "a pair style");
+        if (s->npt)
+            throw Error(ATVH_E_STATE, "atvh_batch_run: NoseHooverNPT members run alone (atvh_md_run)");
         if (!s->nl_built) {
             const int rc = atvh_nlist_build(s);
             if (rc != ATVH_OK)

@@ -120,6 +120,7 @@ void sys_free_all(atvh_system* s) {
     nl_free(s);
     pair_free(s);
     coul_free(s);
+    npt_free(s);
 }
 
 } // namespace

@@ -613,6 +613,7 @@ int atvh_integrator_configure(atvh_system* s, double dt, int thermostat, double 
     s->tau_T = tau_T;
     // publish_global_dof (src/cpu/driver/dof.cpp:49): Ndof = 3N - 3
     s->ndof = ndof > 0 ? ndof : 3 * s->n - 3;
+    npt_free(s); // VelocityVerlet (+ NoseHoover) replaces a NoseHooverNPT
     ATVH_HIP(hipSetDevice(c->device));
     ATVH_HIP(hipStreamSynchronize(c->stream));
     DevCtl h{};
@@ -633,6 +634,8 @@ int atvh_step_vvnh(atvh_system* s, int phase) {
     atvh_ctx* c = s->ctx;
     API_TRY
     check_integ(s);
+    if (s->npt)
+        throw Error(ATVH_E_STATE, "atvh_step_vvnh: a NoseHooverNPT is configured (atvh_npt_phase steps it)");
     ATVH_HIP(hipSetDevice(c->device));
     if (phase == ATVH_PHASE_INITIAL) {
         integ_enqueue_initial(s);
@@ -677,6 +680,15 @@ int atvh_md_run(atvh_system* s, int nsteps, int precision, int energy_every) {
         dev_alloc(&s->d_thermo, size_t(2) * nsteps);
     }
     s->thermo_n = nsteps;
+    if (s->npt) {
+        // NoseHooverNPT: energies every step (its barostat reads the virial
+        // and its conserved quantity the potential energy)
+        std::vector<double> rec(size_t(2) * std::max(s->thermo_cap, 1), 0.0);
+        npt_run(s, nsteps, precision, rec.data(), rec.data() + s->thermo_cap);
+        ATVH_HIP(hipMemcpy(s->d_thermo, rec.data(), size_t(2) * s->thermo_cap * sizeof(double),
+                           hipMemcpyHostToDevice));
+        return ATVH_OK;
+    }
     if (nsteps > 0) {
         // NaN marks steps without an energy request; layout [pe | ke], stride thermo_cap
         std::vector<double> nanv(size_t(2) * s->thermo_cap, std::nan(""));
@@ -718,6 +730,8 @@ int atvh_md_run_noalloc(atvh_system* s, int nsteps, int precision, int energy_ev
                                   "(its mesh buffers and transform plans are made then)");
     if (c->timing)
         throw Error(ATVH_E_STATE, "atvh_md_run_noalloc: kernel timing is on (it takes events from a pool)");
+    if (s->npt)
+        throw Error(ATVH_E_STATE, "atvh_md_run_noalloc: NoseHooverNPT rebuilds its grid for each new box (allocates)");
     ATVH_HIP(hipSetDevice(c->device));
     double* rec = noalloc_scratch(c->device);
     hipStream_t st = c->stream;

@@ -1653,6 +1653,8 @@ void nl_build(atvh_system* s, bool keep_counts) {
             h.rebuild = 0;
             ATVH_HIP(hipMemcpy(s->ctl, &h, sizeof(h), hipMemcpyHostToDevice));
             s->nl_built = true;
+            for (int d = 0; d < 3; ++d)
+                s->nl_box[d] = s->box[d];
             return;
         }
         if (h.halo_overflow || tight) {

@@ -243,6 +243,34 @@ class System:
     def integrator(self, dt, thermostat=0, T=0.0, tau=0.0, ndof=0):
         self._ck(self.lib.atvh_integrator_configure(self.h, dt, thermostat, T, tau, ndof))
 
+    def npt(self, dt, T_target, tau_T, P_target, tau_P, drag=0.0, mtchain=3, mpchain=3, couple=(1, 1, 1),
+            T_stop=-1.0):
+        """MTK NoseHooverNPT as the integrator (replaces VelocityVerlet + NoseHoover)."""
+        cp = np.array([1 if c else 0 for c in couple], np.int32)
+        self._ck(self.lib.atvh_npt_configure(self.h, dt, T_target, tau_T, P_target, tau_P, drag, mtchain,
+                                             mpchain, iptr(cp), T_stop))
+
+    def current_box(self):
+        """The box edges now (a barostat moves them)."""
+        out = np.zeros(3)
+        self._ck(self.lib.atvh_system_box(self.h, dptr(out)))
+        return out
+
+    def npt_run_span(self, nfirst, nlast):
+        self._ck(self.lib.atvh_npt_set_run_span(self.h, int(nfirst), int(nlast)))
+
+    def npt_thermo(self, nsteps):
+        """Per-step conserved quantity H and box volume of the last run."""
+        H = np.empty(nsteps)
+        V = np.empty(nsteps)
+        self._ck(self.lib.atvh_npt_thermo(self.h, dptr(H), dptr(V), int(nsteps)))
+        return H, V
+
+    def npt_state(self, mtchain=3, mpchain=3):
+        out = [np.zeros(3), np.zeros(mtchain), np.zeros(mtchain), np.zeros(mpchain), np.zeros(mpchain)]
+        self._ck(self.lib.atvh_npt_state(self.h, *[dptr(a) for a in out]))
+        return dict(eps_dot=out[0], eta_t=out[1], eta_t_dot=out[2], eta_b=out[3], eta_b_dot=out[4])
+
     def step_phase(self, phase):
         self._ck(self.lib.atvh_step_vvnh(self.h, phase))
 

@@ -116,6 +116,30 @@ def lj_fcc_2048000(thermostat=1):
     return lj_fcc(80, seed=87288, thermostat=thermostat, name="lj_fcc_2048000_nvt")
 
 
+def npt_sc_216():
+    """The reference's MTK NPT regression input (regressions/md/npt_lj_mtk/
+    reg_npt_lj_mtk.cpp:15-40): 6^3 simple-cubic sites at rho = 0.8, unit masses,
+    the deterministic velocity pattern 0.02*((i+j+k)%3-1), 0.02*((i+2j)%3-1),
+    0.02*((2i+k)%3-1) [Å/fs], NeighborList(2.5, 0.3) with its defaults (every 1,
+    delay 10, check). Run: NoseHooverNPT(T = 1/kB, tau_T 0.1, P 0, tau_P 1,
+    dt 0.005 fs), 80 steps."""
+    n_side = 6
+    n = n_side ** 3
+    L = (n / 0.8) ** (1.0 / 3.0)
+    sp = L / n_side
+    ix, iy, iz = np.unravel_index(np.arange(n), (n_side, n_side, n_side))
+    x = (ix + 0.5) * sp
+    y = (iy + 0.5) * sp
+    z = (iz + 0.5) * sp
+    vx = 0.02 * ((ix + iy + iz) % 3 - 1)
+    vy = 0.02 * ((ix + 2 * iy) % 3 - 1)
+    vz = 0.02 * ((2 * ix + iz) % 3 - 1)
+    return LJWorkload("npt_sc_216", x.astype(float), y.astype(float), z.astype(float), vx.astype(float),
+                      vy.astype(float), vz.astype(float), np.ones(n), np.array([L, L, L]), dt=0.005,
+                      T=1.0 / KB, tau_T=0.1, delay=10,
+                      meta=dict(P_target=0.0, tau_P=1.0, steps=80, V_reference_cpu=269.425432))
+
+
 def ewald_alpha(cutoff, accuracy=1e-5):
     """EwaldSum::recommend_alpha (ewald_sum.h:58-60)."""
     return (1.35 - 0.15 * math.log(accuracy)) / cutoff

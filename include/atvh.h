@@ -214,6 +214,33 @@ int atvh_step_vvnh(atvh_system* sys, int phase);
 /* Nose-Hoover state (xi) read/write; synchronises. */
 int atvh_thermostat_state(atvh_system* sys, double* xi_out, const double* xi_in);
 
+/* ---- MTK NoseHooverNPT ---------------------------------------------------------------
+ * Replaces cuda::md::NoseHooverNPT(T_target, tau_T, P_target, tau_P, dt, drag,
+ * mtchain, mpchain, couple_axis, T_stop)
+ * (include/atomiverse/cuda/dynamics/nose_hoover_npt.cuh:56-94; algorithm of
+ * src/cpu/dynamics/nose_hoover_npt.cpp). It is the system's integrator in
+ * place of VelocityVerlet + NoseHoover (atvh_integrator_configure switches
+ * back); atvh_md_run then steps it: P1-P6 at initial_integrate, P7 at
+ * final_integrate, P8-P11 at end_of_step, the virial requested every step,
+ * the box rescaled every step and the cell grid rebuilt for the new box at
+ * each list rebuild. Units: K, fs, eV/Å^3 (positive pressure compressive).
+ * couple[3] (NULL = all axes): per-axis barostat flags; T_stop < 0: no ramp.
+ * Not supported with KSpacePME (E_STATE at the run), in a batch, or in
+ * atvh_step_vvnh. */
+int atvh_npt_configure(atvh_system* sys, double dt, double T_target, double tau_T, double P_target,
+                       double tau_P, double drag, int mtchain, int mpchain, const int couple[3],
+                       double T_stop);
+/* NoseHooverNPT::set_run_span (nose_hoover_npt.cuh): the temperature ramp's
+ * span in cumulative step indices. */
+int atvh_npt_set_run_span(atvh_system* sys, int64_t nfirst, int64_t nlast);
+/* Per-step MTK conserved quantity H [eV] (frame.scalars->conserved) and box
+ * volume [Å^3] of the last atvh_md_run. */
+int atvh_npt_thermo(atvh_system* sys, double* H, double* V, int nsteps);
+/* Barostat strain rates eps_dot[3] [1/fs] and chain positions/momenta
+ * (mtchain / mpchain entries each; any pointer may be NULL). */
+int atvh_npt_state(atvh_system* sys, double* eps_dot3, double* eta_t, double* eta_t_dot, double* eta_b,
+                   double* eta_b_dot);
+
 /* ---- fused MD loop -----------------------------------------------------------------
  * nsteps of Runtime::step_verlet_ (src/cpu/driver/runtime.cpp:452-502) queued
  * on the stream with no host synchronisation: initial integrate, device-side

@@ -695,3 +695,384 @@ int orc_md_run_ff(int64_t n, double* x, double* y, double* z, double* vx, double
 }
 
 } // extern "C"
+
+// ---- MTK NoseHooverNPT ------------------------------------------------------
+//
+// cpu::md::NoseHooverNPT (src/cpu/dynamics/nose_hoover_npt.cpp), without the
+// couple-xy / surface-tension options (the CUDA sibling,
+// include/atomiverse/cuda/dynamics/nose_hoover_npt.cuh, has neither). Every
+// expression keeps the reference's operation order; reductions are serial.
+
+namespace {
+
+// nose_hoover_npt.cpp:45
+constexpr double kEpsDtMaxFraction = 0.005;
+
+// expm1c (nose_hoover_npt.cpp:66-73): (e^{2x} - 1)/(2x), Taylor branch near 0
+double npt_expm1c(double x) {
+    if (std::abs(x) < 1e-6)
+        return 1.0 + x + (2.0 / 3.0) * x * x;
+    return std::expm1(2.0 * x) / (2.0 * x);
+}
+
+struct Mtk {
+    const orc_npt_params* q;
+    orc_npt_state* s;
+    int64_t n;
+    const double* mass;
+    double *vx, *vy, *vz;
+    double dt;
+    int64_t ndof;
+    int pdim;
+    double kT, W;
+    double Q_t[ORC_NPT_MAXCHAIN], Q_b[ORC_NPT_MAXCHAIN];
+    double tdrag, pdrag;
+
+    // compute_global_KE (:205-233)
+    double global_ke() const {
+        double ke = 0.0;
+        for (int64_t i = 0; i < n; ++i) {
+            const double v2 = vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i];
+            ke += mass[i] * v2;
+        }
+        ke *= 0.5 * mass_v2_to_energy;
+        return ke;
+    }
+    // compute_axis_KE (:236-266)
+    void axis_ke(double out[3]) const {
+        double kx = 0.0, ky = 0.0, kz = 0.0;
+        for (int64_t i = 0; i < n; ++i) {
+            const double m = mass[i];
+            kx += m * vx[i] * vx[i];
+            ky += m * vy[i] * vy[i];
+            kz += m * vz[i] * vz[i];
+        }
+        out[0] = 0.5 * mass_v2_to_energy * kx;
+        out[1] = 0.5 * mass_v2_to_energy * ky;
+        out[2] = 0.5 * mass_v2_to_energy * kz;
+    }
+    // compute_temp_target (:194-203) and the per-step Q_t refresh (:838-842)
+    void temp_target(int64_t step) {
+        const double T_stop = q->T_stop < 0.0 ? q->T_target : q->T_stop;
+        double delta = 0.0;
+        if (q->nlast > q->nfirst) {
+            delta = static_cast<double>(step - q->nfirst) / static_cast<double>(q->nlast - q->nfirst);
+            delta = std::clamp(delta, 0.0, 1.0);
+        }
+        const double T_live = q->T_target + delta * (T_stop - q->T_target);
+        kT = kB_ev_per_k * T_live;
+        const double tau_T2 = q->tau_T * q->tau_T;
+        Q_t[0] = static_cast<double>(ndof) * kT * tau_T2;
+        for (int k = 1; k < q->mtchain; ++k)
+            Q_t[k] = kT * tau_T2;
+    }
+    // nhc_atoms_half (:269-340)
+    void nhc_atoms_half(double half_dt) {
+        const double quarter_dt = 0.5 * half_dt;
+        const double eighth_dt = 0.5 * quarter_dt;
+        const int M = std::min(q->mtchain, ORC_NPT_MAXCHAIN);
+        double* ed = s->eta_t_dot;
+        auto G = [&](int k) -> double {
+            if (k == 0)
+                return (2.0 * s->ke_global - static_cast<double>(ndof) * kT) / Q_t[0];
+            return (Q_t[k - 1] * ed[k - 1] * ed[k - 1] - kT) / Q_t[k];
+        };
+        for (int k = M - 1; k >= 0; --k) {
+            const double expfac = (k < M - 1) ? std::exp(-ed[k + 1] * eighth_dt) : 1.0;
+            ed[k] *= expfac;
+            ed[k] += G(k) * quarter_dt;
+            ed[k] *= tdrag;
+            ed[k] *= expfac;
+        }
+        const double scale = std::exp(-ed[0] * half_dt);
+        for (int64_t i = 0; i < n; ++i) {
+            vx[i] *= scale;
+            vy[i] *= scale;
+            vz[i] *= scale;
+        }
+        s->ke_global *= scale * scale;
+        for (int k = 0; k < M; ++k)
+            s->eta_t[k] += ed[k] * half_dt;
+        for (int k = 0; k < M; ++k) {
+            const double expfac = (k < M - 1) ? std::exp(-ed[k + 1] * eighth_dt) : 1.0;
+            ed[k] *= expfac;
+            ed[k] += G(k) * quarter_dt;
+            ed[k] *= expfac;
+        }
+    }
+    // nhc_baro_half (:346-421)
+    void nhc_baro_half(double half_dt) {
+        const double quarter_dt = 0.5 * half_dt;
+        const double eighth_dt = 0.5 * quarter_dt;
+        const int M = std::min(q->mpchain, ORC_NPT_MAXCHAIN);
+        const double W_axis = W / 3.0;
+        double* ed = s->eta_b_dot;
+        auto baro_2ke = [&]() -> double {
+            double t = 0.0;
+            for (int a = 0; a < 3; ++a)
+                if (q->couple[a])
+                    t += W_axis * s->eps_dot[a] * s->eps_dot[a];
+            return t;
+        };
+        auto G = [&](int k) -> double {
+            if (k == 0)
+                return (baro_2ke() - kT) / Q_b[0];
+            return (Q_b[k - 1] * ed[k - 1] * ed[k - 1] - kT) / Q_b[k];
+        };
+        for (int k = M - 1; k >= 0; --k) {
+            const double expfac = (k < M - 1) ? std::exp(-ed[k + 1] * eighth_dt) : 1.0;
+            ed[k] *= expfac;
+            ed[k] += G(k) * quarter_dt;
+            ed[k] *= pdrag;
+            ed[k] *= expfac;
+        }
+        const double baro_scale = std::exp(-ed[0] * half_dt);
+        for (int a = 0; a < 3; ++a)
+            if (q->couple[a])
+                s->eps_dot[a] *= baro_scale;
+        for (int k = 0; k < M; ++k)
+            s->eta_b[k] += ed[k] * half_dt;
+        for (int k = 0; k < M; ++k) {
+            const double expfac = (k < M - 1) ? std::exp(-ed[k + 1] * eighth_dt) : 1.0;
+            ed[k] *= expfac;
+            ed[k] += G(k) * quarter_dt;
+            ed[k] *= expfac;
+        }
+    }
+    // update_eps_dot (:459-581), all-axes / per-axis branches (no couple xy)
+    void update_eps_dot(const double box[3], double half_dt) {
+        const double V = box[0] * box[1] * box[2];
+        if (V <= 0.0)
+            return;
+        double ke_axis[3];
+        axis_ke(ke_axis);
+        double diag[3] = {s->vir[0] + 2.0 * ke_axis[0], s->vir[1] + 2.0 * ke_axis[1],
+                          s->vir[2] + 2.0 * ke_axis[2]};
+        if (pdim == 3) {
+            const double ave = (1.0 / 3.0) * (diag[0] + diag[1] + diag[2]);
+            diag[0] = diag[1] = diag[2] = ave;
+        }
+        double mtk_active_ke = 0.0;
+        for (int a = 0; a < 3; ++a)
+            if (q->couple[a])
+                mtk_active_ke += ke_axis[a];
+        const int64_t N_atoms = (ndof + 3) / 3;
+        const double mtk_term =
+            (pdim > 0) ? 6.0 * mtk_active_ke / (static_cast<double>(pdim) * N_atoms) : 0.0;
+        for (int a = 0; a < 3; ++a) {
+            if (!q->couple[a])
+                continue;
+            const double P_axis = diag[a] / V;
+            const double G_eps_a = (3.0 * V * (P_axis - q->P_target) + mtk_term) / W;
+            s->eps_dot[a] += G_eps_a * half_dt;
+        }
+    }
+    // scale_v_by_baro (:591-637)
+    void scale_v_by_baro(double half_dt) {
+        double sum_eps_dot = 0.0;
+        for (int a = 0; a < 3; ++a)
+            if (q->couple[a])
+                sum_eps_dot += s->eps_dot[a];
+        const int64_t N_atoms = (ndof + 3) / 3;
+        const double mtk_term2 = (pdim > 0) ? sum_eps_dot / (static_cast<double>(pdim) * N_atoms) : 0.0;
+        const double sx = std::exp(-(s->eps_dot[0] + mtk_term2) * half_dt);
+        const double sy = std::exp(-(s->eps_dot[1] + mtk_term2) * half_dt);
+        const double sz = std::exp(-(s->eps_dot[2] + mtk_term2) * half_dt);
+        for (int64_t i = 0; i < n; ++i) {
+            vx[i] *= sx;
+            vy[i] *= sy;
+            vz[i] *= sz;
+        }
+        if (sx == sy && sy == sz)
+            s->ke_global *= sx * sx;
+        else
+            s->ke_global = global_ke();
+    }
+    // kick_v_by_force (:639-660)
+    void kick(const double* fx, const double* fy, const double* fz, double half_dt) {
+        const double dtf = half_dt * force_to_accel;
+        for (int64_t i = 0; i < n; ++i) {
+            const double dtfm = dtf / mass[i];
+            vx[i] += dtfm * fx[i];
+            vy[i] += dtfm * fy[i];
+            vz[i] += dtfm * fz[i];
+        }
+    }
+    // propagate_x_and_box (:675-780), single rank: no ghosts
+    void propagate(double* x, double* y, double* z, double box[3]) {
+        double a[3], drift[3];
+        for (int d = 0; d < 3; ++d) {
+            const double eps_dt = std::clamp(s->eps_dot[d] * dt, -kEpsDtMaxFraction, kEpsDtMaxFraction);
+            a[d] = std::exp(eps_dt);
+            drift[d] = npt_expm1c(eps_dt * 0.5);
+        }
+        for (int64_t i = 0; i < n; ++i) {
+            x[i] = x[i] * a[0] + vx[i] * dt * drift[0];
+            y[i] = y[i] * a[1] + vy[i] * dt * drift[1];
+            z[i] = z[i] * a[2] + vz[i] * dt * drift[2];
+        }
+        for (int d = 0; d < 3; ++d)
+            box[d] *= a[d];
+        // wrap_orthorhombic (include/atomiverse/cpu/frame/pbc.h:37-51) into the new box
+        const double lx = box[0], ly = box[1], lz = box[2];
+        for (int64_t i = 0; i < n; ++i) {
+            x[i] -= lx * std::floor(x[i] / lx);
+            y[i] -= ly * std::floor(y[i] / ly);
+            z[i] -= lz * std::floor(z[i] / lz);
+        }
+    }
+    // extended_hamiltonian (:785-820)
+    double hamiltonian(double kinetic_e, double potential_e, const double box[3]) const {
+        double H = kinetic_e + potential_e;
+        const double W_axis = W / 3.0;
+        for (int a = 0; a < 3; ++a)
+            if (q->couple[a])
+                H += 0.5 * W_axis * s->eps_dot[a] * s->eps_dot[a];
+        for (int k = 0; k < q->mtchain; ++k)
+            H += 0.5 * Q_t[k] * s->eta_t_dot[k] * s->eta_t_dot[k];
+        for (int k = 0; k < q->mpchain; ++k)
+            H += 0.5 * Q_b[k] * s->eta_b_dot[k] * s->eta_b_dot[k];
+        H += static_cast<double>(ndof) * kT * s->eta_t[0];
+        for (int k = 1; k < q->mtchain; ++k)
+            H += kT * s->eta_t[k];
+        for (int k = 0; k < q->mpchain; ++k)
+            H += kT * s->eta_b[k];
+        H += q->P_target * (box[0] * box[1] * box[2]);
+        return H;
+    }
+};
+
+} // namespace
+
+extern "C" {
+
+// Runtime::step_verlet_ (src/cpu/driver/runtime.cpp:452-502) with the NPT
+// integrator's three bindings (nose_hoover_npt.h:259-263): initial_integrate
+// P1-P6, final_integrate P7, end_of_step P8-P11 (NoseHooverNPT::launch,
+// nose_hoover_npt.cpp:826-889). The list is rebuilt on the decide cadence
+// when the box changed since the last build (CellList::needs_rebuild ->
+// grid_stale, src/cpu/neighbor/cell_list.cpp:156-159, 369-370) or an atom
+// moved more than half the skin, on a grid recomputed for the new box
+// (cell_list.cpp:344-347).
+int orc_npt_run(int64_t n, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                double* fx, double* fy, double* fz, const double* mass, double box[3],
+                const orc_md_params* p, const orc_npt_params* q, orc_md_state* st,
+                orc_npt_state* ns, int nsteps, double* pe, double* ke, double* H, double* V) {
+    if (n <= 0 || !p || !q || !st || !ns)
+        return -1;
+    if (q->mtchain < 1 || q->mtchain > ORC_NPT_MAXCHAIN || q->mpchain < 1 || q->mpchain > ORC_NPT_MAXCHAIN)
+        return -1;
+    const int nthreads = p->nthreads < 1 ? 1 : p->nthreads;
+#ifdef _OPENMP
+    omp_set_num_threads(nthreads);
+#endif
+    const double cutoff_skin = p->cutoff + p->skin;
+    Grid g = make_grid(box, cutoff_skin);
+    Bins b;
+    HalfList hl;
+    std::vector<double> xs(n), ys(n), zs(n);
+    double box_built[3] = {box[0], box[1], box[2]};
+    auto rebuild = [&]() {
+        if (box[0] != box_built[0] || box[1] != box_built[1] || box[2] != box_built[2])
+            g = make_grid(box, cutoff_skin);
+        bin_atoms(g, n, x, y, z, b);
+        std::memcpy(xs.data(), x, n * sizeof(double));
+        std::memcpy(ys.data(), y, n * sizeof(double));
+        std::memcpy(zs.data(), z, n * sizeof(double));
+        build_half(g, b, n, x, y, z, nthreads, hl);
+        for (int d = 0; d < 3; ++d)
+            box_built[d] = box[d];
+    };
+    rebuild();
+    if (st->step == 0) {
+        st->ago = 0;
+        st->nrebuilds = 1;
+    }
+    Mtk m{};
+    m.q = q;
+    m.s = ns;
+    m.n = n;
+    m.mass = mass;
+    m.vx = vx;
+    m.vy = vy;
+    m.vz = vz;
+    m.dt = p->dt;
+    m.ndof = p->ndof > 0 ? p->ndof : 3 * n - 3;
+    m.pdim = (q->couple[0] ? 1 : 0) + (q->couple[1] ? 1 : 0) + (q->couple[2] ? 1 : 0);
+    // NoseHooverNPT::build (:101-190)
+    m.kT = kB_ev_per_k * q->T_target;
+    const double tau_T2 = q->tau_T * q->tau_T;
+    const double tau_P2 = q->tau_P * q->tau_P;
+    m.Q_t[0] = static_cast<double>(m.ndof) * m.kT * tau_T2;
+    for (int k = 1; k < ORC_NPT_MAXCHAIN; ++k)
+        m.Q_t[k] = m.kT * tau_T2;
+    m.Q_b[0] = m.kT * tau_P2;
+    for (int k = 1; k < ORC_NPT_MAXCHAIN; ++k)
+        m.Q_b[k] = m.kT * tau_P2;
+    m.W = (static_cast<double>(m.ndof) + 3.0) * m.kT * tau_P2;
+    m.tdrag = 1.0 - q->drag * p->dt / q->tau_T;
+    m.pdrag = 1.0 - q->drag * p->dt / q->tau_P;
+    if (!ns->built) {
+        for (int k = 0; k < ORC_NPT_MAXCHAIN; ++k)
+            ns->eta_t[k] = ns->eta_t_dot[k] = ns->eta_b[k] = ns->eta_b_dot[k] = 0.0;
+        for (int a = 0; a < 3; ++a)
+            ns->eps_dot[a] = 0.0;
+        ns->ke_global = 0.5 * static_cast<double>(m.ndof) * m.kT;
+        ns->built = 1;
+    }
+    const double half_dt = 0.5 * p->dt;
+    for (int s = 0; s < nsteps; ++s) {
+        // initial_integrate: P1-P6
+        m.temp_target(st->step);
+        m.nhc_baro_half(half_dt);
+        m.nhc_atoms_half(half_dt);
+        m.update_eps_dot(box, half_dt);
+        m.scale_v_by_baro(half_dt);
+        m.kick(fx, fy, fz, half_dt);
+        m.propagate(x, y, z, box);
+        // decide
+        st->ago++;
+        const bool delay_ok = st->ago >= p->delay;
+        const bool every_ok = (st->ago % (p->every < 1 ? 1 : p->every)) == 0;
+        bool nflag = false;
+        if (delay_ok && every_ok) {
+            const bool stale = box[0] != box_built[0] || box[1] != box_built[1] || box[2] != box_built[2];
+            nflag = !p->check || stale ||
+                    orc_needs_rebuild(n, x, y, z, xs.data(), ys.data(), zs.data(), box, p->skin, nullptr);
+        }
+        if (nflag) {
+            rebuild();
+            st->ago = 0;
+            st->nrebuilds++;
+        }
+        // pre_force + force (the stress is always requested: requires_system_stress)
+        std::memset(fx, 0, n * sizeof(double));
+        std::memset(fy, 0, n * sizeof(double));
+        std::memset(fz, 0, n * sizeof(double));
+        double e = 0.0, vir[6] = {0, 0, 0, 0, 0, 0};
+        pair_lj(n, x, y, z, box, hl.numneigh.data(), hl.firstneigh.data(), hl.neighbors.data(), p->eps,
+                p->sigma, p->cutoff, 0, nullptr, nullptr, nullptr, nthreads, fx, fy, fz, &e, vir);
+        for (int a = 0; a < 3; ++a)
+            ns->vir[a] = vir[a];
+        // final_integrate: P7
+        m.kick(fx, fy, fz, half_dt);
+        // end_of_step: P8-P11
+        ns->ke_global = m.global_ke();
+        m.scale_v_by_baro(half_dt);
+        m.update_eps_dot(box, half_dt);
+        m.nhc_atoms_half(half_dt);
+        m.nhc_baro_half(half_dt);
+        if (pe)
+            pe[s] = e;
+        if (ke)
+            ke[s] = ns->ke_global;
+        if (H)
+            H[s] = m.hamiltonian(ns->ke_global, e, box);
+        if (V)
+            V[s] = box[0] * box[1] * box[2];
+        st->step++;
+    }
+    return 0;
+}
+
+} // extern "C"

@@ -187,6 +187,47 @@ int orc_md_run_ff(int64_t n, double* x, double* y, double* z, double* vx,
                   orc_md_state* state, int nsteps, double* pe, double* ke,
                   double* virial);
 
+/* ---- MTK NoseHooverNPT ----------------------------------------------------- */
+
+#define ORC_NPT_MAXCHAIN 8
+
+/* cpu::md::NoseHooverNPT constructor arguments (nose_hoover_npt.h:183-188):
+ * T_target [K] (ramp start), tau_T [fs], P_target [eV/Å^3] (positive is
+ * compressive), tau_P [fs], drag, chain lengths, per-axis barostat flags,
+ * T_stop [K] (< 0: no ramp) and the ramp's run span (set_run_span). */
+typedef struct orc_npt_params {
+    double T_target, tau_T, P_target, tau_P, drag;
+    int mtchain, mpchain;
+    int couple[3];
+    double T_stop;
+    int64_t nfirst, nlast;
+} orc_npt_params;
+
+/* Extended-Lagrangian state carried across calls; zero-initialised means
+ * "build the integrator" (masses, equipartition KE). vir holds the virial
+ * diagonal of the last force evaluation (the frame's kStress[0..2]; zero for
+ * a fresh frame). */
+typedef struct orc_npt_state {
+    double eta_t[ORC_NPT_MAXCHAIN], eta_t_dot[ORC_NPT_MAXCHAIN];
+    double eta_b[ORC_NPT_MAXCHAIN], eta_b_dot[ORC_NPT_MAXCHAIN];
+    double eps_dot[3];
+    double ke_global;
+    double vir[3];
+    int built;
+} orc_npt_state;
+
+/* cpu::md::MD::run with NoseHooverNPT (replacing VelocityVerlet + NoseHoover)
+ * + NeighborList + PairLJCut (single type, p->eps / p->sigma), phase order of
+ * Runtime::step_verlet_. box[3] is in/out (the barostat rescales it). Per
+ * step: pe, ke (the post-NPT kinetic energy), H (the MTK conserved
+ * quantity) and V (the box volume) when non-null. p->nh and p->T_target /
+ * p->tau_T are unused (q carries the thermostat). */
+int orc_npt_run(int64_t n, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                double* fx, double* fy, double* fz, const double* mass, double box[3],
+                const orc_md_params* p, const orc_npt_params* q, orc_md_state* state,
+                orc_npt_state* nstate, int nsteps, double* pe, double* ke, double* H,
+                double* V);
+
 /* KineticEnergy::launch: 0.5 * mvv2e * sum m v^2 */
 double orc_kinetic_energy(int64_t n, const double* mass, const double* vx,
                           const double* vy, const double* vz);

@@ -63,6 +63,48 @@ class MDState(C.Structure):
     _fields_ = [("xi", C.c_double), ("step", C.c_int64), ("nrebuilds", C.c_int64), ("ago", C.c_int64)]
 
 
+NPT_MAXCHAIN = 8
+
+
+class NPTParams(C.Structure):
+    """cpu::md::NoseHooverNPT constructor arguments (nose_hoover_npt.h:183-188)."""
+    _fields_ = [
+        ("T_target", C.c_double),
+        ("tau_T", C.c_double),
+        ("P_target", C.c_double),
+        ("tau_P", C.c_double),
+        ("drag", C.c_double),
+        ("mtchain", C.c_int),
+        ("mpchain", C.c_int),
+        ("couple", C.c_int * 3),
+        ("T_stop", C.c_double),
+        ("nfirst", C.c_int64),
+        ("nlast", C.c_int64),
+    ]
+
+
+class NPTState(C.Structure):
+    _fields_ = [
+        ("eta_t", C.c_double * NPT_MAXCHAIN),
+        ("eta_t_dot", C.c_double * NPT_MAXCHAIN),
+        ("eta_b", C.c_double * NPT_MAXCHAIN),
+        ("eta_b_dot", C.c_double * NPT_MAXCHAIN),
+        ("eps_dot", C.c_double * 3),
+        ("ke_global", C.c_double),
+        ("vir", C.c_double * 3),
+        ("built", C.c_int),
+    ]
+
+
+def npt_params(T_target, tau_T, P_target, tau_P, drag=0.0, mtchain=3, mpchain=3, couple=(1, 1, 1),
+               T_stop=-1.0, nfirst=0, nlast=0):
+    q = NPTParams(T_target=T_target, tau_T=tau_T, P_target=P_target, tau_P=tau_P, drag=drag,
+                  mtchain=mtchain, mpchain=mpchain, T_stop=T_stop, nfirst=nfirst, nlast=nlast)
+    for a in range(3):
+        q.couple[a] = 1 if couple[a] else 0
+    return q
+
+
 def build(force: bool = False) -> str:
     """Compile the oracle with its Makefile (g++ -ffp-contract=off -fopenmp)."""
     if force or not os.path.exists(_LIB_PATH):
@@ -106,6 +148,10 @@ def lib():
         L.orc_md_run_ff.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                     C.POINTER(MDParams), C.POINTER(MDForceField), C.POINTER(MDState),
                                     C.c_int, _dp, _dp, _dp]
+        L.orc_npt_run.restype = C.c_int
+        L.orc_npt_run.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                  C.POINTER(MDParams), C.POINTER(NPTParams), C.POINTER(MDState),
+                                  C.POINTER(NPTState), C.c_int, _dp, _dp, _dp, _dp]
         L.orc_good_fft_size.restype = C.c_int
         L.orc_good_fft_size.argtypes = [C.c_int]
         L.orc_bspline.restype = None
@@ -284,6 +330,26 @@ def md_run(state_arrays, box, params: MDParams, nsteps, md_state: MDState | None
     if rc != 0:
         raise RuntimeError(f"orc_md_run failed ({rc})")
     return pe, ke, vir, md_state
+
+
+def npt_run(state_arrays, box, params: MDParams, npt: NPTParams, nsteps, md_state: MDState | None = None,
+            npt_state: NPTState | None = None):
+    """MTK NoseHooverNPT MD loop in place on dict arrays (x..fz, mass); `box`
+    (float64[3]) is rescaled in place. Returns (pe, ke, H, V) per step and the
+    two state structs."""
+    s = state_arrays
+    if box.dtype != np.float64 or not box.flags.c_contiguous:
+        raise ValueError("box must be a contiguous float64 array (updated in place)")
+    md_state = md_state or MDState()
+    npt_state = npt_state or NPTState()
+    out = [np.zeros(nsteps) for _ in range(4)]
+    rc = lib().orc_npt_run(s["x"].size, _d(s["x"]), _d(s["y"]), _d(s["z"]), _d(s["vx"]), _d(s["vy"]),
+                           _d(s["vz"]), _d(s["fx"]), _d(s["fy"]), _d(s["fz"]), _d(s["mass"]), _d(box),
+                           C.byref(params), C.byref(npt), C.byref(md_state), C.byref(npt_state), nsteps,
+                           *[_d(a) for a in out])
+    if rc != 0:
+        raise RuntimeError(f"orc_npt_run failed ({rc})")
+    return (*out, md_state, npt_state)
 
 
 # ---- long-range electrostatics ------------------------------------------------
