@@ -305,6 +305,12 @@ void nl_batch_launch_rebuild(atvh_batch* b);
 void pair_batch_build(atvh_batch* b);
 void pair_batch_launch(atvh_batch* b, int precision, bool ef);
 
+// device bytes held per component (atvh_system_memory)
+int64_t nl_device_bytes(const atvh_system* s);
+int64_t pair_device_bytes(const atvh_system* s);
+int64_t coul_device_bytes(const atvh_system* s);
+int64_t pme_device_bytes(const atvh_system* s);
+
 // npt.hip
 // nsteps of the fused loop with NoseHooverNPT as the integrator; per-step
 // potential/kinetic energy go to pe/ke (host arrays, may be null)

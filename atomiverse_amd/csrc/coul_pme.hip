@@ -1339,6 +1339,26 @@ void pme_enqueue(atvh_system* s, int precision, bool ev) {
 }
 
 int pme_partial_blocks(const atvh_system* s) { return s->pme ? s->pme->nb : 0; }
+int64_t coul_device_bytes(const atvh_system* s) {
+    const CoulState* c = s->coul;
+    if (!c)
+        return 0;
+    const int64_t t2 = int64_t(c->ntypes) * c->ntypes;
+    return t2 * 8 * (8 + 4) + int64_t(c->nbin) * (32 + 16);
+}
+int64_t pme_device_bytes(const atvh_system* s) {
+    const PmeState* p = s->pme;
+    if (!p)
+        return 0;
+    const int64_t ntot = int64_t(p->K[0]) * p->K[1] * p->K[2];
+    const int64_t nmodes = int64_t(p->K[0]) * p->K[1] * p->Kh;
+    int64_t b = ntot * 8 + nmodes * 8 + int64_t(kPmeGridCap) * kPartialStride * 8; // fix, H, partials
+    if (p->real64)
+        b += ntot * 8 + nmodes * 16 + fft_pair_bytes();
+    if (p->real32)
+        b += ntot * 4 + nmodes * 8 + fft_pair_bytes();
+    return b;
+}
 bool pme_ready(const atvh_system* s, int precision) {
     if (!s->pme)
         return true;

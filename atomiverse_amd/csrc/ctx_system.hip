@@ -324,6 +324,33 @@ int atvh_system_destroy(atvh_system* s) {
 
 int64_t atvh_system_natoms(const atvh_system* s) { return s ? s->n : 0; }
 
+int atvh_system_memory(const atvh_system* s, int part, int64_t* bytes) {
+    if (!s || !bytes)
+        return ATVH_E_ARG;
+    const int64_t n = s->n;
+    switch (part) {
+    case ATVH_MEM_FRAME:
+        // 23 FP64 per-atom columns (state, staging, x_save), 4 int32 columns,
+        // the reduction partials and the control block
+        *bytes = n * (23 * 8 + 4 * 4) + int64_t(2) * s->npartials * kPartialStride * 8 + int64_t(sizeof(DevCtl));
+        return ATVH_OK;
+    case ATVH_MEM_NLIST:
+        *bytes = nl_device_bytes(s);
+        return ATVH_OK;
+    case ATVH_MEM_PAIR:
+        *bytes = pair_device_bytes(s);
+        return ATVH_OK;
+    case ATVH_MEM_PME:
+        *bytes = pme_device_bytes(s);
+        return ATVH_OK;
+    case ATVH_MEM_INTEGRATOR:
+        *bytes = int64_t(2) * s->thermo_cap * 8; // the per-step energy record
+        return ATVH_OK;
+    default:
+        return ATVH_E_ARG;
+    }
+}
+
 int atvh_system_box(const atvh_system* s, double box[3]) {
     if (!s || !box)
         return ATVH_E_ARG;

@@ -635,7 +635,7 @@ int atvh_step_vvnh(atvh_system* s, int phase) {
     API_TRY
     check_integ(s);
     if (s->npt)
-        throw Error(ATVH_E_STATE, "atvh_step_vvnh: a NoseHooverNPT is configured (atvh_npt_phase steps it)");
+        throw Error(ATVH_E_STATE, "atvh_step_vvnh: a NoseHooverNPT is configured (atvh_md_run steps it)");
     ATVH_HIP(hipSetDevice(c->device));
     if (phase == ATVH_PHASE_INITIAL) {
         integ_enqueue_initial(s);

@@ -1155,6 +1155,20 @@ BrickGeom brick_geom(const atvh_system* s) {
     return g;
 }
 
+int64_t nl_device_bytes(const atvh_system* s) {
+    if (!s->nl_configured || s->ncells == 0)
+        return 0;
+    const int64_t n = s->n, nc = s->ncells, nbk = s->nbricks;
+    int64_t b = s->brick_mode ? int64_t(s->cap) * n * 2 : int64_t(s->cap) * s->npad * 4;
+    b += n * (4 * 4 + 2);                                   // cell_of, acell_of, perm, numneigh, self16
+    b += nc * 4 * 4 + 4;                                    // cell_rank, cell_count, cell_fill, cell_start
+    b += int64_t(s->nscan_blocks) * 4 + (nbk + 1) * 4;      // scan_sums, brick_cell_base
+    b += int64_t(s->nstencil) * 16 + int64_t(s->ncols) * 16; // stencil, columns
+    if (s->brick_mode)
+        b += nbk * (int64_t(kHaloCellCap) * 4 + int64_t(kHaloCap) * 4 + (kHaloCellCap + 1) * 4 + 4 + 8);
+    return b;
+}
+
 void nl_realloc_capacity(atvh_system* s, int cap) {
     dev_free(s->nbr);
     dev_free(s->nbr16);

@@ -715,6 +715,17 @@ void pair_free(atvh_system* s) { dev_free(s->d_ljtab); }
         return ATVH_E_ARG;                                                              \
     }
 
+namespace atvh {
+int64_t pair_device_bytes(const atvh_system* s) {
+    if (!s->pair_configured)
+        return 0;
+    if (s->pair_style == 1)
+        return coul_device_bytes(s);
+    const int nt = std::max(1, s->lj_ntypes);
+    return s->d_ljtab ? int64_t(nt) * nt * 4 * 8 : 0;
+}
+} // namespace atvh
+
 extern "C" {
 
 int atvh_pair_lj_configure(atvh_system* s, double eps, double sigma, double cutoff, int ntypes,

@@ -91,6 +91,12 @@ int atvh_frame_download(atvh_system* sys, double* x, double* y, double* z,
                         double* vx, double* vy, double* vz, double* fx,
                         double* fy, double* fz);
 int atvh_system_box(const atvh_system* sys, double box[3]);
+/* Device bytes a system holds for one part: the frame columns and
+ * reductions, the neighbor list, the pair style's tables, the PME mesh and
+ * plans, the integrator's records. Backs Component::memory_bytes
+ * (include/atomiverse/cuda/component/component.cuh:202). */
+enum { ATVH_MEM_FRAME = 0, ATVH_MEM_NLIST = 1, ATVH_MEM_PAIR = 2, ATVH_MEM_PME = 3, ATVH_MEM_INTEGRATOR = 4 };
+int atvh_system_memory(const atvh_system* sys, int part, int64_t* bytes);
 
 /* ---- neighbor list ------------------------------------------------------------
  * Replaces cuda::NeighborList (include/atomiverse/cuda/neighbor/neighbor_list.cuh:93-229)
