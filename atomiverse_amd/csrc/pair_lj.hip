@@ -190,6 +190,7 @@ struct BrickLJArgs {
     double rc_sq;
     double eps24, eps4, sig6, vshift;
     double inv_sigma; // 1/sigma of the single-type form (coordinates in sigma units)
+    float band;       // mixed: |r^2 - rc^2| below which the FP64 predicate decides (kernel units)
     const double4* tab;
     int ntypes;
     double *fx, *fy, *fz;
@@ -382,6 +383,15 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
     const double4* __restrict__ tab = a.tab;
     const int ntypes = a.ntypes;
     const uint4* __restrict__ base = reinterpret_cast<const uint4*>(a.nbr16 + (int64_t)a0 * a.cap);
+    // the cutoff decision inside the FP32 rounding band is the plain kernel's
+    // FP64 test on the same images (arguments copied to registers: the lambda
+    // must not take the address of the argument block)
+    const float band = a.band;
+    const double rc_sq_fp64 = a.rc_sq;
+    const double* __restrict__ gx = a.x;
+    const double* __restrict__ gy = a.y;
+    const double* __restrict__ gz = a.z;
+    const double Lx = a.L[0], Ly = a.L[1], Lz = a.L[2];
     // two threads per atom when the brick has at most half a block of atoms:
     // lanes 2l and 2l+1 take alternate neighbour groups of atom l
     const int tpa = (2 * nat <= static_cast<int>(blockDim.x)) ? 2 : 1;
@@ -389,12 +399,14 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
         const int li = tpa == 2 ? (t >> 1) : t;
         const int h = tpa == 2 ? (t & 1) : 0;
         const int i = a0 + li;
-        const float4 pi = sp[a.self16[i]];
+        const int qs = a.self16[i];
+        const float4 pi = sp[qs];
         const int ti = TYPED ? __float_as_int(pi.w) : 0;
         const int nn = a.numneigh[i];
         const uint4* __restrict__ grp = base + li;
         const int ng = (nn + kGroup - 1) / kGroup;
         float fxi = 0.f, fyi = 0.f, fzi = 0.f;
+        bool near = false; // some pair's FP32 r^2 lies within the rounding band of rc^2
         // group indices clamp to the last group (to group 0 when nn == 0: the
         // brick's rows always exist); loads run two groups ahead in three
         // rotating slots, so no group is moved between registers
@@ -415,6 +427,7 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
                 const float dz = pi.z - pj[u].z;
                 const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                 const bool in = r2 < rc_sq; // padding rows are the far sentinel
+                near = near || __builtin_fabsf(r2 - rc_sq) <= band;
                 float eps4 = eps4s, shift = shifts;
                 const float r2inv = __builtin_amdgcn_rcpf(r2);
                 float r6inv, fpair;
@@ -464,6 +477,63 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
             s1 = load(g + 2 * tpa);
             group(s2);
             g += tpa;
+        }
+        if (near) {
+            // Rare: walk this thread's groups again and, for each band pair,
+            // take the plain kernel's FP64 test on the same images; where it
+            // disagrees with the FP32 decision, add (or remove) the pair's
+            // FP32 term. Run after the main loop, its FP64 registers are never
+            // live beside the main loop's.
+            int gsi;
+            double xi, yi, zi;
+            row_image(gx, gy, gz, Lx, Ly, Lz, rows[qs], gsi, xi, yi, zi);
+            for (int g = h; g < ng; g += tpa) {
+                const uint4 cur = load(g);
+                for (int u = 0; u < kGroup; ++u) {
+                    const int q = group_entry(cur, u);
+                    const float4 pj = sp[q];
+                    const float dx = pi.x - pj.x;
+                    const float dy = pi.y - pj.y;
+                    const float dz = pi.z - pj.z;
+                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    if (!(__builtin_fabsf(r2 - rc_sq) <= band))
+                        continue;
+                    int gsj;
+                    double xj, yj, zj;
+                    row_image(gx, gy, gz, Lx, Ly, Lz, rows[q], gsj, xj, yj, zj);
+                    const double ddx = xi - xj, ddy = yi - yj, ddz = zi - zj;
+                    const bool exact = fma(ddz, ddz, fma(ddy, ddy, ddx * ddx)) < rc_sq_fp64;
+                    if (exact == (r2 < rc_sq))
+                        continue;
+                    const float sgn = exact ? 1.0f : -1.0f;
+                    float eps24 = 1.0f, eps4 = eps4s, shift = shifts, sig6 = 1.0f;
+                    if (TYPED) {
+                        const double4 t = tab[ti * ntypes + __float_as_int(pj.w)];
+                        eps24 = static_cast<float>(t.x);
+                        eps4 = static_cast<float>(t.y);
+                        sig6 = static_cast<float>(t.z);
+                        shift = static_cast<float>(t.w);
+                    }
+                    const float r2inv = __builtin_amdgcn_rcpf(r2);
+                    const float r6inv = r2inv * r2inv * r2inv * sig6;
+                    const float fpair = TYPED ? eps24 * r2inv * __builtin_fmaf(2.0f * r6inv, r6inv, -r6inv)
+                                              : r2inv * (r6inv * __builtin_fmaf(2.0f, r6inv, -1.0f));
+                    fxi = __builtin_fmaf(sgn * fpair, dx, fxi);
+                    fyi = __builtin_fmaf(sgn * fpair, dy, fyi);
+                    fzi = __builtin_fmaf(sgn * fpair, dz, fzi);
+                    if (EF)
+                        acc[0] += sgn * static_cast<double>(__builtin_fmaf(eps4, r6inv * r6inv - r6inv, -shift));
+                    if (VF) {
+                        const float fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
+                        acc[1] += sgn * static_cast<double>(dx * fxp);
+                        acc[2] += sgn * static_cast<double>(dy * fyp);
+                        acc[3] += sgn * static_cast<double>(dz * fzp);
+                        acc[4] += sgn * static_cast<double>(dx * fyp);
+                        acc[5] += sgn * static_cast<double>(dx * fzp);
+                        acc[6] += sgn * static_cast<double>(dy * fzp);
+                    }
+                }
+            }
         }
         if (tpa == 2) {
             fxi += __shfl_xor(fxi, 1, 64);
@@ -616,6 +686,24 @@ static BrickLJArgs brick_lj_args(atvh_system* s, const LJArgs& a) {
     b.sig6 = a.sig6;
     b.vshift = a.vshift;
     b.inv_sigma = a.sig6 > 0.0 ? std::pow(a.sig6, -1.0 / 6.0) : 1.0;
+    {
+        // The mixed kernel's r^2 comes from FP32 coordinates relative to the
+        // brick centre, |X| <= E (the brick's half extent plus the stencil
+        // reach and the skin an atom may drift before a rebuild). With unit
+        // roundoff u = 2^-24: each difference carries <= 2Eu + |d|u, and the
+        // fma chain adds <= 3u r^2, so near the cutoff
+        //   |r2_fp32 - r2| <= u (4 sqrt(3) rc E + 5 rc^2).
+        // Pairs within twice that of rc^2 take the plain kernel's FP64 test.
+        double E = 0.0;
+        for (int d = 0; d < 3; ++d)
+            E = std::max(E, (0.5 * s->bdim[d] + s->reach[d]) * s->cs[d] + s->skin);
+        const bool typed = a.ntypes > 0 && a.tab;
+        const double us = typed ? 1.0 : b.inv_sigma;
+        const double rc = std::sqrt(a.rc_sq) * us;
+        E *= us;
+        const double u = std::ldexp(1.0, -24);
+        b.band = static_cast<float>(2.0 * u * (4.0 * std::sqrt(3.0) * rc * E + 5.0 * rc * rc));
+    }
     b.tab = a.tab;
     b.ntypes = a.ntypes;
     b.fx = s->fx;

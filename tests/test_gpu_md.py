@@ -128,6 +128,40 @@ def test_two_atoms_at_sigma_give_minus_vshift(hip_ctx):
         assert f["fx"][1] == pytest.approx(24 * eps / sigma, rel=1e-6 if prec else 1e-14)
 
 
+def test_mixed_cutoff_decision_matches_fp64(hip_ctx):
+    """Dimers whose separation lies within 1e-7 relative of the cutoff, on
+    both sides: the FP32 r^2 of the mixed kernel cannot resolve them, so the
+    kernel takes the FP64 test there and every dimer interacts (or not)
+    exactly as in the plain path; the forces then agree to FP32 rounding."""
+    rc, nside, spacing = 2.5, 6, 6.5
+    k = np.arange(nside ** 3) - nside ** 3 // 2
+    r = np.sqrt(rc * rc * (1.0 + 2e-9 * k))  # both sides of rc, none exactly at it
+    g = np.stack(np.unravel_index(np.arange(nside ** 3), (nside,) * 3), axis=1) * spacing + 1.0
+    x = np.concatenate([g[:, 0], g[:, 0] + r])
+    y = np.concatenate([g[:, 1], g[:, 1] + 0.0])
+    z = np.concatenate([g[:, 2], g[:, 2] + 0.0])
+    box = np.full(3, nside * spacing)
+    n = x.size
+    s = av.System(hip_ctx, n, box)
+    s.upload(x, y, z, np.ones(n))
+    s.nlist_configure(rc, 0.3, 1, 0, True)
+    s.pair_lj(1.0, 1.0, rc)
+    s.nlist_build()
+    assert s.brick_shape()[0] >= 0  # the brick kernels
+    out = {}
+    for prec in (av.PREC_FP64, av.PREC_MIXED):
+        s.compute(prec, True, True)
+        out[prec] = s.download()["fx"][nside ** 3:]
+    f64, f32 = out[av.PREC_FP64], out[av.PREC_MIXED]
+    inside = r * r < rc * rc
+    assert inside.any() and (~inside).any()
+    assert np.array_equal(f64 != 0.0, inside)
+    assert np.array_equal(f32 != 0.0, inside), np.nonzero((f32 != 0.0) != inside)
+    m = inside
+    assert np.max(np.abs(f32[m] - f64[m]) / np.abs(f64[m])) < 1e-5
+    s.close()
+
+
 def test_binary_mixture_per_type(hip_ctx, orc):
     x, y, z, box = jittered_lattice(6, 1.2)
     n = x.size
