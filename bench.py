@@ -122,7 +122,7 @@ def run_charged(av, W, ctx, precision, steps, warmup):
     return w, s, t1 - t0, kt
 
 
-def charged_section(av, W, ctx, steps, warmup):
+def charged_section(av, W, ctx, steps, warmup, threads):
     out = {}
     sys_keep = {}
     for name, prec in (("fp64", av.PREC_FP64), ("mixed", av.PREC_MIXED)):
@@ -142,6 +142,9 @@ def charged_section(av, W, ctx, steps, warmup):
     F = np.stack([F["fx"], F["fy"], F["fz"]])
     M = np.stack([M["fx"], M["fy"], M["fz"]])
     out["force_rms_rel_err_mixed_vs_fp64"] = float(np.sqrt(np.sum((M - F) ** 2) / np.sum(F ** 2)))
+    from oracle import oracle as O
+    log("bench: charged melt mixed forces against the oracle")
+    out["force_rms_rel_err_mixed_vs_oracle"] = oracle_force_rms(O, s, w, threads)
     out["mixed_over_fp64"] = out["mixed_atom_steps_per_s"] / out["fp64_atom_steps_per_s"]
     out["workload"] = "charged_melt_512000_nvt (pme 128^3 order 4, rc 3.0, alpha 1.025646)"
     out["steps"] = steps
@@ -319,29 +322,174 @@ def serving_section(av, W, ctx, count, chunk):
             "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99))}
 
 
-def cpu_baseline(W, nsteps):
+def host_cpu():
+    """lscpu's model, sockets and cores, and the threads this process may use."""
+    info = {}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k] = v
+    except (OSError, ValueError):
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    info["threads_available"] = avail
+    return info
+
+
+def _state(w):
+    n = w.n
+    return dict(x=w.x.copy(), y=w.y.copy(), z=w.z.copy(), vx=w.vx.copy(), vy=w.vy.copy(), vz=w.vz.copy(),
+                fx=np.zeros(n), fy=np.zeros(n), fz=np.zeros(n), mass=w.mass.copy())
+
+
+def _oracle_params(O, w, threads, nh=None):
+    return O.MDParams(dt=w.dt, cutoff=w.cutoff, skin=w.skin, every=w.every, delay=w.delay, check=w.check,
+                      eps=w.eps, sigma=w.sigma, nh=w.thermostat if nh is None else nh, T_target=w.T,
+                      tau_T=w.tau_T, ndof=0, nthreads=threads, stress=0)
+
+
+def _time_oracle(O, w, steps, threads, K=None):
+    """Oracle MD loop on `threads` host threads: one warm step, then `steps` timed."""
+    st = _state(w)
+    p = _oracle_params(O, w, threads)
+    if w.pair == "ljcoul":
+        tab = O.LJCoulTables(w.eps_tab, w.sigma_tab, w.set_tab, w.cutoff, w.alpha)
+        run = lambda k: O.md_run_ff(st, w.box, p, tab, w.type_, w.charge, K, w.pme_order, k)  # noqa: E731
+    else:
+        run = lambda k: O.md_run(st, w.box, p, k)  # noqa: E731
+    run(1)
+    t0 = time.perf_counter()
+    run(steps)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(W, nsteps, threads):
     """The CPU oracle (FP64 restatement of the reference CPU path) on the host
-    cores: a bounded sample of the same workload."""
+    cores: bounded samples of the headline workload on all available threads
+    (`value`) and on one thread, and of the other configs: the 4000-atom NVE
+    fluid, the charged melt with PME, and the 256-system ladder as the
+    reference's sequential Remd loop (each replica propagated in turn)."""
     from oracle import oracle as O
 
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    out = {}
     w = W.lj_fcc_2048000(thermostat=1)
-    n = w.n
-    st = dict(x=w.x.copy(), y=w.y.copy(), z=w.z.copy(), vx=w.vx.copy(), vy=w.vy.copy(), vz=w.vz.copy(),
-              fx=np.zeros(n), fy=np.zeros(n), fz=np.zeros(n), mass=w.mass.copy())
-    p = O.MDParams(dt=w.dt, cutoff=w.cutoff, skin=w.skin, every=w.every, delay=w.delay, check=w.check, eps=w.eps,
-                   sigma=w.sigma, nh=1, T_target=w.T, tau_T=w.tau_T, ndof=0, nthreads=cores, stress=0)
-    O.md_run(st, w.box, p, 1)  # warm (threads, pages)
+    dt = _time_oracle(O, w, nsteps, threads)
+    out.update(value=w.n * nsteps / dt, unit="atom-steps/s", cores=threads, kind="port",
+               sample=f"{nsteps} NVT steps of the 2,048,000-atom LJ config (FP64, its list builds included), "
+                      f"{dt:.1f} s on {threads} host threads")
+    rows = {}
+    serial_steps = max(2, nsteps // 16)
+    dt1 = _time_oracle(O, w, serial_steps, 1)
+    rows["lj_fcc_2048000_serial"] = {"atom_steps_per_s": w.n * serial_steps / dt1, "threads": 1,
+                                     "steps": serial_steps, "seconds": dt1}
+    w0 = W.lj_fcc_4000()
+    for th in (1, threads):
+        d0 = _time_oracle(O, w0, 1000, th)
+        rows[f"lj_fcc_4000_nve_{th}t"] = {"atom_steps_per_s": w0.n * 1000 / d0, "threads": th, "steps": 1000,
+                                          "seconds": d0}
+    wc = W.charged_melt_512000()
+    K = O.pme_grid(wc.box, wc.pme_spacing, wc.pme_order)
+    dc = _time_oracle(O, wc, 3, threads, K=K)
+    rows["charged_melt_512000"] = {"atom_steps_per_s": wc.n * 3 / dc, "threads": threads, "steps": 3,
+                                   "seconds": dc}
+    ladder = W.lj_ladder(256)
+    t_l = 0.0
+    steps_l = 5
+    for m in ladder:  # Remd::propagate_ (src/cpu/path/remd.cpp:80-85): one replica after another
+        t_l += _time_oracle(O, m, steps_l, threads)
+    rows["lj_ladder_256_sequential"] = {"atom_steps_per_s": sum(m.n for m in ladder) * steps_l / t_l,
+                                        "threads": threads, "steps": steps_l, "seconds": t_l}
+    out["configs"] = rows
+    out["host"] = host_cpu()
+    return out
+
+
+def oracle_force_rms(O, s, w, threads):
+    """Mixed-precision forces of the device against the oracle's FP64 forces on
+    the same (downloaded) positions: sqrt(sum |F - F_ref|^2 / sum |F_ref|^2)."""
+    import atomiverse_amd as av
+    s.compute(av.PREC_MIXED, False, False)
+    d = s.download()
+    x, y, z = d["x"], d["y"], d["z"]
+    half = O.build_half(x, y, z, w.box, w.cutoff + w.skin, nthreads=threads)
+    if w.pair == "ljcoul":
+        tab = O.LJCoulTables(w.eps_tab, w.sigma_tab, w.set_tab, w.cutoff, w.alpha)
+        fx, fy, fz, _, _ = O.pair_lj_coul_long(x, y, z, w.charge, w.type_, w.box, half, tab, nthreads=threads)
+        if w.pme_order > 0:
+            K = O.pme_grid(w.box, w.pme_spacing, w.pme_order)
+            kx, ky, kz = O.pme(x, y, z, w.charge, w.box, w.alpha, K, w.pme_order)[:3]
+            fx, fy, fz = fx + kx, fy + ky, fz + kz
+    else:
+        fx, fy, fz, _, _ = O.pair_lj(x, y, z, w.box, half, w.eps, w.sigma, w.cutoff)
+    F = np.stack([fx, fy, fz])
+    M = np.stack([d["fx"], d["fy"], d["fz"]])
+    return float(np.sqrt(np.sum((M - F) ** 2) / np.sum(F ** 2)))
+
+
+def _nve(av, W, ctx, make, steps, every, prec):
+    w = make()
+    s = av.System(ctx, w.n, w.box)
+    W.upload(s, w)
+    W.configure(s, w)
+    s.nlist_build()
     t0 = time.perf_counter()
-    O.md_run(st, w.box, p, nsteps)
-    dt = time.perf_counter() - t0
-    return dict(value=n * nsteps / dt, unit="atom-steps/s", cores=cores, kind="port",
-                sample=f"{nsteps} NVT steps of the 2,048,000-atom LJ config (FP64, one list build "
-                       f"included), {dt:.1f} s on {cores} host threads")
+    s.run(steps, prec, energy_every=every)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    pe, ke = s.thermo(steps)
+    e = (pe + ke) / w.n
+    k = np.nonzero(np.isfinite(e))[0]
+    t_fs = (k + 1) * w.dt
+    slope = float(np.polyfit(t_fs, e[k], 1)[0])
+    return s, w, el, {"drift_ev_per_atom_per_ns": slope * 1e6,
+                      "max_abs_dev_ev_per_atom": float(np.max(np.abs(e[k] - e[k[0]]))),
+                      "span_fs": float(t_fs[-1] - t_fs[0])}
+
+
+def config_rows(av, W, ctx, threads, steps_small=1000):
+    """The other configs' precision figures: NVE drift (both precisions) and the
+    mixed force RMS against the oracle. The 4000-atom NVE fluid also reports its
+    own throughput (1000 steps); for the ladder the smallest, median and
+    largest members, for the served stream one LJ and one charged request,
+    each as an NVE side run of `steps_small` steps."""
+    from oracle import oracle as O
+    rows = {}
+    cases = [("lj_fcc_4000", lambda: W.lj_fcc_4000())]
+    lad = sorted(W.lj_ladder(256), key=lambda m: m.n)
+    for tag, m in (("ladder_smallest", lad[0]), ("ladder_median", lad[len(lad) // 2]), ("ladder_largest", lad[-1])):
+        cases.append((f"{tag}_{m.n}", lambda m=m: W.lj_trimmed(m.n, T_star=m.meta["T_star"], seed=m.meta["seed"],
+                                                               thermostat=0, name=m.name)))
+    reqs = serving_requests(W, 24)
+    lj = next(w for w, _ in reqs if w.pair == "lj")
+    ch = next(w for w, _ in reqs if w.pair == "ljcoul")
+    cases.append((f"served_{lj.name}", lambda lj=lj: W.lj_trimmed(lj.n, seed=lj.meta["seed"], thermostat=0,
+                                                                   name=lj.name)))
+    side = round(ch.n ** (1.0 / 3.0))
+    cases.append((f"served_{ch.name}", lambda side=side, ch=ch: W.charged_melt(
+        side, pme_spacing=ch.pme_spacing, seed=ch.meta["seed"], thermostat=0, name=ch.name)))
+    for label, make in cases:
+        row = {}
+        for name, prec in (("fp64", av.PREC_FP64), ("mixed", av.PREC_MIXED)):
+            s, w, el, drift = _nve(av, W, ctx, make, steps_small, 10, prec)
+            row[f"nve_{name}"] = drift
+            if name == "fp64" and label == "lj_fcc_4000":
+                row["fp64_atom_steps_per_s"] = w.n * steps_small / el
+            if name == "mixed":
+                row["force_rms_rel_err_mixed_vs_oracle"] = oracle_force_rms(O, s, w, threads)
+                row["atoms"] = w.n
+            s.close()
+        rows[label] = row
+        log(f"  {label}: mixed force rms vs oracle {row['force_rms_rel_err_mixed_vs_oracle']:.3e}, "
+            f"NVE drift fp64 {row['nve_fp64']['drift_ev_per_atom_per_ns']:.3e} mixed "
+            f"{row['nve_mixed']['drift_ev_per_atom_per_ns']:.3e} eV/atom/ns")
+    return rows
 
 
 def main():
@@ -361,6 +509,8 @@ def main():
                     help="NVE side run length at 2.048M atoms, both precisions (0: skip)")
     ap.add_argument("--nve-charged-steps", type=int, default=2000,
                     help="NVE side run length for the charged melt, both precisions (0: skip)")
+    ap.add_argument("--config-steps", type=int, default=1000,
+                    help="NVE side-run length of the C0 / ladder-member / served-request rows (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (the reference for byte-counter calibration)")
     args = ap.parse_args()
@@ -398,6 +548,10 @@ def main():
     F = np.stack([f64["fx"], f64["fy"], f64["fz"]])
     M = np.stack([f32["fx"], f32["fy"], f32["fz"]])
     rms = float(np.sqrt(np.sum((M - F) ** 2) / np.sum(F ** 2)))
+    from oracle import oracle as O
+    threads = max(1, min(host_cpu()["threads_available"], 16))  # the host share of one GPU
+    log("bench: mixed forces against the oracle at the headline size")
+    rms_orc = oracle_force_rms(O, s, res["mixed"]["w"], threads)
 
     w = res["mixed"]["w"]
     n = w.n
@@ -432,6 +586,7 @@ def main():
         "fp64_ms_per_step": res["fp64"]["dt"] * 1e3 / steps,
         "mixed_over_fp64": value / fp64_value,
         "force_rms_rel_err_mixed_vs_fp64": rms,
+        "force_rms_rel_err_mixed_vs_oracle": rms_orc,
         "roofline": {"bound": "hbm", "kernel": "k_single<OpLjBrickMixed<false, false, false>>", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "bytes_alg_per_launch": bytes_alg, "avg_launch_us": avg_s * 1e6,
@@ -442,16 +597,20 @@ def main():
     for r in res.values():
         r["s"].close()
     if args.charged_steps > 0:
-        out["charged_melt"] = charged_section(av, W, ctx, args.charged_steps, 10)
+        out["charged_melt"] = charged_section(av, W, ctx, args.charged_steps, 10, threads)
     if args.nve_steps > 0 or args.nve_charged_steps > 0:
         out["nve_drift"] = nve_section(av, W, ctx, args.nve_steps, args.nve_charged_steps)
     if args.ladder_steps > 0:
         out["batched_ladder"] = ladder_section(av, W, ctx, args.ladder_steps, max(20, args.ladder_steps // 5))
     if args.serve_requests > 0:
         out["serving"] = serving_section(av, W, ctx, args.serve_requests, 250)
+    if args.config_steps > 0:
+        log("bench: per-config NVE drift and mixed force error against the oracle")
+        out["config_rows"] = config_rows(av, W, ctx, threads, args.config_steps)
     if not args.no_cpu:
-        log(f"bench: cpu baseline ({args.cpu_steps} oracle steps)")
-        out["cpu_baseline"] = cpu_baseline(W, args.cpu_steps)
+        log(f"bench: cpu baseline ({args.cpu_steps} oracle steps on {threads} threads, then the serial "
+            "and per-config samples)")
+        out["cpu_baseline"] = cpu_baseline(W, args.cpu_steps, threads)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
