@@ -18,6 +18,7 @@ OK = 0
 E_ARG, E_HIP, E_STATE, E_CAPACITY, E_NOMEM = -1, -2, -3, -4, -5
 PREC_FP64, PREC_MIXED = 0, 1
 PHASE_INITIAL, PHASE_FINAL = 0, 1
+MEM_FRAME, MEM_NLIST, MEM_PAIR, MEM_PME, MEM_INTEGRATOR = 0, 1, 2, 3, 4
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -43,6 +44,7 @@ SIGNATURES = {
     "atvh_frame_upload": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _dp]),
     "atvh_frame_download": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "atvh_system_box": (C.c_int, [_vp, _dp]),
+    "atvh_system_memory": (C.c_int, [_vp, C.c_int, _lp]),
     "atvh_nlist_configure": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]),
     "atvh_nlist_build": (C.c_int, [_vp]),
     "atvh_nlist_decide_and_rebuild": (C.c_int, [_vp]),

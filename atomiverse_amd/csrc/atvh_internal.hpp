@@ -215,6 +215,9 @@ struct atvh_batch {
     int* nl_act = nullptr;           // [kNlStages][nsys+1] this step's prefix over rebuilding members
     unsigned char* nl_flags = nullptr; // [nsys] this step's rebuild flags
     bool typed = false;
+    // members whose force call is not the batched PairLJCut stage (charged
+    // LJ + Coulomb with or without PME): queued one by one inside the step
+    std::vector<atvh_system*> solo;
     bool use_graph = false;
     hipGraphExec_t graph[2] = {nullptr, nullptr}; // [energy step?]
     int graph_prec[2] = {-1, -1};
@@ -287,6 +290,10 @@ int pme_partial_blocks(const atvh_system* s);
 // true when the PME buffers and transform plans of this precision exist (or
 // no PME is configured), so a step allocates nothing
 bool pme_ready(const atvh_system* s, int precision);
+int pme_grid_blocks(const atvh_system* s);
+void pme_make_buffers(atvh_system* s, int precision);
+// makes and warms every buffer and plan an evaluation of this precision needs
+void pme_prepare(atvh_system* s, int precision);
 const double* pme_partials(const atvh_system* s);
 void pme_energy_consts(const atvh_system* s, double& escale, double& e_self, double& e_net);
 // device bytes one FFT plan pair took (largest seen; 64 MiB before the first)

@@ -243,14 +243,24 @@ int atvh_batch_run(atvh_batch* b, int nsteps, int precision, int energy_every) {
     if (nsteps == 0)
         return ATVH_OK;
     hipStream_t st = c->stream;
+    // k-space members: every buffer and transform plan made (and each plan
+    // run once) before the step is queued or captured
+    for (atvh_system* s : b->sys)
+        pme_prepare(s, precision);
     build_stages(b, nsteps);
     const int nsys = static_cast<int>(b->sys.size());
     k_batch_reset<<<(nsys + kBlock - 1) / kBlock, kBlock, 0, st>>>(b->d_ctls, nsys);
     ATVH_HIP(hipGetLastError());
     const bool timing = c->timing;
+    // Steps that queue k-space members' transforms are not captured: the FFT
+    // library's calls run eagerly between the batched launches (a serving run
+    // that captured them faulted the GPU; the cause was not found)
+    bool kspace = false;
+    for (atvh_system* s : b->solo)
+        kspace = kspace || s->pme != nullptr;
     for (int step = 0; step < nsteps; ++step) {
         const bool ef = (step == nsteps - 1) || (energy_every > 0 && (step % energy_every) == 0);
-        if (b->use_graph && !timing) {
+        if (b->use_graph && !timing && !kspace) {
             hipGraphExec_t& ge = b->graph[ef ? 1 : 0];
             if (!ge || b->graph_prec[ef ? 1 : 0] != precision) {
                 if (ge)

@@ -20,6 +20,12 @@ constexpr int kHaloCap = 2048;      // LDS rows of a staged halo
 constexpr int kSentinelRow = kHaloCap - 1; // staged far away; pads the last list group
 constexpr int kHaloRows = kHaloCap - 1;    // halo atoms a brick may hold
 constexpr int kHaloFillMax = kHaloRows * 15 / 16; // fullest halo a list build accepts for its shape
+constexpr int kMaxBrickAtoms = 320; // home atoms a brick's list build takes
+constexpr int kBrickAtomsFillMax = kMaxBrickAtoms * 15 / 16; // fullest brick a list build accepts for its shape
+// A brick the list build left out (halo or home atoms over capacity; the
+// rebuild is reported as overflowed and the system halts at that step):
+// its list rows are not this halo's, so a pair kernel reads none of them.
+__host__ __device__ inline bool brick_skipped(int nh, int nat) { return nh > kHaloRows || nat > kMaxBrickAtoms; }
 // Sentinel coordinates: every pair with the sentinel is far beyond any cutoff
 // (r^2 ~ 3e30 Å^2 in FP32 relative coordinates, ~3e36 in FP64 absolute ones)
 // and its arithmetic stays finite.

@@ -50,6 +50,7 @@
 #include <atomic>
 #include <climits>
 #include <cmath>
+#include <complex>
 #include <vector>
 
 namespace atvh {
@@ -139,8 +140,10 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
     __shared__ double ljt[SMALLTAB ? kLjcLdsTypes * kLjcLdsTypes * 8 : 1]; // [ti][tj] lj1..lj4, shift
     const int b = xcd_brick(blockIdx.x, gridDim.x); // one launch block per brick
     const int nh = a.halo_n[b];
+    const int2 br = a.brick_range[b];
+    const int a0 = br.x, nat = br.y;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (nh > kHaloRows) { // reported by the build; nothing valid to compute
+    if (brick_skipped(nh, nat)) { // reported by the build; nothing valid to compute
         if (EF || VF)
             block_partials(acc, a.partials, blockIdx.x);
         return;
@@ -165,8 +168,6 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
         for (int k = threadIdx.x; k < a.ntypes * a.ntypes * 8; k += blockDim.x)
             ljt[(k / (a.ntypes * 8)) * kLjcLdsTypes * 8 + k % (a.ntypes * 8)] = a.tab[k];
     __syncthreads();
-    const int2 br = a.brick_range[b];
-    const int a0 = br.x, nat = br.y;
     const double rc_sq = a.rc_sq, inv_dr2 = a.inv_dr2;
     const int ntypes = a.ntypes, nbin1 = a.nbin - 1;
     const double* __restrict__ tab = a.tab;
@@ -303,14 +304,14 @@ void k_ljc_brick_mixed(LjcArgs a) {
     __shared__ float ljt[SMALLTAB ? kLjcLdsTypes * kLjcLdsTypes * 8 : 1]; // [ti][tj] lj1..lj4, shift
     const int b = xcd_brick(blockIdx.x, gridDim.x); // one launch block per brick
     const int nh = a.halo_n[b];
+    const int2 br = a.brick_range[b];
+    const int a0 = br.x, nat = br.y;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (nh > kHaloRows) {
+    if (brick_skipped(nh, nat)) { // reported by the build; nothing valid to compute
         if (EF || VF)
             block_partials<kLjcMixedThreads>(acc, a.partials, blockIdx.x);
         return;
     }
-    const int2 br = a.brick_range[b];
-    const int a0 = br.x, nat = br.y;
     const int* __restrict__ rows = a.halo_rows + (int64_t)b * kHaloCap;
     double r0x, r0y, r0z;
     brick_centre(b, a.nb, a.bdim, a.nc, a.cs, r0x, r0y, r0z);
@@ -569,7 +570,7 @@ __device__ __forceinline__ int pme_wrap(int g, int K) {
 
 // bspline (pme_spline.h:29-55) in precision T, order N.
 template <class T, int N>
-__device__ __forceinline__ void bspline_t(T frac, T (&th)[N], T (&dth)[N]) {
+__host__ __device__ __forceinline__ void bspline_t(T frac, T (&th)[N], T (&dth)[N]) {
     th[N - 1] = T(0);
     th[1] = frac;
     th[0] = T(1) - frac;
@@ -1085,44 +1086,47 @@ int good_fft_size(int n) {
     return n;
 }
 
-void bspline_host(double frac, int n, double* theta, double* dtheta) {
-    theta[n - 1] = 0.0;
-    theta[1] = frac;
-    theta[0] = 1.0 - frac;
-    for (int k = 3; k < n; ++k) {
-        const double div = 1.0 / (k - 1);
-        theta[k - 1] = div * frac * theta[k - 2];
-        for (int j = 1; j < k - 1; ++j)
-            theta[k - 1 - j] = div * ((frac + j) * theta[k - 2 - j] + (k - j - frac) * theta[k - 1 - j]);
-        theta[0] = div * (1.0 - frac) * theta[0];
-    }
-    dtheta[0] = -theta[0];
-    for (int j = 1; j < n; ++j)
-        dtheta[j] = theta[j - 1] - theta[j];
-    const double div = 1.0 / (n - 1);
-    theta[n - 1] = div * frac * theta[n - 2];
-    for (int j = 1; j < n - 1; ++j)
-        theta[n - 1 - j] = div * ((frac + j) * theta[n - 2 - j] + (n - j - frac) * theta[n - 1 - j]);
-    theta[0] = div * (1.0 - frac) * theta[0];
+// Cardinal B-spline weights of order N at the grid points (frac = 0), from
+// the same template the kernels evaluate, so the table and the kernels agree.
+template <int N>
+std::vector<double> grid_point_weights() {
+    double th[N], dth[N];
+    bspline_t<double, N>(0.0, th, dth);
+    return std::vector<double>(th, th + N);
 }
 
-std::vector<double> spline_moduli(int K, int n) {
-    std::vector<double> data(n, 0.0), dd(n, 0.0), mod(K, 0.0);
-    bspline_host(0.0, n, data.data(), dd.data());
-    for (int i = 0; i < K; ++i) {
-        double sr = 0, si = 0;
-        for (int k = 0; k < n - 1; ++k) {
-            const double arg = 2.0 * kPi * i * k / K;
-            sr += data[k] * std::cos(arg);
-            si += data[k] * std::sin(arg);
-        }
-        const double denom = sr * sr + si * si;
-        mod[i] = denom > 1e-12 ? 1.0 / denom : 0.0;
+std::vector<double> grid_point_weights(int n) {
+    switch (n) {
+    case 3: return grid_point_weights<3>();
+    case 4: return grid_point_weights<4>();
+    case 5: return grid_point_weights<5>();
+    case 6: return grid_point_weights<6>();
+    case 7: return grid_point_weights<7>();
+    case 8: return grid_point_weights<8>();
+    default: throw Error(ATVH_E_ARG, "KSpacePME: spline order must be in [3, 8]");
     }
-    for (int i = 0; i < K; ++i)
-        if (mod[i] == 0.0)
-            mod[i] = 0.5 * (mod[(i - 1 + K) % K] + mod[(i + 1) % K]);
-    return mod;
+}
+
+// Euler exponential-spline factors 1/|b(m)|^2 of one axis: b(m) is the
+// discrete Fourier sum of the grid-point weights at mode m. A mode where the
+// sum vanishes (|b|^2 <= 1e-12, e.g. m = K/2 for odd orders on even K) takes
+// the mean of its two neighbours, the reference's repair
+// (include/atomiverse/cpu/longrange/pme_spline.h:32-75).
+std::vector<double> spline_moduli(int K, int n) {
+    const std::vector<double> w = grid_point_weights(n);
+    std::vector<double> inv_b2(K);
+    for (int m = 0; m < K; ++m) {
+        std::complex<double> b(0.0, 0.0);
+        for (int k = 0; k + 1 < n; ++k) // the last weight is zero at frac = 0
+            b += w[k] * std::polar(1.0, 2.0 * kPi * m * k / K);
+        const double b2 = std::norm(b);
+        inv_b2[m] = b2 > 1e-12 ? 1.0 / b2 : 0.0;
+    }
+    std::vector<double> out(inv_b2);
+    for (int m = 0; m < K; ++m)
+        if (inv_b2[m] == 0.0)
+            out[m] = 0.5 * (out[(m + K - 1) % K] + out[(m + 1) % K]);
+    return out;
 }
 
 void pme_release(PmeState* p) {
@@ -1164,7 +1168,7 @@ void note_fft_pair_bytes(int64_t v) {
 
 PmeArgs make_pme_args(atvh_system* s) {
     PmeState* p = s->pme;
-    PmeArgs a;
+    PmeArgs a{};
     a.n = s->n;
     a.x = s->x;
     a.y = s->y;
@@ -1262,6 +1266,63 @@ void ljc_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
     ATVH_HIP(hipGetLastError());
 }
 
+// Blocks (and fixed-order energy partials) of the influence kernel.
+int pme_grid_blocks(const atvh_system* s) {
+    const PmeState* p = s->pme;
+    const int64_t nmodes = (int64_t)p->K[0] * p->K[1] * p->Kh;
+    return std::min(grid_for(nmodes), kPmeGridCap);
+}
+
+// The mesh buffers and transform plans of one precision.
+void pme_make_buffers(atvh_system* s, int precision) {
+    PmeState* p = s->pme;
+    const int64_t ntot = (int64_t)p->K[0] * p->K[1] * p->K[2];
+    const int64_t nmodes = (int64_t)p->K[0] * p->K[1] * p->Kh;
+    if (precision == ATVH_PREC_MIXED && !p->real32) {
+        dev_alloc(&p->real32, ntot);
+        dev_alloc(&p->cplx32, nmodes);
+        HIPFFT_CHECK(hipfftPlan3d(&p->f32, p->K[0], p->K[1], p->K[2], HIPFFT_R2C));
+        HIPFFT_CHECK(hipfftPlan3d(&p->b32, p->K[0], p->K[1], p->K[2], HIPFFT_C2R));
+    }
+    if (precision != ATVH_PREC_MIXED && !p->real64) {
+        dev_alloc(&p->real64, ntot);
+        dev_alloc(&p->cplx64, nmodes);
+        HIPFFT_CHECK(hipfftPlan3d(&p->f64, p->K[0], p->K[1], p->K[2], HIPFFT_D2Z));
+        HIPFFT_CHECK(hipfftPlan3d(&p->b64, p->K[0], p->K[1], p->K[2], HIPFFT_Z2D));
+    }
+}
+
+// Everything a PME evaluation of this precision would allocate, made now:
+// buffers, plans, and one execution of each transform on the scratch mesh
+// (the FFT library loads its kernels and work areas at the first one), so
+// that later evaluations can be captured into a graph. Leaves the forces
+// untouched. Synchronises.
+void pme_prepare(atvh_system* s, int precision) {
+    PmeState* p = s->pme;
+    if (!p || pme_ready(s, precision))
+        return;
+    size_t free_before = 0, free_after = 0, total_mem = 0;
+    ATVH_HIP(hipMemGetInfo(&free_before, &total_mem));
+    pme_make_buffers(s, precision);
+    hipStream_t st = s->ctx->stream;
+    if (precision == ATVH_PREC_MIXED) {
+        HIPFFT_CHECK(hipfftSetStream(p->f32, st));
+        HIPFFT_CHECK(hipfftExecR2C(p->f32, p->real32, reinterpret_cast<hipfftComplex*>(p->cplx32)));
+        HIPFFT_CHECK(hipfftSetStream(p->b32, st));
+        HIPFFT_CHECK(hipfftExecC2R(p->b32, reinterpret_cast<hipfftComplex*>(p->cplx32), p->real32));
+    } else {
+        HIPFFT_CHECK(hipfftSetStream(p->f64, st));
+        HIPFFT_CHECK(hipfftExecD2Z(p->f64, p->real64, reinterpret_cast<hipfftDoubleComplex*>(p->cplx64)));
+        HIPFFT_CHECK(hipfftSetStream(p->b64, st));
+        HIPFFT_CHECK(hipfftExecZ2D(p->b64, reinterpret_cast<hipfftDoubleComplex*>(p->cplx64), p->real64));
+    }
+    ATVH_HIP(hipStreamSynchronize(st));
+    ATVH_HIP(hipMemGetInfo(&free_after, &total_mem));
+    if (free_before > free_after)
+        note_fft_pair_bytes(static_cast<int64_t>(free_before - free_after));
+    p->nb = pme_grid_blocks(s);
+}
+
 void pme_enqueue(atvh_system* s, int precision, bool ev) {
     PmeState* p = s->pme;
     if (!s->has_charge)
@@ -1270,25 +1331,14 @@ void pme_enqueue(atvh_system* s, int precision, bool ev) {
     hipStream_t st = s->ctx->stream;
     const bool mixed = precision == ATVH_PREC_MIXED;
     const int64_t ntot = (int64_t)p->K[0] * p->K[1] * p->K[2];
-    const int64_t nmodes = (int64_t)p->K[0] * p->K[1] * p->Kh;
     // buffers and plans of this precision, made on first use; the device
     // memory the FFT library takes for a plan pair (work areas, tables, the
     // kernels it loads at the first execution) is measured once it has run
     size_t free_before = 0, total_mem = 0;
     const bool fresh = mixed ? !p->real32 : !p->real64;
-    if (mixed && !p->real32) {
-        dev_alloc(&p->real32, ntot);
-        dev_alloc(&p->cplx32, nmodes);
+    if (fresh) {
         ATVH_HIP(hipMemGetInfo(&free_before, &total_mem));
-        HIPFFT_CHECK(hipfftPlan3d(&p->f32, p->K[0], p->K[1], p->K[2], HIPFFT_R2C));
-        HIPFFT_CHECK(hipfftPlan3d(&p->b32, p->K[0], p->K[1], p->K[2], HIPFFT_C2R));
-    }
-    if (!mixed && !p->real64) {
-        dev_alloc(&p->real64, ntot);
-        dev_alloc(&p->cplx64, nmodes);
-        ATVH_HIP(hipMemGetInfo(&free_before, &total_mem));
-        HIPFFT_CHECK(hipfftPlan3d(&p->f64, p->K[0], p->K[1], p->K[2], HIPFFT_D2Z));
-        HIPFFT_CHECK(hipfftPlan3d(&p->b64, p->K[0], p->K[1], p->K[2], HIPFFT_Z2D));
+        pme_make_buffers(s, precision);
     }
     PmeArgs a = make_pme_args(s);
     SpreadRanges sr;
@@ -1302,7 +1352,7 @@ void pme_enqueue(atvh_system* s, int precision, bool ev) {
         sr.cap = kSpreadCap;
     }
     const int gg = grid_capped(ntot);
-    const int gm = std::min(grid_for(nmodes), kPmeGridCap);
+    const int gm = pme_grid_blocks(s);
     if (mixed) {
         dispatch_order<float>(p->order, true, st, a, nullptr, sr);
         k_pme_unfix<float><<<gg, kBlock, 0, st>>>(ntot, p->fix, p->real32, a.inv_fix);

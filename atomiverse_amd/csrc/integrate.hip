@@ -323,7 +323,7 @@ __global__ void k_run_reset(DevCtl* ctl) {
 }
 
 IntArgs make_int_args(atvh_system* s) {
-    IntArgs a;
+    IntArgs a{};
     a.n = s->n;
     a.x = s->x;
     a.y = s->y;
@@ -357,7 +357,7 @@ IntArgs make_int_args(atvh_system* s) {
 }
 
 PostArgs make_post_args(atvh_system* s, bool final_phase, bool decide) {
-    PostArgs p;
+    PostArgs p{};
     p.ctl = s->ctl;
     p.ke_partials = s->partials + size_t(s->npartials) * kPartialStride;
     p.nb = std::min(grid_for(s->n), kIntGrid);
@@ -415,13 +415,18 @@ void integ_batch_build(atvh_batch* b) {
         p1.step = -1;
         pi.push_back(p0);
         pf.push_back(p1);
-        FinalizeArgs f;
+        FinalizeArgs f{};
         f.ctl = s->ctl;
         f.partials = s->partials;
         f.nb = pair_grid(s);
         f.kpartials = nullptr;
         f.knb = 0;
         f.escale = f.e_self = f.e_net = 0.0;
+        if (s->pme) { // the k-space energy and virial partials of the member's PME call
+            f.kpartials = pme_partials(s);
+            f.knb = pme_grid_blocks(s);
+            pme_energy_consts(s, f.escale, f.e_self, f.e_net);
+        }
         f.record_pe = s->d_thermo;
         f.step = -1;
         fa.push_back(f);
@@ -513,7 +518,7 @@ void md_recover_and_finish(atvh_system* s, int nsteps, int precision, int energy
 }
 
 void energy_enqueue_finalize(atvh_system* s, double* record_pe, int step, bool with_pair) {
-    FinalizeArgs a;
+    FinalizeArgs a{};
     a.ctl = s->ctl;
     a.partials = s->partials;
     a.nb = with_pair ? pair_grid(s) : 0;

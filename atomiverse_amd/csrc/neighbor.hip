@@ -507,8 +507,6 @@ struct BrickBuildArgs {
 };
 
 constexpr int kMaxCols = 25;     // stencil columns: (2*reach+1)^2 with reach <= 2
-constexpr int kMaxBrickAtoms = 320;
-constexpr int kBrickAtomsFillMax = kMaxBrickAtoms * 15 / 16; // fullest brick a list build accepts for its shape
 
 // Rebuild step 1 (one workgroup per brick, small LDS, high occupancy): the
 // brick's halo table from the static halo-cell table and the new cell_start —
@@ -535,7 +533,7 @@ __device__ void OpHalo::run(const BrickBuildArgs& a, int bid, int nbk) {
         if (threadIdx.x == 0) {
             a.halo_n[b] = ht.natoms;
             a.brick_range[b] = make_int2(a0, nat);
-            if (ht.natoms > kHaloRows || nat > kMaxBrickAtoms)
+            if (brick_skipped(ht.natoms, nat))
                 atomicAdd(&a.ctl->halo_overflow, 1);
         }
         int* __restrict__ hoff = a.halo_off + (int64_t)b * (kHaloCellCap + 1);
@@ -694,7 +692,7 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
         const int nh = a.halo_n[b];
         const int2 br = a.brick_range[b];
         const int a0 = br.x, nat = br.y;
-        if (nh > kHaloRows || nat > kMaxBrickAtoms)
+        if (brick_skipped(nh, nat))
             continue; // reported by OpHalo (block-uniform)
         const int cbase = a.g.brick_cell_base[b];
         int bx, by, bz;
@@ -1141,7 +1139,7 @@ static void build_halo_cell_tables(atvh_system* s, const std::vector<int>& rank,
 }
 
 BrickGeom brick_geom(const atvh_system* s) {
-    BrickGeom g;
+    BrickGeom g{};
     for (int a = 0; a < 3; ++a) {
         g.nc[a] = s->nc[a];
         g.R[a] = s->reach[a];
@@ -1329,7 +1327,7 @@ void nl_free(atvh_system* s) {
 }
 
 SortArgs nl_sort_args(atvh_system* s, const int* pred) {
-    SortArgs a;
+    SortArgs a{};
     a.pred = pred;
     a.n = s->n;
     a.ncells = s->ncells;
@@ -1358,7 +1356,7 @@ SortArgs nl_sort_args(atvh_system* s, const int* pred) {
 }
 
 PermArgs nl_perm_args(atvh_system* s, const int* pred, bool permute_forces) {
-    PermArgs a;
+    PermArgs a{};
     a.pred = pred;
     a.n = s->n;
     a.perm = s->perm;
@@ -1396,7 +1394,7 @@ PermArgs nl_perm_args(atvh_system* s, const int* pred, bool permute_forces) {
 }
 
 CopyArgs nl_copy_args(atvh_system* s, const int* pred, bool permute_forces) {
-    CopyArgs a;
+    CopyArgs a{};
     a.pred = pred;
     a.n = s->n;
     a.ax = s->ax;
@@ -1436,7 +1434,7 @@ CopyArgs nl_copy_args(atvh_system* s, const int* pred, bool permute_forces) {
 }
 
 BrickBuildArgs nl_brick_args(atvh_system* s, const int* pred) {
-    BrickBuildArgs ba;
+    BrickBuildArgs ba{};
     ba.pred = pred;
     ba.g = brick_geom(s);
     ba.halo_cells = s->halo_cells;

@@ -210,8 +210,10 @@ __device__ void OpLjBrickFp64<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid,
     __shared__ int stype[TYPED ? kHaloCap : 1];
     const int b = xcd_brick(bid, nbk); // one launch block per brick
     const int nh = a.halo_n[b];
+    const int2 br = a.brick_range[b];
+    const int a0 = br.x, nat = br.y;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (nh > kHaloRows) { // reported by the build; nothing valid to compute
+    if (brick_skipped(nh, nat)) { // reported by the build; nothing valid to compute
         if (EF || VF)
             block_partials<kThreads>(acc, a.partials, bid);
         return;
@@ -233,8 +235,6 @@ __device__ void OpLjBrickFp64<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid,
             stype[kSentinelRow] = 0;
     }
     __syncthreads();
-    const int2 br = a.brick_range[b];
-    const int a0 = br.x, nat = br.y;
     // kernel arguments copied to registers: the per-pair lambda below must not
     // take the address of the argument block
     const double rc_sq = a.rc_sq;
@@ -344,14 +344,14 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
     __shared__ float4 sp[kHaloCap]; // x, y, z relative to the brick centre; w = type bits
     const int b = xcd_brick(bid, nbk); // one launch block per brick
     const int nh = a.halo_n[b];
+    const int2 br = a.brick_range[b];
+    const int a0 = br.x, nat = br.y;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (nh > kHaloRows) {
+    if (brick_skipped(nh, nat)) { // reported by the build; nothing valid to compute
         if (EF || VF)
             block_partials<kThreads>(acc, a.partials, bid);
         return;
     }
-    const int2 br = a.brick_range[b];
-    const int a0 = br.x, nat = br.y;
     const int* __restrict__ rows = a.halo_rows + (int64_t)b * kHaloCap;
     // reference point: the brick centre (every halo row lies within the brick
     // widened by the stencil reach, so |x - centre| < ~7 Å and the FP32
@@ -491,6 +491,8 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
                 const uint4 cur = load(g);
                 for (int u = 0; u < kGroup; ++u) {
                     const int q = group_entry(cur, u);
+                    if (q >= nh) // the padding row has no halo row
+                        continue;
                     const float4 pj = sp[q];
                     const float dx = pi.x - pj.x;
                     const float dy = pi.y - pj.y;
@@ -628,7 +630,7 @@ void pair_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
 }
 
 static LJArgs lj_args(atvh_system* s) {
-    LJArgs a;
+    LJArgs a{};
     a.x = s->x;
     a.y = s->y;
     a.z = s->z;
@@ -661,7 +663,7 @@ static LJArgs lj_args(atvh_system* s) {
 }
 
 static BrickLJArgs brick_lj_args(atvh_system* s, const LJArgs& a) {
-    BrickLJArgs b;
+    BrickLJArgs b{};
     b.x = s->x;
     b.y = s->y;
     b.z = s->z;
@@ -737,15 +739,22 @@ static void lj_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
 }
 
 // Batched pair stage: one workgroup per brick of every member (PairLJCut).
+// The force stage of a batched step: PairLJCut members (brick layout, no
+// k-space) share one launch per kernel; every other member (LJ + Coulomb,
+// with or without KSpacePME, or a small-grid layout) runs its own force call
+// inside the same step, in member order.
 void pair_batch_build(atvh_batch* b) {
     std::vector<BrickLJArgs> args;
     std::vector<int> blocks;
     bool typed_any = false, typed_all = true;
+    b->solo.clear();
     for (atvh_system* s : b->sys) {
-        if (!s->pair_configured || s->pair_style != 0)
-            throw Error(ATVH_E_ARG, "batch: every member needs PairLJCut");
-        if (s->pme)
-            throw Error(ATVH_E_ARG, "batch: k-space members are not batched");
+        if (!s->pair_configured)
+            throw Error(ATVH_E_ARG, "batch: every member needs a pair style");
+        if (s->pair_style != 0 || s->pme || !s->brick_mode) {
+            b->solo.push_back(s);
+            continue;
+        }
         const bool t = s->lj_ntypes > 0;
         typed_any |= t;
         typed_all &= t;
@@ -753,9 +762,10 @@ void pair_batch_build(atvh_batch* b) {
         blocks.push_back(s->nbricks);
     }
     if (typed_any && !typed_all)
-        throw Error(ATVH_E_ARG, "batch: members must all be single-type or all per-type");
+        throw Error(ATVH_E_ARG, "batch: PairLJCut members must all be single-type or all per-type");
     b->typed = typed_any;
-    batch_stage(b, "pair", args, blocks);
+    if (!args.empty())
+        batch_stage(b, "pair", args, blocks);
 }
 
 template <class Op>
@@ -766,6 +776,10 @@ static void pair_batch_one(atvh_batch* b) {
 }
 
 void pair_batch_launch(atvh_batch* b, int precision, bool ef) {
+    for (atvh_system* s : b->solo)
+        pair_enqueue(s, precision, ef, ef);
+    if (b->stages.find("pair") == b->stages.end())
+        return;
     const bool mixed = precision == ATVH_PREC_MIXED;
     if (mixed) {
         if (ef)

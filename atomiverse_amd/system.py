@@ -256,6 +256,13 @@ class System:
         self._ck(self.lib.atvh_system_box(self.h, dptr(out)))
         return out
 
+    def memory(self, part):
+        """Device bytes this system holds for one part (MEM_FRAME, MEM_NLIST,
+        MEM_PAIR, MEM_PME, MEM_INTEGRATOR)."""
+        out = C.c_int64(0)
+        self._ck(self.lib.atvh_system_memory(self.h, int(part), C.byref(out)))
+        return out.value
+
     def npt_run_span(self, nfirst, nlast):
         self._ck(self.lib.atvh_npt_set_run_span(self.h, int(nfirst), int(nlast)))
 

@@ -60,15 +60,66 @@ def test_batch_equals_members_alone(hip_ctx, prec, graph):
         s.close()
 
 
-def test_batch_continues_and_matches_one_long_run(hip_ctx):
-    """Two batched runs of 60 steps equal one of 120 for every member."""
+def charged_members(ctx):
+    """Two LJ fluids and two charged melts with PME (1000 and 1728 ions,
+    different meshes)."""
+    out = []
+    for k, w in enumerate([W.lj_trimmed(900, seed=11, thermostat=1),
+                           W.charged_melt(10, seed=21, name="charged_melt_1000"),
+                           W.lj_trimmed(2000, seed=12, thermostat=1),
+                           W.charged_melt(12, seed=22, name="charged_melt_1728")]):
+        s = av.System(ctx, w.n, w.box)
+        W.upload(s, w)
+        W.configure(s, w)
+        out.append((w, s))
+    return out
+
+
+@pytest.mark.parametrize("prec", [av.PREC_FP64, av.PREC_MIXED])
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_with_pme_members_equals_members_alone(hip_ctx, prec, graph):
+    """Charged members (LJ + Coulomb + KSpacePME) advance inside the batched
+    step (their force calls queued in the step; a step holding k-space
+    members is not captured into a graph even when graphs are on) and equal
+    their runs alone bit for bit, energies and k-space included."""
+    nsteps = 60
+    alone = []
+    for w, s in charged_members(hip_ctx):
+        s.run(nsteps, prec, energy_every=10)
+        alone.append(snapshot(s, nsteps))
+        s.close()
+    ms = charged_members(hip_ctx)
+    b = av.Batch(hip_ctx, [s for _, s in ms])
+    b.set_graph(graph)
+    b.run(nsteps // 2, prec, energy_every=10)
+    b.run(nsteps - nsteps // 2, prec, energy_every=10)
+    for (w, s), (d0, pe0, ke0, xi0, nr0) in zip(ms, alone):
+        d1, _, _, xi1, nr1 = snapshot(s, nsteps - nsteps // 2)
+        for k in d0:
+            np.testing.assert_array_equal(d1[k], d0[k], err_msg=f"{w.name} {k}")
+        pe1, ke1 = s.thermo(nsteps - nsteps // 2)
+        np.testing.assert_array_equal(pe1, pe0[nsteps // 2:])
+        np.testing.assert_array_equal(ke1, ke0[nsteps // 2:])
+        assert xi1 == xi0 and nr1 == nr0
+    b.close()
+    for _, s in ms:
+        s.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_continues_and_matches_one_long_run(hip_ctx, graph):
+    """Batched runs of 60, 30 and 30 steps equal one of 120 for every member,
+    with and without the step captured into graphs (each run captures its
+    own)."""
     ms_a = members(hip_ctx, SIZES[:3])
     ms_b = members(hip_ctx, SIZES[:3])
     ba = av.Batch(hip_ctx, [s for _, s in ms_a])
     bb = av.Batch(hip_ctx, [s for _, s in ms_b])
+    bb.set_graph(graph)
     ba.run(120, av.PREC_MIXED)
     bb.run(60, av.PREC_MIXED)
-    bb.run(60, av.PREC_MIXED)
+    bb.run(30, av.PREC_MIXED)
+    bb.run(30, av.PREC_MIXED)
     for (_, sa), (_, sb) in zip(ms_a, ms_b):
         da, db = sa.download(), sb.download()
         for k in da:
