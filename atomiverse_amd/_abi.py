@@ -50,6 +50,8 @@ SIGNATURES = {
     "atvh_nlist_decide_and_rebuild": (C.c_int, [_vp]),
     "atvh_nlist_brick_shape": (C.c_int, [_vp, _ip, _ip]),
     "atvh_nlist_set_brick_shape": (C.c_int, [_vp, C.c_int]),
+    "atvh_nlist_brick_fill": (C.c_int, [_vp, _ip, _ip]),
+    "atvh_nlist_set_brick_limit": (C.c_int, [_vp, C.c_int]),
     "atvh_nlist_autotune": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _dp]),
     "atvh_nlist_set_capacity": (C.c_int, [_vp, C.c_int]),
     "atvh_nlist_stats": (C.c_int, [_vp, _lp, _lp, _ip, _ip]),

@@ -189,6 +189,9 @@ struct atvh_system {
     int npartials = 0;
     atvh::DevCtl* ctl = nullptr;
 
+    // home atoms a list build takes per brick (0: the LDS-bound maximum)
+    int brick_atom_limit = 0;
+
     // MTK NoseHooverNPT in place of VelocityVerlet + NoseHoover (npt.hip)
     atvh::NptState* npt = nullptr;
 

@@ -20,12 +20,19 @@ constexpr int kHaloCap = 2048;      // LDS rows of a staged halo
 constexpr int kSentinelRow = kHaloCap - 1; // staged far away; pads the last list group
 constexpr int kHaloRows = kHaloCap - 1;    // halo atoms a brick may hold
 constexpr int kHaloFillMax = kHaloRows * 15 / 16; // fullest halo a list build accepts for its shape
-constexpr int kMaxBrickAtoms = 320; // home atoms a brick's list build takes
-constexpr int kBrickAtomsFillMax = kMaxBrickAtoms * 15 / 16; // fullest brick a list build accepts for its shape
+constexpr int kMaxBrickAtoms = 320; // home atoms a brick's list build can take
+// Home atoms a list build takes per brick on this system: kMaxBrickAtoms
+// unless lowered (atvh_nlist_set_brick_limit).
+inline int brick_atom_limit(const atvh_system* s) {
+    return s->brick_atom_limit > 0 ? s->brick_atom_limit : kMaxBrickAtoms;
+}
 // A brick the list build left out (halo or home atoms over capacity; the
 // rebuild is reported as overflowed and the system halts at that step):
 // its list rows are not this halo's, so a pair kernel reads none of them.
-__host__ __device__ inline bool brick_skipped(int nh, int nat) { return nh > kHaloRows || nat > kMaxBrickAtoms; }
+// Every reader passes the limit the build used.
+__host__ __device__ inline bool brick_skipped(int nh, int nat, int max_atoms) {
+    return nh > kHaloRows || nat > max_atoms;
+}
 // Sentinel coordinates: every pair with the sentinel is far beyond any cutoff
 // (r^2 ~ 3e30 Å^2 in FP32 relative coordinates, ~3e36 in FP64 absolute ones)
 // and its arithmetic stays finite.
@@ -47,6 +54,7 @@ struct BrickGeom {
     double cs[3];
     const int* cell_rank;       // reference cell index -> layout rank
     const int* brick_cell_base; // [nbricks+1] first rank of each brick
+    int max_atoms;              // home atoms the build takes per brick
 };
 
 struct HaloTable {

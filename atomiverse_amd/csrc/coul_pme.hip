@@ -108,6 +108,7 @@ struct LjcArgs {
     const uint16_t* self16;
     const int* numneigh;
     int cap;
+    int max_atoms; // home atoms the list build took per brick
     double L[3], cs[3];
     int nc[3], nb[3], bdim[3];
     double rc_sq;
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_ljc_brick_fp64(LjcArgs a) {
     const int2 br = a.brick_range[b];
     const int a0 = br.x, nat = br.y;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (brick_skipped(nh, nat)) { // reported by the build; nothing valid to compute
+    if (brick_skipped(nh, nat, a.max_atoms)) { // reported by the build; nothing valid to compute
         if (EF || VF)
             block_partials(acc, a.partials, blockIdx.x);
         return;
@@ -307,7 +308,7 @@ void k_ljc_brick_mixed(LjcArgs a) {
     const int2 br = a.brick_range[b];
     const int a0 = br.x, nat = br.y;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (brick_skipped(nh, nat)) { // reported by the build; nothing valid to compute
+    if (brick_skipped(nh, nat, a.max_atoms)) { // reported by the build; nothing valid to compute
         if (EF || VF)
             block_partials<kLjcMixedThreads>(acc, a.partials, blockIdx.x);
         return;
@@ -1223,6 +1224,7 @@ void ljc_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
     a.self16 = s->self16;
     a.numneigh = s->numneigh;
     a.cap = s->cap;
+    a.max_atoms = brick_atom_limit(s);
     for (int d = 0; d < 3; ++d) {
         a.L[d] = s->box[d];
         a.cs[d] = s->cs[d];

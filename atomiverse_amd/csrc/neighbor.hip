@@ -533,7 +533,7 @@ __device__ void OpHalo::run(const BrickBuildArgs& a, int bid, int nbk) {
         if (threadIdx.x == 0) {
             a.halo_n[b] = ht.natoms;
             a.brick_range[b] = make_int2(a0, nat);
-            if (brick_skipped(ht.natoms, nat))
+            if (brick_skipped(ht.natoms, nat, a.g.max_atoms))
                 atomicAdd(&a.ctl->halo_overflow, 1);
         }
         int* __restrict__ hoff = a.halo_off + (int64_t)b * (kHaloCellCap + 1);
@@ -692,7 +692,7 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
         const int nh = a.halo_n[b];
         const int2 br = a.brick_range[b];
         const int a0 = br.x, nat = br.y;
-        if (brick_skipped(nh, nat))
+        if (brick_skipped(nh, nat, a.g.max_atoms))
             continue; // reported by OpHalo (block-uniform)
         const int cbase = a.g.brick_cell_base[b];
         int bx, by, bz;
@@ -1150,6 +1150,7 @@ BrickGeom brick_geom(const atvh_system* s) {
     }
     g.cell_rank = s->cell_rank;
     g.brick_cell_base = s->brick_cell_base;
+    g.max_atoms = brick_atom_limit(s);
     return g;
 }
 
@@ -1655,7 +1656,7 @@ void nl_build(atvh_system* s, bool keep_counts) {
         if (s->brick_mode && s->brick_pick < kLastBrickCand && h.halo_overflow == 0) {
             int max_halo = 0, max_atoms = 0;
             brick_fill(s, max_halo, max_atoms);
-            tight = max_halo > kHaloFillMax || max_atoms > kBrickAtomsFillMax;
+            tight = max_halo > kHaloFillMax || max_atoms > brick_atom_limit(s) * 15 / 16;
         }
         if (h.overflow == 0 && h.halo_overflow == 0 && !tight) {
             h.ago = 0;
@@ -1783,6 +1784,46 @@ int atvh_nlist_set_brick_shape(atvh_system* s, int candidate) {
         s->brick_pick_min = candidate;
         nl_setup_grid(s);
         s->nl_built = false;
+    }
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+int atvh_nlist_brick_fill(atvh_system* s, int* max_halo, int* max_atoms) {
+    if (!s || !max_halo || !max_atoms)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    *max_halo = *max_atoms = 0;
+    if (!s->brick_mode)
+        return ATVH_OK;
+    if (!s->nl_built)
+        throw Error(ATVH_E_STATE, "atvh_nlist_brick_fill: no neighbor list");
+    ATVH_HIP(hipSetDevice(c->device));
+    ATVH_HIP(hipStreamSynchronize(c->stream));
+    brick_fill(s, *max_halo, *max_atoms);
+    return ATVH_OK;
+    API_CATCH(c)
+}
+
+int atvh_nlist_set_brick_limit(atvh_system* s, int max_atoms) {
+    if (!s)
+        return ATVH_E_ARG;
+    atvh_ctx* c = s->ctx;
+    API_TRY
+    if (max_atoms < 0 || max_atoms > kMaxBrickAtoms)
+        throw Error(ATVH_E_ARG, "atvh_nlist_set_brick_limit: max_atoms must be in [0, " +
+                                    std::to_string(kMaxBrickAtoms) + "] (0: " + std::to_string(kMaxBrickAtoms) +
+                                    ")");
+    ATVH_HIP(hipSetDevice(c->device));
+    ATVH_HIP(hipStreamSynchronize(c->stream));
+    s->brick_atom_limit = max_atoms;
+    if (s->brick_mode && s->nl_built) {
+        // the list stays valid while no brick it holds is over the new limit
+        int mh = 0, ma = 0;
+        brick_fill(s, mh, ma);
+        if (ma > brick_atom_limit(s))
+            s->nl_built = false;
     }
     return ATVH_OK;
     API_CATCH(c)

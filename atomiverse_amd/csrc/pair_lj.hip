@@ -191,6 +191,7 @@ struct BrickLJArgs {
     double eps24, eps4, sig6, vshift;
     double inv_sigma; // 1/sigma of the single-type form (coordinates in sigma units)
     float band;       // mixed: |r^2 - rc^2| below which the FP64 predicate decides (kernel units)
+    int max_atoms;    // home atoms the list build took per brick
     const double4* tab;
     int ntypes;
     double *fx, *fy, *fz;
@@ -213,7 +214,7 @@ __device__ void OpLjBrickFp64<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid,
     const int2 br = a.brick_range[b];
     const int a0 = br.x, nat = br.y;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (brick_skipped(nh, nat)) { // reported by the build; nothing valid to compute
+    if (brick_skipped(nh, nat, a.max_atoms)) { // reported by the build; nothing valid to compute
         if (EF || VF)
             block_partials<kThreads>(acc, a.partials, bid);
         return;
@@ -347,7 +348,7 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
     const int2 br = a.brick_range[b];
     const int a0 = br.x, nat = br.y;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (brick_skipped(nh, nat)) { // reported by the build; nothing valid to compute
+    if (brick_skipped(nh, nat, a.max_atoms)) { // reported by the build; nothing valid to compute
         if (EF || VF)
             block_partials<kThreads>(acc, a.partials, bid);
         return;
@@ -675,6 +676,7 @@ static BrickLJArgs brick_lj_args(atvh_system* s, const LJArgs& a) {
     b.self16 = s->self16;
     b.numneigh = s->numneigh;
     b.cap = s->cap;
+    b.max_atoms = brick_atom_limit(s);
     for (int d = 0; d < 3; ++d) {
         b.L[d] = s->box[d];
         b.cs[d] = s->cs[d];

@@ -157,6 +157,17 @@ class System:
     def set_brick_shape(self, candidate):
         self._ck(self.lib.atvh_nlist_set_brick_shape(self.h, int(candidate)))
 
+    def brick_fill(self):
+        """(fullest halo, fullest brick's home atoms) of the current list."""
+        mh = C.c_int()
+        ma = C.c_int()
+        self._ck(self.lib.atvh_nlist_brick_fill(self.h, C.byref(mh), C.byref(ma)))
+        return mh.value, ma.value
+
+    def set_brick_limit(self, max_atoms):
+        """Home atoms a list build takes per brick (<= 320; 0 restores 320)."""
+        self._ck(self.lib.atvh_nlist_set_brick_limit(self.h, int(max_atoms)))
+
     def autotune(self, precision=_abi.PREC_MIXED, trials=3):
         """Pick the fastest brick shape on this system; returns (candidate, microseconds)."""
         k = C.c_int()

@@ -121,6 +121,17 @@ int atvh_nlist_brick_shape(atvh_system* sys, int* candidate, int* bdim);
 /* Use candidate k or the next smaller valid one; the list must be built
  * again (atvh_nlist_build). */
 int atvh_nlist_set_brick_shape(atvh_system* sys, int candidate);
+/* The fullest halo and the fullest brick (home atoms) of the current list;
+ * zeros for the small-grid layout. Synchronises. */
+int atvh_nlist_brick_fill(atvh_system* sys, int* max_halo, int* max_atoms);
+/* Home atoms a list build takes per brick, at most 320 (0 restores 320). A
+ * rebuild that finds a fuller brick overflows as one past the LDS halo
+ * capacity does: inside atvh_md_run or atvh_batch_run the system halts at
+ * that step's force phase, the brick shape steps down and the run resumes
+ * (see atvh_nlist_set_capacity). The current list stays valid while none of
+ * its bricks is over the new limit. Tests lower it to provoke that path;
+ * the reference's cluster lists have no such bound. */
+int atvh_nlist_set_brick_limit(atvh_system* sys, int max_atoms);
 /* Autotuner: time the current brick shape and the next two smaller ones on
  * this system (one rebuild + `trials` pair launches each, HIP events), keep
  * the fastest and rebuild the list with it. The frame's forces are kept;
