@@ -658,11 +658,15 @@ struct OpPairs {
     static __device__ void run(const BrickBuildArgs& a, int bid, int nbk);
 };
 
-// LDS of one OpPairs workgroup (77.6 KB: two workgroups per CU).
+constexpr int kMaxBrickCells = 100; // home cells of the largest brick shape (4 x 5 x 5)
+
+// LDS of one OpPairs workgroup (79.2 KB: two workgroups per CU).
 struct PairsLds {
-    float sx[kHaloCap], sy[kHaloCap], sz[kHaloCap]; // image coordinates relative to the brick centre
-    float ss[kHaloCap];                             // their squared norm
+    float4 sp[kHaloCap]; // image coordinates relative to the brick centre; w = their squared norm
     int h_off[kHaloCellCap + 1];
+    // per home cell: first slot, halo cell, halo rows [lo, hi) packed lo | hi << 16,
+    // brick-local cell coordinates packed x | y << 8 | z << 16
+    int4 cells[kMaxBrickCells];
     unsigned short cand[OpPairs::kWaves][kCandCap];
     uint4 out[OpPairs::kWaves][2][kOutGroups]; // staged rows of the wave's two atoms
     long long tot[OpPairs::kWaves];
@@ -717,11 +721,17 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
             const float X = static_cast<float>((a.x[gs] + shift_of(code, 0, Lx)) - bc0);
             const float Y = static_cast<float>((a.y[gs] + shift_of(code, 1, Ly)) - bc1);
             const float Z = static_cast<float>((a.z[gs] + shift_of(code, 2, Lz)) - bc2);
-            L.sx[q] = X;
-            L.sy[q] = Y;
-            L.sz[q] = Z;
-            L.ss[q] = __builtin_fmaf(Z, Z, __builtin_fmaf(Y, Y, X * X));
+            L.sp[q] = make_float4(X, Y, Z, __builtin_fmaf(Z, Z, __builtin_fmaf(Y, Y, X * X)));
         }
+        // the home cells' table (the divisions run once per cell and brick)
+        const int nbc = d0 * d1 * d2;
+        for (int c = threadIdx.x; c < nbc; c += blockDim.x) {
+            const int lz = c % d2, ly = (c / d2) % d1, lx = c / (d1 * d2);
+            const int hs = ((lx + R0) * HY + (ly + R1)) * HZ + lz + R2;
+            L.cells[c] = make_int4(a.start[cbase + c], hs, hoff[hs] | (hoff[hs + 1] << 16), lx | (ly << 8) | (lz << 16));
+        }
+        // this lane's stencil column as an offset of halo cells from the home cell
+        const int colofs = (rw.x * HY + rw.y) * HZ;
         // the halo reaches (d/2 + R) cells from the centre on each axis
         const float ext = static_cast<float>(fmax(fmax((0.5 * d0 + R0) * a.g.cs[0], (0.5 * d1 + R1) * a.g.cs[1]),
                                                   (0.5 * d2 + R2) * a.g.cs[2]));
@@ -731,7 +741,6 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
             L.next_cell = kWaves;
         __syncthreads();
         uint16_t* __restrict__ lst = a.nbr16 + (int64_t)a0 * cap;
-        const int nbc = d0 * d1 * d2;
         // cells are handed out one at a time (wave w starts with cell w and
         // then takes the next unclaimed one), which evens out the waves' loads
         // when cells hold different numbers of atoms
@@ -743,19 +752,21 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
             [&] {
             // cells of a brick are ranked in this order, so the cell's atoms are
             // the slots from cell_start of its rank on
-            const int slot0 = __builtin_amdgcn_readfirstlane(a.start[cbase + c]);
-            const int lz = c % d2, ly = (c / d2) % d1, lx = c / (d1 * d2);
-            const int hz0 = lz + R2;
-            const int hs = ((lx + R0) * HY + (ly + R1)) * HZ + hz0;
-            const int hq0 = __builtin_amdgcn_readfirstlane(L.h_off[hs]);
-            const int hq1 = __builtin_amdgcn_readfirstlane(L.h_off[hs + 1]);
+            const int4 cell = L.cells[c];
+            const int slot0 = __builtin_amdgcn_readfirstlane(cell.x);
+            const int hs = __builtin_amdgcn_readfirstlane(cell.y);
+            const int hq = __builtin_amdgcn_readfirstlane(cell.z);
+            const int hq0 = hq & 0xffff, hq1 = hq >> 16;
             if (hq0 == hq1)
                 return;
+            const int lpack = __builtin_amdgcn_readfirstlane(cell.w);
+            const int lx = lpack & 0xff, ly = (lpack >> 8) & 0xff, lz = lpack >> 16;
+            const int hz0 = lz + R2;
             const int cx = o0 + lx, cy = o1 + ly, cz = o2 + lz;
             // this cell's z-run of halo rows in every stencil column (lane = column)
             int qb = 0, len = 0;
             if (lane < ncols) {
-                const int hcol = ((lx + R0 + rw.x) * HY + (ly + R1 + rw.y)) * HZ + hz0;
+                const int hcol = hs + colofs;
                 qb = L.h_off[hcol - rw.z];
                 len = L.h_off[hcol + rw.z + 1] - qb;
             }
@@ -763,9 +774,20 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
             const int total = __builtin_amdgcn_readlane(inc, 63);
             const bool flat = total <= kCandCap;
             if (flat) {
-                const int ex = inc - len;
-                for (int t = 0; t < len; ++t)
-                    L.cand[wid][ex + t] = static_cast<unsigned short>(qb + t);
+                // the column's rows qb, qb+1, ... two to a 32-bit store after an
+                // odd first slot
+                int p = inc - len, v = qb, rem = len;
+                if ((p & 1) && rem > 0) {
+                    L.cand[wid][p] = static_cast<unsigned short>(v);
+                    ++p;
+                    ++v;
+                    --rem;
+                }
+                unsigned* const c32 = reinterpret_cast<unsigned*>(&L.cand[wid][0]);
+                for (; rem >= 2; rem -= 2, p += 2, v += 2)
+                    c32[p >> 1] = static_cast<unsigned>(v) | (static_cast<unsigned>(v + 1) << 16);
+                if (rem > 0)
+                    L.cand[wid][p] = static_cast<unsigned short>(v);
                 if (total + lane < ((total + 63) & ~63)) // kCandCap is a multiple of 64
                     L.cand[wid][total + lane] = 0;
                 __builtin_amdgcn_wave_barrier();
@@ -780,9 +802,10 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
                 const int qb2 = two ? qa + 1 : qa;
                 // -2 X of the two atoms and their thresholds (rc+skin)^2 -/+ band - |X|^2
                 auto uni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
-                const float ax = uni(-2.0f * L.sx[qa]), ay = uni(-2.0f * L.sy[qa]), az = uni(-2.0f * L.sz[qa]);
-                const float bx = uni(-2.0f * L.sx[qb2]), by = uni(-2.0f * L.sy[qb2]), bz = uni(-2.0f * L.sz[qb2]);
-                const float sa = L.ss[qa], sb = L.ss[qb2];
+                const float4 pa = L.sp[qa], pb = L.sp[qb2];
+                const float ax = uni(-2.0f * pa.x), ay = uni(-2.0f * pa.y), az = uni(-2.0f * pa.z);
+                const float bx = uni(-2.0f * pb.x), by = uni(-2.0f * pb.y), bz = uni(-2.0f * pb.z);
+                const float sa = pa.w, sb = pb.w;
                 const float tlo_a = uni(lo_acc - sa), thi_a = uni(hi_rej - sa);
                 const float tlo_b = uni(lo_acc - sb), thi_b = uni(hi_rej - sb);
                 const int lia = li0 + (qa - hq0), lib = li0 + (qb2 - hq0);
@@ -821,10 +844,10 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
                     count += static_cast<int>(__popcll(m));
                 };
                 auto step = [&](int q, unsigned long long vmask) {
-                    const float qx = L.sx[q], qy = L.sy[q], qz = L.sz[q], qss = L.ss[q];
-                    test(qx, qy, qz, qss, q, vmask, ax, ay, az, tlo_a, thi_a, qa, lia, dsta, oa, ca);
+                    const float4 qv = L.sp[q];
+                    test(qv.x, qv.y, qv.z, qv.w, q, vmask, ax, ay, az, tlo_a, thi_a, qa, lia, dsta, oa, ca);
                     if (two)
-                        test(qx, qy, qz, qss, q, vmask, bx, by, bz, tlo_b, thi_b, qb2, lib, dstb, ob, cb);
+                        test(qv.x, qv.y, qv.z, qv.w, q, vmask, bx, by, bz, tlo_b, thi_b, qb2, lib, dstb, ob, cb);
                 };
                 if (flat) {
                     // the candidate slots past `total` up to the next multiple of
@@ -1044,9 +1067,16 @@ namespace atvh {
 // two threads per atom busy in the 512-thread mixed pair kernel (4x5x5 vs
 // 4x4x5 at 2.048M atoms: 141 vs 158 us); the list build then measures the
 // real fill and steps down when the input is denser than uniform.
-static const int kBrickCand[kNumBrickCands][3] = {{4, 5, 5}, {4, 4, 5}, {4, 4, 4}, {3, 4, 4},
-                                                  {3, 3, 4}, {3, 3, 3}, {2, 3, 3}, {2, 2, 3},
-                                                  {2, 2, 2}, {1, 2, 2}, {1, 1, 2}, {1, 1, 1}};
+constexpr int kBrickCand[kNumBrickCands][3] = {{4, 5, 5}, {4, 4, 5}, {4, 4, 4}, {3, 4, 4},
+                                               {3, 3, 4}, {3, 3, 3}, {2, 3, 3}, {2, 2, 3},
+                                               {2, 2, 2}, {1, 2, 2}, {1, 1, 2}, {1, 1, 1}};
+constexpr bool brick_cells_fit() {
+    for (const auto& c : kBrickCand)
+        if (c[0] * c[1] * c[2] > kMaxBrickCells)
+            return false;
+    return true;
+}
+static_assert(brick_cells_fit(), "the list build's LDS cell table holds every brick shape's home cells");
 
 static void choose_bricks(atvh_system* s, int first) {
     const double vol = s->box[0] * s->box[1] * s->box[2];
