@@ -293,7 +293,7 @@ int pme_partial_blocks(const atvh_system* s);
 // true when the PME buffers and transform plans of this precision exist (or
 // no PME is configured), so a step allocates nothing
 bool pme_ready(const atvh_system* s, int precision);
-int pme_grid_blocks(const atvh_system* s);
+int pme_grid_blocks(const atvh_system* s, int precision); // energy partial blocks of one evaluation
 void pme_make_buffers(atvh_system* s, int precision);
 // makes and warms every buffer and plan an evaluation of this precision needs
 void pme_prepare(atvh_system* s, int precision);

@@ -43,6 +43,7 @@
 #include "atvh_internal.hpp"
 #include "brick.hpp"
 #include "pair_common.hpp"
+#include "pme_fft.hpp"
 
 #include <hipfft/hipfft.h>
 
@@ -50,6 +51,7 @@
 #include <atomic>
 #include <climits>
 #include <cmath>
+#include <type_traits>
 #include <complex>
 #include <vector>
 
@@ -88,6 +90,11 @@ struct PmeState {
     double* partials = nullptr;
     int nb = 0;
     hipfftHandle f64 = nullptr, b64 = nullptr, f32 = nullptr, b32 = nullptr;
+    // power-of-two meshes transform in FP32 with this library's own kernels
+    // (pme_fft.hpp; FP64 keeps the FFT library, faster there: 197 vs 227 us
+    // of PME at 128^3): twiddle tables exp(-2 pi i k / K_a), k <= K_a / 2
+    bool native = false;
+    float2* twf[3] = {nullptr, nullptr, nullptr};
 };
 
 } // namespace atvh
@@ -1144,6 +1151,8 @@ void pme_release(PmeState* p) {
     dev_free(p->cplx32);
     dev_free(p->H);
     dev_free(p->partials);
+    for (int d = 0; d < 3; ++d)
+        dev_free(p->twf[d]);
     delete p;
 }
 
@@ -1268,9 +1277,123 @@ void ljc_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
     ATVH_HIP(hipGetLastError());
 }
 
+// The mesh transforms run as this library's kernels when every axis is a
+// power of two the LDS lines and tiles hold (x and y: 16 to 256 points, z:
+// 32 to 512); otherwise through the FFT library.
+bool fft_native(const int K[3]) {
+    auto pow2 = [](int k) { return k > 0 && (k & (k - 1)) == 0; };
+    for (int d = 0; d < 3; ++d)
+        if (!pow2(K[d]))
+            return false;
+    // the x and y tiles hold kFftTile columns: K2/2 and the outer axes are multiples of it
+    const int maxk = 1 << pmefft::kFftMaxLog2, tile = pmefft::kFftTile;
+    return K[0] >= tile && K[0] <= maxk && K[1] >= tile && K[1] <= maxk && K[2] >= 2 * tile && K[2] <= 2 * maxk;
+}
+
+// Whether an evaluation of this precision runs the native transforms.
+bool pme_native(const PmeState* p, int precision) { return p->native && precision == ATVH_PREC_MIXED; }
+
+// exp(-2 pi i k / L) for k = 0 .. L/2, from long double sin and cos.
+template <class T2, class T>
+T2* fft_twiddles(int L) {
+    std::vector<T2> h(L / 2 + 1);
+    const long double two_pi = 2.0L * std::acos(-1.0L);
+    for (int k = 0; k <= L / 2; ++k) {
+        const long double ang = two_pi * static_cast<long double>(k) / static_cast<long double>(L);
+        h[k].x = static_cast<T>(std::cos(ang));
+        h[k].y = static_cast<T>(-std::sin(ang));
+    }
+    T2* d = nullptr;
+    dev_alloc(&d, h.size());
+    ATVH_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(T2), hipMemcpyHostToDevice));
+    return d;
+}
+
+int ilog2(int k) {
+    int l = 0;
+    while ((1 << l) < k)
+        ++l;
+    return l;
+}
+
+#define ATVH_FFT_LOG2_SWITCH(v, CALL)                                                           \
+    switch (v) {                                                                                \
+    case 3: CALL(3); break;                                                                     \
+    case 4: CALL(4); break;                                                                     \
+    case 5: CALL(5); break;                                                                     \
+    case 6: CALL(6); break;                                                                     \
+    case 7: CALL(7); break;                                                                     \
+    case 8: CALL(8); break;                                                                     \
+    default: throw Error(ATVH_E_ARG, "KSpacePME: mesh axis outside the native transform sizes"); \
+    }
+
+// Passes A-E of pme_fft.hpp on the stream: fixed-point mesh in, real
+// potential mesh out, influence applied, energy/virial partials (ev) in
+// gm blocks.
+template <class T>
+void fft_native_enqueue(atvh_system* s, const PmeArgs& pa, bool ev, int gm, hipStream_t st) {
+    using namespace pmefft;
+    PmeState* p = s->pme;
+    const bool f32 = std::is_same<T, float>::value;
+    FftArgs f{};
+    f.K0 = p->K[0];
+    f.K1 = p->K[1];
+    f.K2 = p->K[2];
+    f.Kh = p->Kh;
+    f.ks = fft_row(p->Kh);
+    f.inv_fix = pa.inv_fix;
+    f.fix = p->fix;
+    f.real = f32 ? static_cast<void*>(p->real32) : static_cast<void*>(p->real64);
+    f.cplx = f32 ? static_cast<void*>(p->cplx32) : static_cast<void*>(p->cplx64);
+    static_assert(std::is_same<T, float>::value, "the native transforms run in FP32");
+    f.tw0 = p->twf[0];
+    f.tw1 = p->twf[1];
+    f.tw2 = p->twf[2];
+    f.H = p->H;
+    f.ic0 = pa.ic0;
+    f.ic1 = pa.ic1;
+    f.ic2 = pa.ic2;
+    f.pi2_a2 = pa.pi2_a2;
+    f.partials = p->partials;
+    const int l0 = ilog2(p->K[0]), l1 = ilog2(p->K[1]), lm = ilog2(p->K[2]) - 1;
+    // passes A and E: lines of M = K2/2 points, min(64, M/2) lanes each
+    const int M = p->K[2] / 2, nt = std::min(64, M / 2), lines_per_wg = (kFftThreads / 64) * (64 / nt);
+    const int nlines = p->K[0] * p->K[1];
+    const int ga = std::min((nlines + lines_per_wg - 1) / lines_per_wg, kFftLineGrid);
+    // passes B and D: in FP32, 512 workgroups (two per CU) stride over the
+    // tiles, each loading its next tile while the current one transforms; in
+    // FP64 one workgroup per tile (see kFftPrefetch)
+    const int ntiles_y = p->K[0] * ((p->Kh - 1) / kFftTile) + p->K[0] / kFftTile;
+    const int gb = std::min(ntiles_y, f32 ? kFftTileGrid : kFftLineGrid);
+#define ATVH_FFT_A(L) k_fft_r2c_z<T, L><<<ga, kFftThreads, 0, st>>>(f)
+#define ATVH_FFT_B(L) k_fft_y<T, L, false><<<gb, kFftTileThreads, 0, st>>>(f)
+#define ATVH_FFT_C(L)                                                    \
+    do {                                                                 \
+        if (ev)                                                          \
+            k_fft_x_influence<T, L, true><<<gm, kFftTileThreads, 0, st>>>(f); \
+        else                                                             \
+            k_fft_x_influence<T, L, false><<<gm, kFftTileThreads, 0, st>>>(f); \
+    } while (0)
+#define ATVH_FFT_D(L) k_fft_y<T, L, true><<<gb, kFftTileThreads, 0, st>>>(f)
+#define ATVH_FFT_E(L) k_fft_c2r_z<T, L><<<ga, kFftThreads, 0, st>>>(f)
+    ATVH_FFT_LOG2_SWITCH(lm, ATVH_FFT_A)
+    ATVH_FFT_LOG2_SWITCH(l1, ATVH_FFT_B)
+    ATVH_FFT_LOG2_SWITCH(l0, ATVH_FFT_C)
+    ATVH_FFT_LOG2_SWITCH(l1, ATVH_FFT_D)
+    ATVH_FFT_LOG2_SWITCH(lm, ATVH_FFT_E)
+#undef ATVH_FFT_A
+#undef ATVH_FFT_B
+#undef ATVH_FFT_C
+#undef ATVH_FFT_D
+#undef ATVH_FFT_E
+    ATVH_HIP(hipGetLastError());
+}
+
 // Blocks (and fixed-order energy partials) of the influence kernel.
-int pme_grid_blocks(const atvh_system* s) {
+int pme_grid_blocks(const atvh_system* s, int precision) {
     const PmeState* p = s->pme;
+    if (pme_native(p, precision)) // pass C: one workgroup per tile, grid-stride beyond the partials' cap
+        return std::min(p->K[1] * ((p->Kh - 1) / pmefft::kFftTile) + p->K[1] / pmefft::kFftTile, kPmeGridCap);
     const int64_t nmodes = (int64_t)p->K[0] * p->K[1] * p->Kh;
     return std::min(grid_for(nmodes), kPmeGridCap);
 }
@@ -1280,6 +1403,15 @@ void pme_make_buffers(atvh_system* s, int precision) {
     PmeState* p = s->pme;
     const int64_t ntot = (int64_t)p->K[0] * p->K[1] * p->K[2];
     const int64_t nmodes = (int64_t)p->K[0] * p->K[1] * p->Kh;
+    if (pme_native(p, precision)) {
+        if (!p->real32) {
+            dev_alloc(&p->real32, ntot);
+            dev_alloc(&p->cplx32, (int64_t)p->K[0] * p->K[1] * pmefft::fft_row(p->Kh));
+            for (int d = 0; d < 3; ++d)
+                p->twf[d] = fft_twiddles<float2, float>(p->K[d]);
+        }
+        return;
+    }
     if (precision == ATVH_PREC_MIXED && !p->real32) {
         dev_alloc(&p->real32, ntot);
         dev_alloc(&p->cplx32, nmodes);
@@ -1301,8 +1433,18 @@ void pme_make_buffers(atvh_system* s, int precision) {
 // untouched. Synchronises.
 void pme_prepare(atvh_system* s, int precision) {
     PmeState* p = s->pme;
-    if (!p || pme_ready(s, precision))
+    if (!p)
         return;
+    // the energy partials an evaluation of this precision leaves (the count
+    // differs between the native and the library transforms)
+    p->nb = pme_grid_blocks(s, precision);
+    if (pme_ready(s, precision))
+        return;
+    if (pme_native(p, precision)) { // nothing to load: the transforms are this library's kernels
+        pme_make_buffers(s, precision);
+        p->nb = pme_grid_blocks(s, precision);
+        return;
+    }
     size_t free_before = 0, free_after = 0, total_mem = 0;
     ATVH_HIP(hipMemGetInfo(&free_before, &total_mem));
     pme_make_buffers(s, precision);
@@ -1322,7 +1464,7 @@ void pme_prepare(atvh_system* s, int precision) {
     ATVH_HIP(hipMemGetInfo(&free_after, &total_mem));
     if (free_before > free_after)
         note_fft_pair_bytes(static_cast<int64_t>(free_before - free_after));
-    p->nb = pme_grid_blocks(s);
+    p->nb = pme_grid_blocks(s, precision);
 }
 
 void pme_enqueue(atvh_system* s, int precision, bool ev) {
@@ -1354,8 +1496,13 @@ void pme_enqueue(atvh_system* s, int precision, bool ev) {
         sr.cap = kSpreadCap;
     }
     const int gg = grid_capped(ntot);
-    const int gm = pme_grid_blocks(s);
-    if (mixed) {
+    const int gm = pme_grid_blocks(s, precision);
+    const bool native = pme_native(p, precision);
+    if (native) {
+        dispatch_order<float>(p->order, true, st, a, nullptr, sr);
+        fft_native_enqueue<float>(s, a, ev, gm, st);
+        dispatch_order<float>(p->order, false, st, a, p->real32, sr);
+    } else if (mixed) {
         dispatch_order<float>(p->order, true, st, a, nullptr, sr);
         k_pme_unfix<float><<<gg, kBlock, 0, st>>>(ntot, p->fix, p->real32, a.inv_fix);
         HIPFFT_CHECK(hipfftSetStream(p->f32, st));
@@ -1380,7 +1527,7 @@ void pme_enqueue(atvh_system* s, int precision, bool ev) {
         HIPFFT_CHECK(hipfftExecZ2D(p->b64, reinterpret_cast<hipfftDoubleComplex*>(p->cplx64), p->real64));
         dispatch_order<double>(p->order, false, st, a, p->real64, sr);
     }
-    if (fresh) {
+    if (fresh && !native) {
         size_t free_after = 0;
         ATVH_HIP(hipMemGetInfo(&free_after, &total_mem));
         if (free_before > free_after)
@@ -1407,8 +1554,13 @@ int64_t pme_device_bytes(const atvh_system* s) {
     int64_t b = ntot * 8 + nmodes * 8 + int64_t(kPmeGridCap) * kPartialStride * 8; // fix, H, partials
     if (p->real64)
         b += ntot * 8 + nmodes * 16 + fft_pair_bytes();
-    if (p->real32)
-        b += ntot * 4 + nmodes * 8 + fft_pair_bytes();
+    if (p->real32) {
+        if (p->native) // padded rows and three twiddle tables, no plans
+            b += ntot * 4 + int64_t(p->K[0]) * p->K[1] * pmefft::fft_row(p->Kh) * 8 +
+                 int64_t(p->K[0] + p->K[1] + p->K[2] + 3) * 8;
+        else
+            b += ntot * 4 + nmodes * 8 + fft_pair_bytes();
+    }
     return b;
 }
 bool pme_ready(const atvh_system* s, int precision) {
@@ -1593,6 +1745,7 @@ int atvh_pme_configure(atvh_system* s, double alpha, double spacing, int order) 
             p->ic[d] = 1.0 / s->box[d];
         }
         p->Kh = p->K[2] / 2 + 1;
+        p->native = fft_native(p->K);
         p->vol = s->box[0] * s->box[1] * s->box[2];
         const int64_t ntot = (int64_t)p->K[0] * p->K[1] * p->K[2];
         const int64_t nmodes = (int64_t)p->K[0] * p->K[1] * p->Kh;

@@ -424,7 +424,7 @@ void integ_batch_build(atvh_batch* b) {
         f.escale = f.e_self = f.e_net = 0.0;
         if (s->pme) { // the k-space energy and virial partials of the member's PME call
             f.kpartials = pme_partials(s);
-            f.knb = pme_grid_blocks(s);
+            f.knb = pme_partial_blocks(s); // set for this run's precision by pme_prepare
             pme_energy_consts(s, f.escale, f.e_self, f.e_net);
         }
         f.record_pe = s->d_thermo;

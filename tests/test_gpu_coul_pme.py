@@ -4,8 +4,9 @@ PairLJCutCoulLong (LJ + tabulated erfc Coulomb) and KSpacePME.
 Bars (each written in its test):
   * FP64 pair forces: rounding only, max |dF| <= 1e-12 max |F|;
   * FP64 PME: the oracle transforms with its own DFT and spreads in FP64 in
-    atom order, the device with hipFFT and an exact fixed-point grid, so the
-    bar is 1e-9 of max |F| for forces and 1e-9 relative for energies;
+    atom order, the device with an exact fixed-point grid and hipFFT, so the
+    bar is 1e-9 of max |F| for forces and 1e-9 relative for energies (FP32
+    transforms of power-of-two meshes are the device's own kernels);
   * mixed precision (FP32 pair / spread / FFT / gather, FP64 accumulation):
     per-atom force RMS relative error <= 1e-5 against the FP64 oracle;
   * physics: the NaCl Madelung energy within 5e-4 (test_pme_coul.cpp:157-175).
@@ -108,6 +109,50 @@ def test_pme_matches_oracle(hip_ctx, orc, order):
     pe_m, vir_m, _ = s.energy()
     assert pe_m == pytest.approx(e, rel=1e-5)
     assert np.linalg.norm(vir_m - ovir) <= 1e-5 * np.linalg.norm(ovir)
+
+
+@pytest.mark.parametrize("K,order,stretch", [(32, 4, 1), (32, 5, 1), (32, 8, 1), (64, 6, 1), (32, 4, 2)])
+def test_native_transform_meshes_match_oracle(hip_ctx, orc, K, order, stretch):
+    """Power-of-two meshes transform in FP32 with the library's own kernels
+    (five passes over the half spectrum, the influence function applied
+    between the forward and inverse x-lines; FP64 stays on hipFFT); the bars
+    are those of the other meshes above. stretch 2 doubles the box along z,
+    so the mesh is K x K x 2K and every axis runs a different line length.
+    The run switches precision back and forth, so the energy partials of
+    both transform paths are read with their own block counts."""
+    w = snapshot(12, seed=40 + order)
+    w.pme_order = order
+    if stretch != 1:
+        w.box = np.array([w.box[0], w.box[1], stretch * w.box[2]])
+        w.z = w.z * stretch
+    w.pme_spacing = w.box[0] / K * (1.0 + 1e-9)
+    s = make(hip_ctx, w)
+    Kd, _ = s.pme_info()
+    Ko = orc.pme_grid(w.box, w.pme_spacing, order)
+    np.testing.assert_array_equal(Kd, Ko)
+    np.testing.assert_array_equal(Kd, [K, K, stretch * K])
+    gx, gy, gz, e, ovir, parts = orc.pme(w.x, w.y, w.z, w.charge, w.box, w.alpha, Ko, order)
+    F = np.stack([gx, gy, gz])
+    s.pme_compute(av.PREC_FP64, True, True)
+    d = s.download()
+    pe, vir, _ = s.energy()
+    _, gparts = s.pme_info()
+    assert np.max(np.abs(forces(d) - F)) <= 1e-9 * np.max(np.abs(F))
+    assert pe == pytest.approx(e, rel=1e-9)
+    assert gparts[0] == pytest.approx(parts[0], rel=1e-9)
+    np.testing.assert_allclose(vir, ovir, rtol=1e-8, atol=1e-9 * np.abs(ovir[:3]).max())
+    s.pme_compute(av.PREC_MIXED, True, True)
+    m = s.download()
+    assert rms_rel(forces(m), F) <= 1e-5
+    pe_m, vir_m, _ = s.energy()
+    assert pe_m == pytest.approx(e, rel=1e-5)
+    assert np.linalg.norm(vir_m - ovir) <= 1e-5 * np.linalg.norm(ovir)
+    # evaluations repeat bit for bit, also after the other precision ran
+    s.pme_compute(av.PREC_FP64, True, True)
+    assert s.energy()[0] == pe
+    s.pme_compute(av.PREC_MIXED, True, True)
+    np.testing.assert_array_equal(forces(s.download()), forces(m))
+    assert s.energy()[0] == pe_m
 
 
 def test_pme_is_bitwise_reproducible(hip_ctx):
