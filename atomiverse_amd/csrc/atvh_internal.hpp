@@ -227,6 +227,7 @@ struct atvh_batch {
     int thermo_cap = 0;
     int* d_slot_member = nullptr; // replica exchange: member holding ladder slot k
     int* d_accept = nullptr;      // accepted exchanges of the last round
+    int* d_halted = nullptr;      // [nsys] members a rebuild overflow halted in the last run
 };
 
 namespace atvh {

@@ -23,6 +23,13 @@ using namespace atvh;
 
 namespace {
 
+// Which members a rebuild overflow halted (read back in one copy).
+__global__ void k_batch_halted(DevCtl* const* ctls, int nsys, int* halted_out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nsys)
+        halted_out[k] = halted(ctls[k]) ? 1 : 0;
+}
+
 __global__ void k_batch_reset(DevCtl** ctls, int nsys) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < nsys) {
@@ -111,8 +118,9 @@ void build_stages(atvh_batch* b, int nsteps) {
             dev_alloc(&s->d_thermo, size_t(2) * nsteps);
         }
         s->thermo_n = nsteps;
-        std::vector<double> nanv(size_t(2) * s->thermo_cap, std::nan(""));
-        ATVH_HIP(hipMemcpy(s->d_thermo, nanv.data(), nanv.size() * sizeof(double), hipMemcpyHostToDevice));
+        // all-ones bytes are a NaN double (queued, not captured: the step
+        // graphs are made after this)
+        ATVH_HIP(hipMemsetAsync(s->d_thermo, 0xff, size_t(2) * s->thermo_cap * sizeof(double), b->ctx->stream));
     }
     batch_free_stages(b);
     free_graphs(b);
@@ -181,10 +189,12 @@ int atvh_batch_create(atvh_ctx* ctx, atvh_system* const* systems, int nsys, atvh
         dev_alloc(&b->d_slot_member, slots.size());
         ATVH_HIP(hipMemcpy(b->d_slot_member, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice));
         dev_alloc(&b->d_accept, 1);
+        dev_alloc(&b->d_halted, ctls.size());
     } catch (...) {
         dev_free(b->d_ctls);
         dev_free(b->d_slot_member);
         dev_free(b->d_accept);
+        dev_free(b->d_halted);
         delete b;
         throw;
     }
@@ -205,6 +215,7 @@ int atvh_batch_destroy(atvh_batch* b) {
     dev_free(b->d_ctls);
     dev_free(b->d_slot_member);
     dev_free(b->d_accept);
+    dev_free(b->d_halted);
     c->live_systems -= 1;
     delete b;
     return ATVH_OK;
@@ -291,9 +302,14 @@ int atvh_batch_run(atvh_batch* b, int nsteps, int precision, int energy_every) {
     // a member whose rebuild overflowed its list capacity or a brick halo
     // halted at that step; it regrows its list and finishes the run alone
     // (bit-identical to the batched step), the others are done
+    k_batch_halted<<<(nsys + kBlock - 1) / kBlock, kBlock, 0, st>>>(b->d_ctls, nsys, b->d_halted);
+    ATVH_HIP(hipGetLastError());
+    std::vector<int> halted_h(nsys);
+    ATVH_HIP(hipMemcpyAsync(halted_h.data(), b->d_halted, nsys * sizeof(int), hipMemcpyDeviceToHost, st));
     ATVH_HIP(hipStreamSynchronize(st));
-    for (atvh_system* s : b->sys)
-        md_recover_and_finish(s, nsteps, precision, energy_every);
+    for (int k = 0; k < nsys; ++k)
+        if (halted_h[k])
+            md_recover_and_finish(b->sys[k], nsteps, precision, energy_every);
     return ATVH_OK;
     API_CATCH(c)
 }
