@@ -1,6 +1,7 @@
 // atvh_internal.hpp — shared definitions of libatv_hip.so (gfx950).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -251,6 +252,23 @@ struct ScopedTimer {
     ScopedTimer(atvh_ctx* c, const char* nm);
     ~ScopedTimer();
 };
+hipEvent_t take_event(atvh_ctx* c);
+bool timing_on(const atvh_ctx* c, const char* nm);
+
+// One kernel launch on stream st; when the context times the family `name`,
+// the kernel itself carries the start and stop events (hipExtLaunchKernelGGL),
+// so the interval is the kernel's own and no marker is queued beside it.
+template <class... KArgs, class... Args>
+void launch_timed(atvh_ctx* c, const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 block,
+                  hipStream_t st, Args... args) {
+    if (!timing_on(c, name)) {
+        hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+        return;
+    }
+    hipEvent_t start = take_event(c), stop = take_event(c);
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, st, start, stop, 0, args...);
+    c->timers[name].pending.emplace_back(start, stop);
+}
 
 template <class T>
 void dev_alloc(T** p, size_t count) {

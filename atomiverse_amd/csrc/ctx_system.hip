@@ -13,7 +13,7 @@ using namespace atvh;
 
 namespace atvh {
 
-static hipEvent_t take_event(atvh_ctx* c) {
+hipEvent_t take_event(atvh_ctx* c) {
     if (!c->event_pool.empty()) {
         hipEvent_t e = c->event_pool.back();
         c->event_pool.pop_back();
@@ -24,8 +24,12 @@ static hipEvent_t take_event(atvh_ctx* c) {
     return e;
 }
 
+bool timing_on(const atvh_ctx* c, const char* nm) {
+    return c->timing && (c->timing_only.empty() || c->timing_only == nm);
+}
+
 ScopedTimer::ScopedTimer(atvh_ctx* c, const char* nm) : ctx(c), name(nm) {
-    if (ctx->timing && (ctx->timing_only.empty() || ctx->timing_only == nm)) {
+    if (timing_on(ctx, nm)) {
         start = take_event(ctx);
         ATVH_HIP(hipEventRecord(start, ctx->stream));
     }

@@ -384,10 +384,13 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
     const double4* __restrict__ tab = a.tab;
     const int ntypes = a.ntypes;
     const uint4* __restrict__ base = reinterpret_cast<const uint4*>(a.nbr16 + (int64_t)a0 * a.cap);
-    // the cutoff decision inside the FP32 rounding band is the plain kernel's
-    // FP64 test on the same images (arguments copied to registers: the lambda
-    // must not take the address of the argument block)
-    const float band = a.band;
+    // the cutoff decision inside the FP32 rounding band [rc^2 - band, rc^2 +
+    // band) is the plain kernel's FP64 test on the same images: the main loop
+    // counts the pairs below the band and the fix-up below adds the band pairs
+    // the FP64 test accepts, so a rejected pair is never touched (arguments
+    // copied to registers: the lambda must not take the address of the
+    // argument block)
+    const float rc_hi = rc_sq + a.band, rc_lo = rc_sq - a.band;
     const double rc_sq_fp64 = a.rc_sq;
     const double* __restrict__ gx = a.x;
     const double* __restrict__ gy = a.y;
@@ -427,8 +430,8 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
                 const float dy = pi.y - pj[u].y;
                 const float dz = pi.z - pj[u].z;
                 const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                const bool in = r2 < rc_sq; // padding rows are the far sentinel
-                near = near || __builtin_fabsf(r2 - rc_sq) <= band;
+                const bool in = r2 < rc_lo; // padding rows are the far sentinel
+                near = near || (!in && r2 < rc_hi);
                 float eps4 = eps4s, shift = shifts;
                 const float r2inv = __builtin_amdgcn_rcpf(r2);
                 float r6inv, fpair;
@@ -482,9 +485,8 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
         if (near) {
             // Rare: walk this thread's groups again and, for each band pair,
             // take the plain kernel's FP64 test on the same images; where it
-            // disagrees with the FP32 decision, add (or remove) the pair's
-            // FP32 term. Run after the main loop, its FP64 registers are never
-            // live beside the main loop's.
+            // accepts the pair, add the pair's FP32 term. Run after the main
+            // loop, its FP64 registers are never live beside the main loop's.
             int gsi;
             double xi, yi, zi;
             row_image(gx, gy, gz, Lx, Ly, Lz, rows[qs], gsi, xi, yi, zi);
@@ -499,16 +501,15 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
                     const float dy = pi.y - pj.y;
                     const float dz = pi.z - pj.z;
                     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                    if (!(__builtin_fabsf(r2 - rc_sq) <= band))
+                    if (!(r2 >= rc_lo && r2 < rc_hi))
                         continue;
                     int gsj;
                     double xj, yj, zj;
                     row_image(gx, gy, gz, Lx, Ly, Lz, rows[q], gsj, xj, yj, zj);
                     const double ddx = xi - xj, ddy = yi - yj, ddz = zi - zj;
                     const bool exact = fma(ddz, ddz, fma(ddy, ddy, ddx * ddx)) < rc_sq_fp64;
-                    if (exact == (r2 < rc_sq))
+                    if (!exact)
                         continue;
-                    const float sgn = exact ? 1.0f : -1.0f;
                     float eps24 = 1.0f, eps4 = eps4s, shift = shifts, sig6 = 1.0f;
                     if (TYPED) {
                         const double4 t = tab[ti * ntypes + __float_as_int(pj.w)];
@@ -521,19 +522,19 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
                     const float r6inv = r2inv * r2inv * r2inv * sig6;
                     const float fpair = TYPED ? eps24 * r2inv * __builtin_fmaf(2.0f * r6inv, r6inv, -r6inv)
                                               : r2inv * (r6inv * __builtin_fmaf(2.0f, r6inv, -1.0f));
-                    fxi = __builtin_fmaf(sgn * fpair, dx, fxi);
-                    fyi = __builtin_fmaf(sgn * fpair, dy, fyi);
-                    fzi = __builtin_fmaf(sgn * fpair, dz, fzi);
+                    fxi = __builtin_fmaf(fpair, dx, fxi);
+                    fyi = __builtin_fmaf(fpair, dy, fyi);
+                    fzi = __builtin_fmaf(fpair, dz, fzi);
                     if (EF)
-                        acc[0] += sgn * static_cast<double>(__builtin_fmaf(eps4, r6inv * r6inv - r6inv, -shift));
+                        acc[0] += static_cast<double>(__builtin_fmaf(eps4, r6inv * r6inv - r6inv, -shift));
                     if (VF) {
                         const float fxp = fpair * dx, fyp = fpair * dy, fzp = fpair * dz;
-                        acc[1] += sgn * static_cast<double>(dx * fxp);
-                        acc[2] += sgn * static_cast<double>(dy * fyp);
-                        acc[3] += sgn * static_cast<double>(dz * fzp);
-                        acc[4] += sgn * static_cast<double>(dx * fyp);
-                        acc[5] += sgn * static_cast<double>(dx * fzp);
-                        acc[6] += sgn * static_cast<double>(dy * fzp);
+                        acc[1] += static_cast<double>(dx * fxp);
+                        acc[2] += static_cast<double>(dy * fyp);
+                        acc[3] += static_cast<double>(dz * fzp);
+                        acc[4] += static_cast<double>(dx * fyp);
+                        acc[5] += static_cast<double>(dx * fzp);
+                        acc[6] += static_cast<double>(dy * fzp);
                     }
                 }
             }
@@ -559,26 +560,26 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
     }
 }
 
-#define LJ_BRICK_DISPATCH(OP, EF, VF, TY, GRID, ST, ARGS)                                      \
-    do {                                                                                      \
-        if (EF) {                                                                             \
-            if (VF) {                                                                         \
-                if (TY) k_single<OP<true, true, true>><<<GRID, op_threads<OP<true, true, true>>::value, 0, ST>>>(ARGS);        \
-                else k_single<OP<true, true, false>><<<GRID, op_threads<OP<true, true, false>>::value, 0, ST>>>(ARGS);          \
-            } else {                                                                          \
-                if (TY) k_single<OP<true, false, true>><<<GRID, op_threads<OP<true, false, true>>::value, 0, ST>>>(ARGS);       \
-                else k_single<OP<true, false, false>><<<GRID, op_threads<OP<true, false, false>>::value, 0, ST>>>(ARGS);         \
-            }                                                                                 \
-        } else {                                                                              \
-            if (VF) {                                                                         \
-                if (TY) k_single<OP<false, true, true>><<<GRID, op_threads<OP<false, true, true>>::value, 0, ST>>>(ARGS);       \
-                else k_single<OP<false, true, false>><<<GRID, op_threads<OP<false, true, false>>::value, 0, ST>>>(ARGS);         \
-            } else {                                                                          \
-                if (TY) k_single<OP<false, false, true>><<<GRID, op_threads<OP<false, false, true>>::value, 0, ST>>>(ARGS);      \
-                else k_single<OP<false, false, false>><<<GRID, op_threads<OP<false, false, false>>::value, 0, ST>>>(ARGS);        \
-            }                                                                                 \
-        }                                                                                     \
-    } while (0)
+// The brick kernels, launched through launch_timed (family "pair").
+template <class OpT>
+static void lj_brick_launch(atvh_ctx* c, int grid, hipStream_t st, const BrickLJArgs& b) {
+    launch_timed(c, "pair", k_single<OpT>, dim3(grid), dim3(op_threads<OpT>::value), st, b);
+}
+template <template <bool, bool, bool> class OP>
+static void lj_brick_dispatch(atvh_ctx* c, bool ef, bool vf, bool ty, int grid, hipStream_t st,
+                              const BrickLJArgs& b) {
+    if (ef) {
+        if (vf)
+            ty ? lj_brick_launch<OP<true, true, true>>(c, grid, st, b) : lj_brick_launch<OP<true, true, false>>(c, grid, st, b);
+        else
+            ty ? lj_brick_launch<OP<true, false, true>>(c, grid, st, b) : lj_brick_launch<OP<true, false, false>>(c, grid, st, b);
+    } else {
+        if (vf)
+            ty ? lj_brick_launch<OP<false, true, true>>(c, grid, st, b) : lj_brick_launch<OP<false, true, false>>(c, grid, st, b);
+        else
+            ty ? lj_brick_launch<OP<false, false, true>>(c, grid, st, b) : lj_brick_launch<OP<false, false, false>>(c, grid, st, b);
+    }
+}
 
 #define LJ_DISPATCH(KERNEL, EF, VF, TY, GRID, ST, N, ARGS)                                   \
     do {                                                                                      \
@@ -619,12 +620,11 @@ void pair_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
         throw Error(ATVH_E_STATE, "pair style: no neighbor list (add a NeighborList and build it)");
     if (s->lj_cutoff > s->cutoff + 1e-12)
         throw Error(ATVH_E_ARG, "pair style: pair cutoff exceeds the neighbor-list cutoff");
-    {
+    if (s->pair_style == 1) {
         ScopedTimer tm(s->ctx, "pair");
-        if (s->pair_style == 1)
-            ljc_enqueue(s, precision, eflag, vflag);
-        else
-            lj_enqueue(s, precision, eflag, vflag);
+        ljc_enqueue(s, precision, eflag, vflag);
+    } else {
+        lj_enqueue(s, precision, eflag, vflag); // times its own launch
     }
     if (s->pme)
         pme_enqueue(s, precision, eflag || vflag);
@@ -726,13 +726,15 @@ static void lj_enqueue(atvh_system* s, int precision, bool eflag, bool vflag) {
     hipStream_t st = s->ctx->stream;
     if (s->brick_mode) {
         const BrickLJArgs b = brick_lj_args(s, a);
+        atvh_ctx* c = s->ctx;
         if (precision == ATVH_PREC_MIXED)
-            LJ_BRICK_DISPATCH(OpLjBrickMixed, eflag, vflag, typed, g, st, b);
+            lj_brick_dispatch<OpLjBrickMixed>(c, eflag, vflag, typed, g, st, b);
         else
-            LJ_BRICK_DISPATCH(OpLjBrickFp64, eflag, vflag, typed, g, st, b);
+            lj_brick_dispatch<OpLjBrickFp64>(c, eflag, vflag, typed, g, st, b);
         ATVH_HIP(hipGetLastError());
         return;
     }
+    ScopedTimer tm(s->ctx, "pair");
     if (precision == ATVH_PREC_MIXED)
         LJ_DISPATCH(k_lj_mixed, eflag, vflag, typed, g, st, s->n, a);
     else
