@@ -15,6 +15,7 @@
 
 #include "atvh_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -22,6 +23,12 @@
 using namespace atvh;
 
 namespace {
+
+// A member's thermo record cleared to NaN (no energy asked at that step).
+__global__ void k_fill_nan(double* p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = __builtin_nan("");
+}
 
 // Which members a rebuild overflow halted (read back in one copy).
 __global__ void k_batch_halted(DevCtl* const* ctls, int nsys, int* halted_out) {
@@ -118,9 +125,10 @@ void build_stages(atvh_batch* b, int nsteps) {
             dev_alloc(&s->d_thermo, size_t(2) * nsteps);
         }
         s->thermo_n = nsteps;
-        // all-ones bytes are a NaN double (queued, not captured: the step
-        // graphs are made after this)
-        ATVH_HIP(hipMemsetAsync(s->d_thermo, 0xff, size_t(2) * s->thermo_cap * sizeof(double), b->ctx->stream));
+        const int64_t nrec = int64_t(2) * s->thermo_cap;
+        k_fill_nan<<<static_cast<int>(std::min<int64_t>((nrec + kBlock - 1) / kBlock, 1024)), kBlock, 0, b->ctx->stream>>>(
+            s->d_thermo, nrec);
+        ATVH_HIP(hipGetLastError());
     }
     batch_free_stages(b);
     free_graphs(b);
@@ -305,8 +313,8 @@ int atvh_batch_run(atvh_batch* b, int nsteps, int precision, int energy_every) {
     k_batch_halted<<<(nsys + kBlock - 1) / kBlock, kBlock, 0, st>>>(b->d_ctls, nsys, b->d_halted);
     ATVH_HIP(hipGetLastError());
     std::vector<int> halted_h(nsys);
-    ATVH_HIP(hipMemcpyAsync(halted_h.data(), b->d_halted, nsys * sizeof(int), hipMemcpyDeviceToHost, st));
     ATVH_HIP(hipStreamSynchronize(st));
+    ATVH_HIP(hipMemcpy(halted_h.data(), b->d_halted, nsys * sizeof(int), hipMemcpyDeviceToHost));
     for (int k = 0; k < nsys; ++k)
         if (halted_h[k])
             md_recover_and_finish(b->sys[k], nsteps, precision, energy_every);
