@@ -384,13 +384,13 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
     const double4* __restrict__ tab = a.tab;
     const int ntypes = a.ntypes;
     const uint4* __restrict__ base = reinterpret_cast<const uint4*>(a.nbr16 + (int64_t)a0 * a.cap);
-    // the cutoff decision inside the FP32 rounding band [rc^2 - band, rc^2 +
-    // band) is the plain kernel's FP64 test on the same images: the main loop
+    // the cutoff decision inside the FP32 rounding band |r^2 - rc^2| <= band
+    // is the plain kernel's FP64 test on the same images: the main loop
     // counts the pairs below the band and the fix-up below adds the band pairs
     // the FP64 test accepts, so a rejected pair is never touched (arguments
     // copied to registers: the lambda must not take the address of the
     // argument block)
-    const float rc_hi = rc_sq + a.band, rc_lo = rc_sq - a.band;
+    const float band = a.band, rc_lo = rc_sq - a.band;
     const double rc_sq_fp64 = a.rc_sq;
     const double* __restrict__ gx = a.x;
     const double* __restrict__ gy = a.y;
@@ -431,7 +431,7 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
                 const float dz = pi.z - pj[u].z;
                 const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                 const bool in = r2 < rc_lo; // padding rows are the far sentinel
-                near = near || (!in && r2 < rc_hi);
+                near = near || __builtin_fabsf(r2 - rc_sq) <= band;
                 float eps4 = eps4s, shift = shifts;
                 const float r2inv = __builtin_amdgcn_rcpf(r2);
                 float r6inv, fpair;
@@ -501,7 +501,7 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
                     const float dy = pi.y - pj.y;
                     const float dz = pi.z - pj.z;
                     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                    if (!(r2 >= rc_lo && r2 < rc_hi))
+                    if (!(__builtin_fabsf(r2 - rc_sq) <= band))
                         continue;
                     int gsj;
                     double xj, yj, zj;
