@@ -265,7 +265,7 @@ __device__ void OpLjBrickFp64<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid,
         for (int g = h; g < ng; g += tpa) {
             const uint4 nx2 = grp[__umul24(min(g + 2 * tpa, gl), nat)];
             // the whole group's LDS reads issue before any arithmetic; padding
-            // rows (the atom itself, r^2 == 0) and pairs beyond r_c add zero
+            // rows (the far sentinel) and pairs beyond r_c add zero
             const unsigned w[4] = {cur.x, cur.y, cur.z, cur.w};
             double jx[kGroup], jy[kGroup], jz[kGroup];
             int tj[kGroup];
@@ -384,13 +384,14 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
     const double4* __restrict__ tab = a.tab;
     const int ntypes = a.ntypes;
     const uint4* __restrict__ base = reinterpret_cast<const uint4*>(a.nbr16 + (int64_t)a0 * a.cap);
-    // the cutoff decision inside the FP32 rounding band |r^2 - rc^2| <= band
-    // is the plain kernel's FP64 test on the same images: the main loop
-    // counts the pairs below the band and the fix-up below adds the band pairs
-    // the FP64 test accepts, so a rejected pair is never touched (arguments
-    // copied to registers: the lambda must not take the address of the
-    // argument block)
-    const float band = a.band, rc_lo = rc_sq - a.band;
+    // the cutoff decision inside the FP32 rounding band [rc_lo, rc_hi) around
+    // rc^2 is the plain kernel's FP64 test on the same images: the main loop
+    // counts the pairs below rc_lo and the fix-up below adds the band pairs the
+    // FP64 test accepts, so a rejected pair is never touched. Both sides use
+    // the same two compares, so every pair below rc_hi is taken exactly once.
+    // (Arguments copied to registers: the lambda must not take the address of
+    // the argument block.)
+    const float rc_lo = rc_sq - a.band, rc_hi = rc_sq + a.band;
     const double rc_sq_fp64 = a.rc_sq;
     const double* __restrict__ gx = a.x;
     const double* __restrict__ gy = a.y;
@@ -410,7 +411,7 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
         const uint4* __restrict__ grp = base + li;
         const int ng = (nn + kGroup - 1) / kGroup;
         float fxi = 0.f, fyi = 0.f, fzi = 0.f;
-        bool near = false; // some pair's FP32 r^2 lies within the rounding band of rc^2
+        bool near = false; // some pair's FP32 r^2 lies in [rc_lo, rc_hi)
         // group indices clamp to the last group (to group 0 when nn == 0: the
         // brick's rows always exist); loads run two groups ahead in three
         // rotating slots, so no group is moved between registers
@@ -418,7 +419,7 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
         auto load = [&](int g) { return grp[__umul24(min(g, gl), nat)]; };
         auto group = [&](const uint4 cur) {
             // the whole group's LDS reads issue before any arithmetic; padding
-            // rows (the atom itself, r^2 == 0) and pairs beyond r_c add zero
+            // rows (the far sentinel) and pairs beyond r_c add zero
             const unsigned w[4] = {cur.x, cur.y, cur.z, cur.w};
             float4 pj[kGroup];
 #pragma unroll
@@ -431,7 +432,9 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
                 const float dz = pi.z - pj[u].z;
                 const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                 const bool in = r2 < rc_lo; // padding rows are the far sentinel
-                near = near || __builtin_fabsf(r2 - rc_sq) <= band;
+                // rc_lo < rc_hi, so the band is (r2 < rc_hi) xor in: one more
+                // compare, the lane masks combine on the scalar unit
+                near = near || ((r2 < rc_hi) != in);
                 float eps4 = eps4s, shift = shifts;
                 const float r2inv = __builtin_amdgcn_rcpf(r2);
                 float r6inv, fpair;
@@ -501,7 +504,7 @@ __device__ void OpLjBrickMixed<EF, VF, TYPED>::run(const BrickLJArgs& a, int bid
                     const float dy = pi.y - pj.y;
                     const float dz = pi.z - pj.z;
                     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                    if (!(__builtin_fabsf(r2 - rc_sq) <= band))
+                    if (r2 < rc_lo || !(r2 < rc_hi))
                         continue;
                     int gsj;
                     double xj, yj, zj;
@@ -697,7 +700,9 @@ static BrickLJArgs brick_lj_args(atvh_system* s, const LJArgs& a) {
         // roundoff u = 2^-24: each difference carries <= 2Eu + |d|u, and the
         // fma chain adds <= 3u r^2, so near the cutoff
         //   |r2_fp32 - r2| <= u (4 sqrt(3) rc E + 5 rc^2).
-        // Pairs within twice that of rc^2 take the plain kernel's FP64 test.
+        // Pairs within twice that of rc^2 take the plain kernel's FP64 test;
+        // the roundings of rc^2 and of rc^2 -/+ band (<= 2u rc^2 together)
+        // stay inside the second half.
         double E = 0.0;
         for (int d = 0; d < 3; ++d)
             E = std::max(E, (0.5 * s->bdim[d] + s->reach[d]) * s->cs[d] + s->skin);
