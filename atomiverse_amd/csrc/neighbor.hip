@@ -632,12 +632,6 @@ __device__ __forceinline__ bool band_pair(const double* __restrict__ x, const do
     return exact_pair(x, y, z, rc_sq, gi, gj, kind, shx, shy, shz);
 }
 
-// Number of set bits of m below this lane.
-__device__ __forceinline__ int lanes_below(unsigned long long m) {
-    return __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(m >> 32),
-                                     __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(m), 0u));
-}
-
 // Inclusive prefix sum over the 64 lanes (DPP row shifts, then the row
 // broadcasts of lanes 15 and 31).
 __device__ __forceinline__ int wave_inclusive_sum(int x) {
@@ -674,8 +668,11 @@ struct PairsLds {
     int next_cell; // next unclaimed cell of the brick
 };
 
+// One workgroup builds the rows of one brick (t-th in XCD-aware order). The
+// grid is the brick count, so no kernel-wide value has to survive from one
+// brick to the next through the candidate loop.
 template <bool STAGED>
-__device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int nbk, PairsLds& L) {
+__device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int t, PairsLds& L) {
     constexpr int kWaves = OpPairs::kWaves;
     // wave-uniform values go through readfirstlane so their loops run on
     // scalar registers
@@ -691,13 +688,13 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
     // stencil column of this lane (lanes >= ncols have none)
     const int4 rw = lane < ncols ? a.rows[lane] : make_int4(0, 0, 0, 0);
     unsigned short* const obuf = reinterpret_cast<unsigned short*>(&L.out[wid][0][0]);
-    for (int t = bid; t < a.nbricks; t += nbk) {
+    {
         const int b = xcd_brick(t, a.nbricks);
         const int nh = a.halo_n[b];
         const int2 br = a.brick_range[b];
         const int a0 = br.x, nat = br.y;
         if (brick_skipped(nh, nat, a.g.max_atoms))
-            continue; // reported by OpHalo (block-uniform)
+            return; // reported by OpHalo (block-uniform)
         const int cbase = a.g.brick_cell_base[b];
         int bx, by, bz;
         brick_coords(a.g, b, bx, by, bz);
@@ -737,8 +734,12 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
                                                   (0.5 * d2 + R2) * a.g.cs[2]));
         const float band_w = screen_band(ext, rc2f);
         const float lo_acc = rc2f - band_w, hi_rej = rc2f + band_w;
-        if (threadIdx.x == 0)
+        if (threadIdx.x == 0) {
             L.next_cell = kWaves;
+            // the sentinel row (no halo atom: nh <= kHaloRows) at |X|^2 = 1e30,
+            // which every screen rejects
+            L.sp[kSentinelRow] = make_float4(0.0f, 0.0f, 0.0f, 1e30f);
+        }
         __syncthreads();
         uint16_t* __restrict__ lst = a.nbr16 + (int64_t)a0 * cap;
         // cells are handed out one at a time (wave w starts with cell w and
@@ -789,7 +790,7 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
                 if (rem > 0)
                     L.cand[wid][p] = static_cast<unsigned short>(v);
                 if (total + lane < ((total + 63) & ~63)) // kCandCap is a multiple of 64
-                    L.cand[wid][total + lane] = 0;
+                    L.cand[wid][total + lane] = static_cast<unsigned short>(kSentinelRow);
                 __builtin_amdgcn_wave_barrier();
             }
             const int li0 = slot0 - a0; // brick-local index of the cell's first atom
@@ -819,11 +820,11 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
                 // the compares themselves (one v_cmp each) combined on scalar
                 // registers; the band, where the FP32 screen cannot decide, is
                 // rare and takes the branch.
-                auto test = [&](float qx, float qy, float qz, float qss, int q, unsigned long long vmask, float px,
-                                float py, float pz, float tlo, float thi, int qs, int li, uint16_t* __restrict__ dst,
-                                unsigned short* ost, int& count) {
+                auto test = [&](float qx, float qy, float qz, float qss, int q, float px, float py, float pz,
+                                float tlo, float thi, int qs, int li, uint16_t* __restrict__ dst, unsigned short* ost,
+                                int& count) {
                     const float v = __builtin_fmaf(px, qx, __builtin_fmaf(py, qy, __builtin_fmaf(pz, qz, qss)));
-                    const unsigned long long other = vmask & ~__builtin_amdgcn_ballot_w64(q == qs);
+                    const unsigned long long other = __builtin_amdgcn_ballot_w64(q != qs);
                     unsigned long long m = __builtin_amdgcn_ballot_w64(v < tlo) & other;
                     const unsigned long long band = __builtin_amdgcn_ballot_w64(v <= thi) & other & ~m;
                     if (band) { // the reference predicate for the band lanes
@@ -835,7 +836,10 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
                         m |= __builtin_amdgcn_ballot_w64(e);
                     }
                     if (__builtin_amdgcn_inverse_ballot_w64(m)) {
-                        const unsigned pos = static_cast<unsigned>(count + lanes_below(m));
+                        // the running count enters as mbcnt's addend
+                        const unsigned pos = __builtin_amdgcn_mbcnt_hi(
+                            static_cast<unsigned>(m >> 32),
+                            __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(m), static_cast<unsigned>(count)));
                         if (STAGED) // rows beyond the capacity land in the last group, which is never stored
                             ost[min(pos, static_cast<unsigned>(kOutCap))] = static_cast<unsigned short>(q);
                         else if (pos < static_cast<unsigned>(cap))
@@ -843,27 +847,26 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
                     }
                     count += static_cast<int>(__popcll(m));
                 };
-                auto step = [&](int q, unsigned long long vmask) {
+                // lanes past the candidates take the sentinel row, staged far
+                // beyond every threshold, so no lane mask is needed
+                auto step = [&](int q) {
                     const float4 qv = L.sp[q];
-                    test(qv.x, qv.y, qv.z, qv.w, q, vmask, ax, ay, az, tlo_a, thi_a, qa, lia, dsta, oa, ca);
+                    test(qv.x, qv.y, qv.z, qv.w, q, ax, ay, az, tlo_a, thi_a, qa, lia, dsta, oa, ca);
                     if (two)
-                        test(qv.x, qv.y, qv.z, qv.w, q, vmask, bx, by, bz, tlo_b, thi_b, qb2, lib, dstb, ob, cb);
+                        test(qv.x, qv.y, qv.z, qv.w, q, bx, by, bz, tlo_b, thi_b, qb2, lib, dstb, ob, cb);
                 };
                 if (flat) {
                     // the candidate slots past `total` up to the next multiple of
-                    // 64 hold row 0 (a valid LDS row; masked out by `valid`)
-                    for (int k0 = 0; k0 < total; k0 += 64) {
-                        const int cc = k0 + lane;
-                        step(static_cast<int>(L.cand[wid][cc]), __builtin_amdgcn_ballot_w64(cc < total));
-                    }
+                    // 64 hold the sentinel row
+                    const unsigned short* cp = &L.cand[wid][lane];
+                    for (int k0 = 0; k0 < total; k0 += 64, cp += 64)
+                        step(static_cast<int>(*cp));
                 } else {
                     for (int col = 0; col < ncols; ++col) {
                         const int cqb = __builtin_amdgcn_readlane(qb, col);
                         const int clen = __builtin_amdgcn_readlane(len, col);
-                        for (int k0 = 0; k0 < clen; k0 += 64) {
-                            const bool valid = k0 + lane < clen;
-                            step(valid ? cqb + k0 + lane : qa, __builtin_amdgcn_ballot_w64(valid));
-                        }
+                        for (int k0 = 0; k0 < clen; k0 += 64)
+                            step(k0 + lane < clen ? cqb + k0 + lane : kSentinelRow);
                     }
                 }
                 // pad the last group with the sentinel row (staged out of reach by
@@ -910,7 +913,6 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
             }();
             c = __builtin_amdgcn_readfirstlane(grab);
         }
-        __syncthreads(); // the coordinates and h_off are refilled for the next brick
     }
     for (int off = 32; off > 0; off >>= 1) {
         my_cnt_sum += __shfl_down(my_cnt_sum, off, 64);
@@ -924,28 +926,30 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int bid, int 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        long long t = 0;
+        long long tt = 0;
         int m = 0, ov = 0;
         for (int w = 0; w < kWaves; ++w) {
-            t += L.tot[w];
+            tt += L.tot[w];
             m = max(m, L.max[w]);
             ov += L.over[w];
         }
-        atomicAdd(reinterpret_cast<unsigned long long*>(&a.ctl->nbr_total), static_cast<unsigned long long>(t));
+        atomicAdd(reinterpret_cast<unsigned long long*>(&a.ctl->nbr_total), static_cast<unsigned long long>(tt));
         atomicMax(&a.ctl->max_nbr, m);
         if (ov)
             atomicAdd(&a.ctl->overflow, ov);
     }
 }
 
-__device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int nbk) {
-    if (!*a.pred)
+// The grid is the brick count (the pair stage of nl_stage_grids): one brick
+// per workgroup.
+__device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int) {
+    if (!*a.pred || bid >= a.nbricks)
         return;
     __shared__ PairsLds lds;
     if (a.cap <= kOutCap)
-        pairs_run<true>(a, bid, nbk, lds);
+        pairs_run<true>(a, bid, lds);
     else
-        pairs_run<false>(a, bid, nbk, lds);
+        pairs_run<false>(a, bid, lds);
 }
 
 // Brick-mode export: decode halo rows back to global slots, emit tag pairs a<b.
@@ -1493,10 +1497,14 @@ BrickBuildArgs nl_brick_args(atvh_system* s, const int* pred) {
 
 // Launch grids of the rebuild stages for one system (the batched rebuild
 // gives every system the same block counts).
-// Every stage strides over its units, so the grids are capped at
-// kStageGridCap blocks (a multiple of the 8 XCDs, which keeps the XCD-aware
-// brick order): on the steps without a rebuild each stage is launched and
-// leaves at its first instruction, and a smaller grid costs less to drain.
+// Every stage but the pair build strides over its units, so their grids are
+// capped at kStageGridCap blocks (a multiple of the 8 XCDs, which keeps the
+// XCD-aware brick order): on the steps without a rebuild each stage is
+// launched and leaves at its first instruction, and a smaller grid costs less
+// to drain. The pair build takes one workgroup per brick: with no loop over
+// bricks its candidate loop keeps more scalar registers (C1 rebuild 751 ->
+// 711 us, 512k-ion melt 318 -> 282 us), worth the ~3 us more that its idle
+// launch takes to drain.
 constexpr int kStageGridCap = 2048;
 
 void nl_stage_grids(const atvh_system* s, int g[kNlStages]) {
@@ -1510,7 +1518,7 @@ void nl_stage_grids(const atvh_system* s, int g[kNlStages]) {
     g[6] = ga;                                                   // permute
     g[7] = ga;                                                   // copy back
     g[8] = std::min(s->nbricks, kStageGridCap);                  // halo rows
-    g[9] = std::min(s->nbricks, kStageGridCap);                  // pair build
+    g[9] = s->nbricks;                                           // pair build: one workgroup per brick
 }
 
 void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
