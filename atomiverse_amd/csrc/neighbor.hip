@@ -45,11 +45,11 @@ struct SortArgs {
     const int* pred;
     int64_t n;
     int ncells;
-    double *x, *y, *z;
+    const double *x, *y, *z;
     double lx, ly, lz, csx, csy, csz;
     int ncx, ncy, ncz;
     const int* cell_rank;
-    int* cell_of;
+    int* cell_of; // cell of every slot in the current (pre-sort) order
     int* count;
     int* start;
     int* tile_sums;
@@ -57,6 +57,16 @@ struct SortArgs {
     int* fill;
     int* perm;
     const int* tag;
+    // OpBin's snapshot of the per-atom arrays in the current slot order
+    // (positions wrapped); OpPermute gathers from it into the system's arrays
+    const double *vx, *vy, *vz, *fx, *fy, *fz, *q;
+    const int* type;
+    double *ox, *oy, *oz, *ovx, *ovy, *ovz, *ofx, *ofy, *ofz, *oq;
+    int *otype, *otag;
+    // arrays whose permutation is a no-op or unread are skipped: forces
+    // outside the first build, charges of an uncharged frame (all zero),
+    // types of a one-type frame (all zero)
+    int with_forces, with_q, with_type;
 };
 
 // Runs of equal keys in consecutive lanes (slots are in cell order from the
@@ -83,7 +93,9 @@ __device__ __forceinline__ LaneRun lane_run(int key) {
 }
 
 // CellList::bin_atoms: cell of every slot (current order) + histogram (one
-// atomic per run of equal cells in a wave).
+// atomic per run of equal cells in a wave). The slot's arrays are copied to
+// the snapshot on the way (positions wrapped), so the permutation can write
+// the system's arrays in place.
 struct OpBin {
     using Args = SortArgs;
     static __device__ void run(const SortArgs& a, int bid, int nbk) {
@@ -103,9 +115,22 @@ struct OpBin {
                 xi -= a.lx * floor(xi / a.lx);
                 yi -= a.ly * floor(yi / a.ly);
                 zi -= a.lz * floor(zi / a.lz);
-                a.x[i] = xi;
-                a.y[i] = yi;
-                a.z[i] = zi;
+                a.ox[i] = xi;
+                a.oy[i] = yi;
+                a.oz[i] = zi;
+                a.ovx[i] = a.vx[i];
+                a.ovy[i] = a.vy[i];
+                a.ovz[i] = a.vz[i];
+                a.otag[i] = a.tag[i];
+                if (a.with_q)
+                    a.oq[i] = a.q[i];
+                if (a.with_type)
+                    a.otype[i] = a.type[i];
+                if (a.with_forces) {
+                    a.ofx[i] = a.fx[i];
+                    a.ofy[i] = a.fy[i];
+                    a.ofz[i] = a.fz[i];
+                }
                 // bin_atoms wraps its own copy once more (a no-op except at x == L)
                 xi -= a.lx * floor(xi / a.lx);
                 yi -= a.ly * floor(yi / a.ly);
@@ -262,10 +287,12 @@ struct PermArgs {
     const int* pred;
     int64_t n;
     const int* perm;
-    const double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *q;
-    const int *type, *tag, *cell_of;
-    double *ax, *ay, *az, *avx, *avy, *avz, *afx, *afy, *afz, *aq;
-    int *atype, *atag, *acell_of;
+    // OpBin's snapshot (previous slot order)
+    const double *ox, *oy, *oz, *ovx, *ovy, *ovz, *ofx, *ofy, *ofz, *oq;
+    const int *otype, *otag, *ocell_of;
+    // the system's arrays, written in the new slot order, and x_save
+    double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *q, *xs, *ys, *zs;
+    int *type, *tag, *cell_of;
     int with_forces;
     // arrays whose permutation is a no-op or unread are skipped: charges of an
     // uncharged frame (all zero), types of a one-type frame (all zero), and
@@ -273,6 +300,8 @@ struct PermArgs {
     int with_q, with_type, with_cell_of;
 };
 
+// The sorted order, gathered from the snapshot into the system's arrays,
+// and x_save (CellList::save_positions) at the same positions.
 struct OpPermute {
     using Args = PermArgs;
     static __device__ void run(const PermArgs& a, int bid, int nbk) {
@@ -280,67 +309,27 @@ struct OpPermute {
         return;
     for (int64_t s = bid * (int64_t)blockDim.x + threadIdx.x; s < a.n; s += (int64_t)nbk * blockDim.x) {
         const int p = a.perm[s];
-        a.ax[s] = a.x[p];
-        a.ay[s] = a.y[p];
-        a.az[s] = a.z[p];
-        a.avx[s] = a.vx[p];
-        a.avy[s] = a.vy[p];
-        a.avz[s] = a.vz[p];
-        if (a.with_q)
-            a.aq[s] = a.q[p];
-        if (a.with_type)
-            a.atype[s] = a.type[p];
-        a.atag[s] = a.tag[p];
-        if (a.with_cell_of)
-            a.acell_of[s] = a.cell_of[p];
-        if (a.with_forces) {
-            a.afx[s] = a.fx[p];
-            a.afy[s] = a.fy[p];
-            a.afz[s] = a.fz[p];
-        }
-    }
-    }
-};
-
-struct CopyArgs {
-    const int* pred;
-    int64_t n;
-    const double *ax, *ay, *az, *avx, *avy, *avz, *afx, *afy, *afz, *aq;
-    const int *atype, *atag, *acell_of;
-    double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *q, *xs, *ys, *zs;
-    int *type, *tag, *cell_of;
-    int with_forces;
-    int with_q, with_type, with_cell_of; // as in PermArgs
-};
-
-// Copy the permuted arrays back and snapshot x_save (CellList::save_positions).
-struct OpCopyback {
-    using Args = CopyArgs;
-    static __device__ void run(const CopyArgs& a, int bid, int nbk) {
-    if (!*a.pred)
-        return;
-    for (int64_t s = bid * (int64_t)blockDim.x + threadIdx.x; s < a.n; s += (int64_t)nbk * blockDim.x) {
-        const double xx = a.ax[s], yy = a.ay[s], zz = a.az[s];
+        const double xx = a.ox[p], yy = a.oy[p], zz = a.oz[p];
         a.x[s] = xx;
         a.y[s] = yy;
         a.z[s] = zz;
         a.xs[s] = xx;
         a.ys[s] = yy;
         a.zs[s] = zz;
-        a.vx[s] = a.avx[s];
-        a.vy[s] = a.avy[s];
-        a.vz[s] = a.avz[s];
+        a.vx[s] = a.ovx[p];
+        a.vy[s] = a.ovy[p];
+        a.vz[s] = a.ovz[p];
         if (a.with_q)
-            a.q[s] = a.aq[s];
+            a.q[s] = a.oq[p];
         if (a.with_type)
-            a.type[s] = a.atype[s];
-        a.tag[s] = a.atag[s];
+            a.type[s] = a.otype[p];
+        a.tag[s] = a.otag[p];
         if (a.with_cell_of)
-            a.cell_of[s] = a.acell_of[s];
+            a.cell_of[s] = a.ocell_of[p];
         if (a.with_forces) {
-            a.fx[s] = a.afx[s];
-            a.fy[s] = a.afy[s];
-            a.fz[s] = a.afz[s];
+            a.fx[s] = a.ofx[p];
+            a.fy[s] = a.ofy[p];
+            a.fz[s] = a.ofz[p];
         }
     }
     }
@@ -1361,7 +1350,7 @@ void nl_free(atvh_system* s) {
     dev_free(s->brick_range);
 }
 
-SortArgs nl_sort_args(atvh_system* s, const int* pred) {
+SortArgs nl_sort_args(atvh_system* s, const int* pred, bool permute_forces) {
     SortArgs a{};
     a.pred = pred;
     a.n = s->n;
@@ -1379,7 +1368,7 @@ SortArgs nl_sort_args(atvh_system* s, const int* pred) {
     a.ncy = s->nc[1];
     a.ncz = s->nc[2];
     a.cell_rank = s->cell_rank;
-    a.cell_of = s->cell_of;
+    a.cell_of = s->acell_of; // pre-sort order; the permutation writes s->cell_of
     a.count = s->cell_count;
     a.start = s->cell_start;
     a.tile_sums = s->scan_sums;
@@ -1387,17 +1376,6 @@ SortArgs nl_sort_args(atvh_system* s, const int* pred) {
     a.fill = s->cell_fill;
     a.perm = s->perm;
     a.tag = s->tag;
-    return a;
-}
-
-PermArgs nl_perm_args(atvh_system* s, const int* pred, bool permute_forces) {
-    PermArgs a{};
-    a.pred = pred;
-    a.n = s->n;
-    a.perm = s->perm;
-    a.x = s->x;
-    a.y = s->y;
-    a.z = s->z;
     a.vx = s->vx;
     a.vy = s->vy;
     a.vz = s->vz;
@@ -1406,45 +1384,42 @@ PermArgs nl_perm_args(atvh_system* s, const int* pred, bool permute_forces) {
     a.fz = s->fz;
     a.q = s->q;
     a.type = s->type;
-    a.tag = s->tag;
-    a.cell_of = s->cell_of;
-    a.ax = s->ax;
-    a.ay = s->ay;
-    a.az = s->az;
-    a.avx = s->avx;
-    a.avy = s->avy;
-    a.avz = s->avz;
-    a.afx = s->afx;
-    a.afy = s->afy;
-    a.afz = s->afz;
-    a.aq = s->aq;
-    a.atype = s->atype;
-    a.atag = s->atag;
-    a.acell_of = s->acell_of;
+    a.ox = s->ax;
+    a.oy = s->ay;
+    a.oz = s->az;
+    a.ovx = s->avx;
+    a.ovy = s->avy;
+    a.ovz = s->avz;
+    a.ofx = s->afx;
+    a.ofy = s->afy;
+    a.ofz = s->afz;
+    a.oq = s->aq;
+    a.otype = s->atype;
+    a.otag = s->atag;
     a.with_forces = permute_forces ? 1 : 0;
     a.with_q = s->has_charge ? 1 : 0;
     a.with_type = s->ntypes > 1 ? 1 : 0;
-    a.with_cell_of = s->brick_mode ? 0 : 1;
     return a;
 }
 
-CopyArgs nl_copy_args(atvh_system* s, const int* pred, bool permute_forces) {
-    CopyArgs a{};
+PermArgs nl_perm_args(atvh_system* s, const int* pred, bool permute_forces) {
+    PermArgs a{};
     a.pred = pred;
     a.n = s->n;
-    a.ax = s->ax;
-    a.ay = s->ay;
-    a.az = s->az;
-    a.avx = s->avx;
-    a.avy = s->avy;
-    a.avz = s->avz;
-    a.afx = s->afx;
-    a.afy = s->afy;
-    a.afz = s->afz;
-    a.aq = s->aq;
-    a.atype = s->atype;
-    a.atag = s->atag;
-    a.acell_of = s->acell_of;
+    a.perm = s->perm;
+    a.ox = s->ax;
+    a.oy = s->ay;
+    a.oz = s->az;
+    a.ovx = s->avx;
+    a.ovy = s->avy;
+    a.ovz = s->avz;
+    a.ofx = s->afx;
+    a.ofy = s->afy;
+    a.ofz = s->afz;
+    a.oq = s->aq;
+    a.otype = s->atype;
+    a.otag = s->atag;
+    a.ocell_of = s->acell_of;
     a.x = s->x;
     a.y = s->y;
     a.z = s->z;
@@ -1515,10 +1490,9 @@ void nl_stage_grids(const atvh_system* s, int g[kNlStages]) {
     g[3] = s->nscan_blocks;                                      // scan add
     g[4] = ga;                                                   // scatter
     g[5] = std::min(grid_capped(s->ncells), kStageGridCap);      // in-cell order
-    g[6] = ga;                                                   // permute
-    g[7] = ga;                                                   // copy back
-    g[8] = std::min(s->nbricks, kStageGridCap);                  // halo rows
-    g[9] = s->nbricks;                                           // pair build: one workgroup per brick
+    g[6] = ga;                                                   // permute (+ x_save)
+    g[7] = std::min(s->nbricks, kStageGridCap);                  // halo rows
+    g[8] = s->nbricks;                                           // pair build: one workgroup per brick
 }
 
 void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
@@ -1526,7 +1500,7 @@ void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
     hipStream_t st = s->ctx->stream;
     int g[kNlStages];
     nl_stage_grids(s, g);
-    const SortArgs sa = nl_sort_args(s, pred);
+    const SortArgs sa = nl_sort_args(s, pred, permute_forces);
     k_single<OpBin><<<g[0], kBlock, 0, st>>>(sa);
     k_single<OpScanTiles><<<g[1], kBlock, 0, st>>>(sa);
     k_single<OpScanSums><<<g[2], kBlock, 0, st>>>(sa);
@@ -1534,11 +1508,10 @@ void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
     k_single<OpScatter><<<g[4], kBlock, 0, st>>>(sa);
     k_single<OpCellOrder><<<g[5], kBlock, 0, st>>>(sa);
     k_single<OpPermute><<<g[6], kBlock, 0, st>>>(nl_perm_args(s, pred, permute_forces));
-    k_single<OpCopyback><<<g[7], kBlock, 0, st>>>(nl_copy_args(s, pred, permute_forces));
     if (s->brick_mode) {
         const BrickBuildArgs ba = nl_brick_args(s, pred);
-        k_single<OpHalo><<<g[8], kBlock, 0, st>>>(ba);
-        k_single<OpPairs><<<g[9], OpPairs::kThreads, 0, st>>>(ba);
+        k_single<OpHalo><<<g[7], kBlock, 0, st>>>(ba);
+        k_single<OpPairs><<<g[8], OpPairs::kThreads, 0, st>>>(ba);
     } else {
         BuildArgs ba{s->x, s->y, s->z, s->cell_of, s->cell_rank, s->cell_start,
                      reinterpret_cast<const int4*>(s->d_stencil), s->nstencil, s->nc[0], s->nc[1],
@@ -1549,22 +1522,20 @@ void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
     ATVH_HIP(hipGetLastError());
 }
 
-// Batched rebuild: the ten stages of a brick-mode rebuild, one launch each
+// Batched rebuild: the nine stages of a brick-mode rebuild, one launch each
 // over every member; a member whose rebuild flag is clear leaves each stage
 // at its first instruction.
 void nl_batch_build(atvh_batch* b) {
     std::vector<SortArgs> sa;
     std::vector<PermArgs> pa;
-    std::vector<CopyArgs> ca;
     std::vector<BrickBuildArgs> ba;
     std::vector<int> g[kNlStages];
     for (atvh_system* s : b->sys) {
         if (!s->brick_mode)
             throw Error(ATVH_E_ARG, "batch: every member needs the brick layout (>= 2*reach+1 cells per axis)");
         const int* pred = &s->ctl->rebuild;
-        sa.push_back(nl_sort_args(s, pred));
+        sa.push_back(nl_sort_args(s, pred, false));
         pa.push_back(nl_perm_args(s, pred, false));
-        ca.push_back(nl_copy_args(s, pred, false));
         ba.push_back(nl_brick_args(s, pred));
         int gs[kNlStages];
         nl_stage_grids(s, gs);
@@ -1578,9 +1549,8 @@ void nl_batch_build(atvh_batch* b) {
     batch_stage(b, "nl_scatter", sa, g[4]);
     batch_stage(b, "nl_cell_order", sa, g[5]);
     batch_stage(b, "nl_permute", pa, g[6]);
-    batch_stage(b, "nl_copyback", ca, g[7]);
-    batch_stage(b, "nl_halo", ba, g[8]);
-    batch_stage(b, "nl_pairs", ba, g[9]);
+    batch_stage(b, "nl_halo", ba, g[7]);
+    batch_stage(b, "nl_pairs", ba, g[8]);
     const size_t nsys = b->sys.size();
     std::vector<int> nb;
     for (int k = 0; k < kNlStages; ++k)
@@ -1643,9 +1613,8 @@ void nl_batch_launch_rebuild(atvh_batch* b) {
     nl_batch_one<OpScatter>(b, "nl_scatter", 4);
     nl_batch_one<OpCellOrder>(b, "nl_cell_order", 5);
     nl_batch_one<OpPermute>(b, "nl_permute", 6);
-    nl_batch_one<OpCopyback>(b, "nl_copyback", 7);
-    nl_batch_one<OpHalo>(b, "nl_halo", 8);
-    nl_batch_one<OpPairs>(b, "nl_pairs", 9);
+    nl_batch_one<OpHalo>(b, "nl_halo", 7);
+    nl_batch_one<OpPairs>(b, "nl_pairs", 8);
     ATVH_HIP(hipGetLastError());
 }
 
