@@ -288,7 +288,7 @@ void dev_free(T*& p) {
 }
 
 // neighbor.hip
-constexpr int kNlStages = 9; // kernels of one brick-mode rebuild
+constexpr int kNlStages = 7; // kernels of one brick-mode rebuild
 void nl_setup_grid(atvh_system* s);
 void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces);
 void nl_enqueue_decide(atvh_system* s);

@@ -195,7 +195,10 @@ struct OpScanTiles {
     }
 };
 
-// Scan phase 2: one block scans the tile totals (any count, chunked).
+// Scan phase 2: one block scans the tile totals (any count, chunked) into
+// the tiles' offsets. A cell's start is its tile-local start plus its
+// tile's offset: the scatter adds it on the fly, and the in-cell order pass
+// writes the sum back as the cell's start.
 struct OpScanSums {
     using Args = SortArgs;
     static __device__ void run(const SortArgs& a, int, int) {
@@ -214,20 +217,6 @@ struct OpScanSums {
         }
         if (threadIdx.x == 0)
             a.start[a.ncells] = carry;
-    }
-};
-
-// Scan phase 3: add the tile offsets.
-struct OpScanAdd {
-    using Args = SortArgs;
-    static __device__ void run(const SortArgs& a, int bid, int) {
-        if (!*a.pred)
-            return;
-        const int off = a.tile_sums[bid];
-        const int base = bid * kScanTile + threadIdx.x * kScanItems;
-        for (int k = 0; k < kScanItems; ++k)
-            if (base + k < a.ncells)
-                a.start[base + k] += off;
     }
 };
 
@@ -252,7 +241,7 @@ struct OpScatter {
                 base = atomicAdd(&a.fill[c], r.length);
             base = __shfl(base, r.head, 64);
             if (act) {
-                a.perm[a.start[c] + base + (lane - r.head)] = static_cast<int>(i);
+                a.perm[a.start[c] + a.tile_sums[c / kScanTile] + base + (lane - r.head)] = static_cast<int>(i);
                 a.count[c] = 0;
             }
         }
@@ -260,14 +249,17 @@ struct OpScatter {
 };
 
 // Deterministic order inside a cell: ascending tag (insertion sort; cells
-// hold a handful of atoms). Also clears the fill counters.
+// hold a handful of atoms). The cell's start becomes final here (tile-local
+// start plus tile offset; its end from its own fill count, so no thread reads
+// a start another one rewrites). Also clears the fill counters.
 struct OpCellOrder {
     using Args = SortArgs;
     static __device__ void run(const SortArgs& a, int bid, int nbk) {
         if (!*a.pred)
             return;
         for (int c = bid * blockDim.x + threadIdx.x; c < a.ncells; c += nbk * blockDim.x) {
-            const int b = a.start[c], e = a.start[c + 1];
+            const int b = a.start[c] + a.tile_sums[c / kScanTile], e = b + a.fill[c];
+            a.start[c] = b;
             for (int k = b + 1; k < e; ++k) {
                 const int p = a.perm[k];
                 const int tp = a.tag[p];
@@ -540,6 +532,26 @@ __device__ void OpHalo::run(const BrickBuildArgs& a, int bid, int nbk) {
         __syncthreads(); // LDS reused by the next brick
     }
 }
+
+// The permutation (blocks [0, gp)) and the halo tables (the remaining
+// blocks) in one launch: both need only the final cell starts, and neither
+// reads what the other writes. A rebuild stage fewer also means one launch
+// fewer on every step without a rebuild.
+struct PermHaloArgs {
+    PermArgs p;
+    BrickBuildArgs h;
+    int gp; // permutation blocks
+};
+
+struct OpPermuteHalo {
+    using Args = PermHaloArgs;
+    static __device__ void run(const PermHaloArgs& a, int bid, int nbk) {
+        if (bid < a.gp)
+            OpPermute::run(a.p, bid, a.gp);
+        else
+            OpHalo::run(a.h, bid - a.gp, nbk - a.gp);
+    }
+};
 
 // Rebuild step 2 (one workgroup per brick, one wavefront per home cell). The
 // wave lays the cell's candidates out flat — every stencil column's z-run of
@@ -1482,37 +1494,38 @@ BrickBuildArgs nl_brick_args(atvh_system* s, const int* pred) {
 // launch takes to drain.
 constexpr int kStageGridCap = 2048;
 
-void nl_stage_grids(const atvh_system* s, int g[kNlStages]) {
+// Block counts of the rebuild kernels, one more than the stages: in brick
+// mode the permutation (g[5]) and the halo tables (g[6]) share a launch.
+void nl_stage_grids(const atvh_system* s, int g[kNlStages + 1]) {
     const int ga = std::min(grid_capped(s->n), kStageGridCap);
     g[0] = ga;                                                   // bin
     g[1] = s->nscan_blocks;                                      // scan tiles
     g[2] = 1;                                                    // scan sums
-    g[3] = s->nscan_blocks;                                      // scan add
-    g[4] = ga;                                                   // scatter
-    g[5] = std::min(grid_capped(s->ncells), kStageGridCap);      // in-cell order
-    g[6] = ga;                                                   // permute (+ x_save)
-    g[7] = std::min(s->nbricks, kStageGridCap);                  // halo rows
-    g[8] = s->nbricks;                                           // pair build: one workgroup per brick
+    g[3] = ga;                                                   // scatter
+    g[4] = std::min(grid_capped(s->ncells), kStageGridCap);      // in-cell order (final cell starts)
+    g[5] = ga;                                                   // permute (+ x_save); the halo
+    g[6] = std::min(s->nbricks, kStageGridCap);                  //   tables' blocks follow in its launch
+    g[7] = s->nbricks;                                           // pair build: one workgroup per brick
 }
 
 void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
     ScopedTimer tm(s->ctx, "nlist");
     hipStream_t st = s->ctx->stream;
-    int g[kNlStages];
+    int g[kNlStages + 1];
     nl_stage_grids(s, g);
     const SortArgs sa = nl_sort_args(s, pred, permute_forces);
     k_single<OpBin><<<g[0], kBlock, 0, st>>>(sa);
     k_single<OpScanTiles><<<g[1], kBlock, 0, st>>>(sa);
     k_single<OpScanSums><<<g[2], kBlock, 0, st>>>(sa);
-    k_single<OpScanAdd><<<g[3], kBlock, 0, st>>>(sa);
-    k_single<OpScatter><<<g[4], kBlock, 0, st>>>(sa);
-    k_single<OpCellOrder><<<g[5], kBlock, 0, st>>>(sa);
-    k_single<OpPermute><<<g[6], kBlock, 0, st>>>(nl_perm_args(s, pred, permute_forces));
+    k_single<OpScatter><<<g[3], kBlock, 0, st>>>(sa);
+    k_single<OpCellOrder><<<g[4], kBlock, 0, st>>>(sa);
     if (s->brick_mode) {
         const BrickBuildArgs ba = nl_brick_args(s, pred);
-        k_single<OpHalo><<<g[7], kBlock, 0, st>>>(ba);
-        k_single<OpPairs><<<g[8], OpPairs::kThreads, 0, st>>>(ba);
+        const PermHaloArgs ph{nl_perm_args(s, pred, permute_forces), ba, g[5]};
+        k_single<OpPermuteHalo><<<g[5] + g[6], kBlock, 0, st>>>(ph);
+        k_single<OpPairs><<<g[7], OpPairs::kThreads, 0, st>>>(ba);
     } else {
+        k_single<OpPermute><<<g[5], kBlock, 0, st>>>(nl_perm_args(s, pred, permute_forces));
         BuildArgs ba{s->x, s->y, s->z, s->cell_of, s->cell_rank, s->cell_start,
                      reinterpret_cast<const int4*>(s->d_stencil), s->nstencil, s->nc[0], s->nc[1],
                      s->nc[2], s->box[0], s->box[1], s->box[2], s->cutoff_skin * s->cutoff_skin,
@@ -1522,12 +1535,12 @@ void nl_enqueue_rebuild(atvh_system* s, const int* pred, bool permute_forces) {
     ATVH_HIP(hipGetLastError());
 }
 
-// Batched rebuild: the nine stages of a brick-mode rebuild, one launch each
+// Batched rebuild: the seven stages of a brick-mode rebuild, one launch each
 // over every member; a member whose rebuild flag is clear leaves each stage
 // at its first instruction.
 void nl_batch_build(atvh_batch* b) {
     std::vector<SortArgs> sa;
-    std::vector<PermArgs> pa;
+    std::vector<PermHaloArgs> pa;
     std::vector<BrickBuildArgs> ba;
     std::vector<int> g[kNlStages];
     for (atvh_system* s : b->sys) {
@@ -1535,22 +1548,22 @@ void nl_batch_build(atvh_batch* b) {
             throw Error(ATVH_E_ARG, "batch: every member needs the brick layout (>= 2*reach+1 cells per axis)");
         const int* pred = &s->ctl->rebuild;
         sa.push_back(nl_sort_args(s, pred, false));
-        pa.push_back(nl_perm_args(s, pred, false));
         ba.push_back(nl_brick_args(s, pred));
-        int gs[kNlStages];
+        int gs[kNlStages + 1];
         nl_stage_grids(s, gs);
+        pa.push_back(PermHaloArgs{nl_perm_args(s, pred, false), ba.back(), gs[5]});
+        gs[5] += gs[6]; // the halo tables' blocks ride in the permutation's launch
+        gs[6] = gs[7];
         for (int k = 0; k < kNlStages; ++k)
             g[k].push_back(gs[k]);
     }
     batch_stage(b, "nl_bin", sa, g[0]);
     batch_stage(b, "nl_scan_tiles", sa, g[1]);
     batch_stage(b, "nl_scan_sums", sa, g[2]);
-    batch_stage(b, "nl_scan_add", sa, g[3]);
-    batch_stage(b, "nl_scatter", sa, g[4]);
-    batch_stage(b, "nl_cell_order", sa, g[5]);
-    batch_stage(b, "nl_permute", pa, g[6]);
-    batch_stage(b, "nl_halo", ba, g[7]);
-    batch_stage(b, "nl_pairs", ba, g[8]);
+    batch_stage(b, "nl_scatter", sa, g[3]);
+    batch_stage(b, "nl_cell_order", sa, g[4]);
+    batch_stage(b, "nl_permute_halo", pa, g[5]);
+    batch_stage(b, "nl_pairs", ba, g[6]);
     const size_t nsys = b->sys.size();
     std::vector<int> nb;
     for (int k = 0; k < kNlStages; ++k)
@@ -1609,12 +1622,10 @@ void nl_batch_launch_rebuild(atvh_batch* b) {
     nl_batch_one<OpBin>(b, "nl_bin", 0);
     nl_batch_one<OpScanTiles>(b, "nl_scan_tiles", 1);
     nl_batch_one<OpScanSums>(b, "nl_scan_sums", 2);
-    nl_batch_one<OpScanAdd>(b, "nl_scan_add", 3);
-    nl_batch_one<OpScatter>(b, "nl_scatter", 4);
-    nl_batch_one<OpCellOrder>(b, "nl_cell_order", 5);
-    nl_batch_one<OpPermute>(b, "nl_permute", 6);
-    nl_batch_one<OpHalo>(b, "nl_halo", 7);
-    nl_batch_one<OpPairs>(b, "nl_pairs", 8);
+    nl_batch_one<OpScatter>(b, "nl_scatter", 3);
+    nl_batch_one<OpCellOrder>(b, "nl_cell_order", 4);
+    nl_batch_one<OpPermuteHalo>(b, "nl_permute_halo", 5);
+    nl_batch_one<OpPairs>(b, "nl_pairs", 6);
     ATVH_HIP(hipGetLastError());
 }
 
