@@ -105,13 +105,30 @@ void free_graphs(atvh_batch* b) {
     }
 }
 
+// One batched step. With timing on (an eager run: timed steps are never
+// captured) each launch family is bracketed by events of its own, named
+// apart from the single-system families (a charged member's own force call
+// inside the pair family times itself as "pair").
 void enqueue_step(atvh_batch* b, int precision, bool ef) {
-    integ_batch_launch(b, "initial");
-    integ_batch_launch(b, "post_initial");
-    nl_batch_launch_rebuild(b);
-    pair_batch_launch(b, precision, ef);
-    if (ef)
+    atvh_ctx* c = b->ctx;
+    {
+        ScopedTimer tm(c, "batch_integrate");
+        integ_batch_launch(b, "initial");
+        integ_batch_launch(b, "post_initial");
+    }
+    {
+        ScopedTimer tm(c, "batch_nlist");
+        nl_batch_launch_rebuild(b);
+    }
+    {
+        ScopedTimer tm(c, "batch_pair");
+        pair_batch_launch(b, precision, ef);
+    }
+    if (ef) {
+        ScopedTimer tm(c, "batch_energy");
         integ_batch_launch(b, "finalize");
+    }
+    ScopedTimer tm(c, "batch_integrate");
     integ_batch_launch(b, "final");
     integ_batch_launch(b, "post_final");
 }

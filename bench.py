@@ -224,10 +224,21 @@ def ladder_section(av, W, ctx, steps, loop_steps):
     b.run(steps, av.PREC_MIXED)
     ctx.synchronize()
     dt_b = time.perf_counter() - t0
+    # per-family breakdown: a separate eager pass (timed steps are not
+    # captured) with HIP events around each launch family of the step
+    nb = 40
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    b.run(nb, av.PREC_MIXED)
+    ctx.synchronize()
+    ctx.enable_timing(False)
+    stages = {k: 1e3 * ctx.kernel_time(k)[0] / nb
+              for k in ("batch_step", "batch_integrate", "batch_nlist", "batch_pair")}
     b.close()
     for s in ms:
         s.close()
     log(f"  batched: {dt_b * 1e3 / steps:.3f} ms/step, {total * steps / dt_b:.4e} atom-steps/s")
+    log("  eager pass with events, us per step: " + ", ".join(f"{k} {v:.1f}" for k, v in stages.items()))
     log(f"bench: ladder per-system loop, {loop_steps} steps")
     ms = make()
     for s in ms:
@@ -244,6 +255,7 @@ def ladder_section(av, W, ctx, steps, loop_steps):
     return {"workload": "lj_ladder_256 (N log-uniform 500..20000, T* 0.7..1.5, NVT, mixed)",
             "atoms": total, "systems": len(ws), "autotuned_brick_shapes": tuned,
             "batched_atom_steps_per_s": total * steps / dt_b, "batched_steps": steps,
+            "eager_us_per_step_with_events": stages,
             "loop_atom_steps_per_s": total * loop_steps / dt_l, "loop_steps": loop_steps,
             "batched_over_loop": (total * steps / dt_b) / (total * loop_steps / dt_l)}
 
