@@ -776,20 +776,38 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int t, PairsL
             const int total = __builtin_amdgcn_readlane(inc, 63);
             const bool flat = total <= kCandCap;
             if (flat) {
-                // the column's rows qb, qb+1, ... two to a 32-bit store after an
-                // odd first slot
-                int p = inc - len, v = qb, rem = len;
-                if ((p & 1) && rem > 0) {
-                    L.cand[wid][p] = static_cast<unsigned short>(v);
-                    ++p;
-                    ++v;
-                    --rem;
+                // the column's rows qb, qb+1, ...: single slots up to a multiple
+                // of four, then four rows per 8-byte store (two packed pairs,
+                // each advanced by 4 per store; rows stay below 2^11, so the
+                // halves never carry), then the last single slots
+                int p = inc - len, v = qb;
+                const int end = inc;
+                unsigned short* const cw = &L.cand[wid][0];
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    if ((p & 3) != 0 && p < end) {
+                        cw[p] = static_cast<unsigned short>(v);
+                        ++p;
+                        ++v;
+                    }
+                const int n4 = max(end - p, 0) >> 2;
+                uint2* const c64 = reinterpret_cast<uint2*>(cw) + (p >> 2);
+                unsigned lo = static_cast<unsigned>(v) | (static_cast<unsigned>(v + 1) << 16);
+                unsigned hi = static_cast<unsigned>(v + 2) | (static_cast<unsigned>(v + 3) << 16);
+                for (int k = 0; k < n4; ++k) {
+                    c64[k] = make_uint2(lo, hi);
+                    lo += 0x00040004u;
+                    hi += 0x00040004u;
                 }
-                unsigned* const c32 = reinterpret_cast<unsigned*>(&L.cand[wid][0]);
-                for (; rem >= 2; rem -= 2, p += 2, v += 2)
-                    c32[p >> 1] = static_cast<unsigned>(v) | (static_cast<unsigned>(v + 1) << 16);
-                if (rem > 0)
-                    L.cand[wid][p] = static_cast<unsigned short>(v);
+                p += n4 << 2;
+                v += n4 << 2;
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    if (p < end) {
+                        cw[p] = static_cast<unsigned short>(v);
+                        ++p;
+                        ++v;
+                    }
                 if (total + lane < ((total + 63) & ~63)) // kCandCap is a multiple of 64
                     L.cand[wid][total + lane] = static_cast<unsigned short>(kSentinelRow);
                 __builtin_amdgcn_wave_barrier();
