@@ -653,6 +653,14 @@ struct OpPairs {
     static __device__ void run(const BrickBuildArgs& a, int bid, int nbk);
 };
 
+// The same build for the batched rebuild stage. Its kernel keeps more values
+// live (the active-member search), and there the opaque home slot of the
+// band branch (see pairs_run) costs vector spills instead of saving scalar
+// ones (ladder 3.06e9 -> 2.68e9 atom-steps/s), so it keeps the plain form.
+struct OpPairsBatched : OpPairs {
+    static __device__ void run(const BrickBuildArgs& a, int bid, int nbk);
+};
+
 constexpr int kMaxBrickCells = 100; // home cells of the largest brick shape (4 x 5 x 5)
 
 // LDS of one OpPairs workgroup (79.2 KB: two workgroups per CU).
@@ -672,7 +680,7 @@ struct PairsLds {
 // One workgroup builds the rows of one brick (t-th in XCD-aware order). The
 // grid is the brick count, so no kernel-wide value has to survive from one
 // brick to the next through the candidate loop.
-template <bool STAGED>
+template <bool STAGED, bool OPAQUE_SLOT>
 __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int t, PairsLds& L) {
     constexpr int kWaves = OpPairs::kWaves;
     // wave-uniform values go through readfirstlane so their loops run on
@@ -848,10 +856,19 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int t, PairsL
                     const unsigned long long band = __builtin_amdgcn_ballot_w64(v <= thi) & other & ~m;
                     if (band) { // the reference predicate for the band lanes
                         bool e = false;
-                        if (__builtin_amdgcn_inverse_ballot_w64(band))
+                        if (__builtin_amdgcn_inverse_ballot_w64(band)) {
+                            // with OPAQUE_SLOT the home atom's slot is made opaque
+                            // here, so the compiler cannot hoist its coordinate
+                            // addresses out of the candidate loop (six 64-bit
+                            // pointers held in scalar registers through it for this
+                            // rare branch: C1 rebuild 689 -> 636 us)
+                            int gi = a0 + li;
+                            if constexpr (OPAQUE_SLOT)
+                                asm volatile("" : "+v"(gi));
                             e = band_pair(a.x, a.y, a.z, a.rc_sq, Lx, Ly, Lz, a.g.nc[0], a.g.nc[1], a.g.nc[2],
-                                          L.h_off, ncell, HY, HZ, q, lx + R0, ly + R1, hz0, cx, cy, cz, a0 + li,
+                                          L.h_off, ncell, HY, HZ, q, lx + R0, ly + R1, hz0, cx, cy, cz, gi,
                                           rows[q] & kRowSlotMask);
+                        }
                         m |= __builtin_amdgcn_ballot_w64(e);
                     }
                     if (__builtin_amdgcn_inverse_ballot_w64(m)) {
@@ -966,9 +983,19 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int) {
         return;
     __shared__ PairsLds lds;
     if (a.cap <= kOutCap)
-        pairs_run<true>(a, bid, lds);
+        pairs_run<true, true>(a, bid, lds);
     else
-        pairs_run<false>(a, bid, lds);
+        pairs_run<false, true>(a, bid, lds);
+}
+
+__device__ void OpPairsBatched::run(const BrickBuildArgs& a, int bid, int) {
+    if (!*a.pred || bid >= a.nbricks)
+        return;
+    __shared__ PairsLds lds;
+    if (a.cap <= kOutCap)
+        pairs_run<true, false>(a, bid, lds);
+    else
+        pairs_run<false, false>(a, bid, lds);
 }
 
 // Brick-mode export: decode halo rows back to global slots, emit tag pairs a<b.
@@ -1643,7 +1670,7 @@ void nl_batch_launch_rebuild(atvh_batch* b) {
     nl_batch_one<OpScatter>(b, "nl_scatter", 3);
     nl_batch_one<OpCellOrder>(b, "nl_cell_order", 4);
     nl_batch_one<OpPermuteHalo>(b, "nl_permute_halo", 5);
-    nl_batch_one<OpPairs>(b, "nl_pairs", 6);
+    nl_batch_one<OpPairsBatched>(b, "nl_pairs", 6);
     ATVH_HIP(hipGetLastError());
 }
 
