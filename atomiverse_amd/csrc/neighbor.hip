@@ -633,6 +633,28 @@ __device__ __forceinline__ bool band_pair(const double* __restrict__ x, const do
     return exact_pair(x, y, z, rc_sq, gi, gj, kind, shx, shy, shz);
 }
 
+// band_pair with everything but the brick b, the home cell's packed
+// brick-local coordinates, the candidate row and the home slot taken from the
+// argument block and recomputed here (the brick geometry as at the brick's
+// start). The caller passes the block through an opaque pointer, so none of
+// these values is hoisted out of the candidate loop for this rare branch.
+__device__ __forceinline__ bool band_pair_args(const BrickBuildArgs& ab, const int* h_off, int b, int lpack, int q,
+                                            int gi) {
+    const BrickGeom& g = ab.g;
+    int bx, by, bz;
+    brick_coords(g, b, bx, by, bz);
+    const int o0 = bx * g.bdim[0], o1 = by * g.bdim[1], o2 = bz * g.bdim[2];
+    const int d0 = min(g.bdim[0], g.nc[0] - o0);
+    const int d1 = min(g.bdim[1], g.nc[1] - o1), d2 = min(g.bdim[2], g.nc[2] - o2);
+    const int HY = d1 + 2 * g.R[1], HZ = d2 + 2 * g.R[2];
+    const int ncell = (d0 + 2 * g.R[0]) * HY * HZ;
+    const int lx = lpack & 0xff, ly = (lpack >> 8) & 0xff, lz = lpack >> 16;
+    const int* rows = ab.halo_rows + (int64_t)b * kHaloCap;
+    return band_pair(ab.x, ab.y, ab.z, ab.rc_sq, g.L[0], g.L[1], g.L[2], g.nc[0], g.nc[1], g.nc[2], h_off, ncell, HY,
+                     HZ, q, lx + g.R[0], ly + g.R[1], lz + g.R[2], o0 + lx, o1 + ly, o2 + lz, gi,
+                     rows[q] & kRowSlotMask);
+}
+
 // Inclusive prefix sum over the 64 lanes (DPP row shifts, then the row
 // broadcasts of lanes 15 and 31).
 __device__ __forceinline__ int wave_inclusive_sum(int x) {
@@ -863,11 +885,22 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int t, PairsL
                             // pointers held in scalar registers through it for this
                             // rare branch: C1 rebuild 689 -> 636 us)
                             int gi = a0 + li;
-                            if constexpr (OPAQUE_SLOT)
+                            if constexpr (OPAQUE_SLOT) {
+                                // and the argument block too: the band test reads it
+                                // from the kernel-argument segment (the block is the
+                                // only argument of k_single<OpPairs>, at offset 0;
+                                // taking the address of the argument itself would copy
+                                // it to scratch) and recomputes its geometry
+                                // (band_pair_args)
                                 asm volatile("" : "+v"(gi));
-                            e = band_pair(a.x, a.y, a.z, a.rc_sq, Lx, Ly, Lz, a.g.nc[0], a.g.nc[1], a.g.nc[2],
-                                          L.h_off, ncell, HY, HZ, q, lx + R0, ly + R1, hz0, cx, cy, cz, gi,
-                                          rows[q] & kRowSlotMask);
+                                auto kp = __builtin_amdgcn_kernarg_segment_ptr();
+                                asm volatile("" : "+s"(kp));
+                                e = band_pair_args(*(const BrickBuildArgs*)kp, L.h_off, b, L.cells[c].w, q, gi);
+                            } else {
+                                e = band_pair(a.x, a.y, a.z, a.rc_sq, Lx, Ly, Lz, a.g.nc[0], a.g.nc[1], a.g.nc[2],
+                                              L.h_off, ncell, HY, HZ, q, lx + R0, ly + R1, hz0, cx, cy, cz, gi,
+                                              rows[q] & kRowSlotMask);
+                            }
                         }
                         m |= __builtin_amdgcn_ballot_w64(e);
                     }
