@@ -676,9 +676,9 @@ struct OpPairs {
 };
 
 // The same build for the batched rebuild stage. Its kernel keeps more values
-// live (the active-member search), and there the opaque home slot of the
-// band branch (see pairs_run) costs vector spills instead of saving scalar
-// ones (ladder 3.06e9 -> 2.68e9 atom-steps/s), so it keeps the plain form.
+// live (the active-member search), and there the opaque values of the band
+// branch (see pairs_run) cost vector spills instead of saving scalar ones
+// (ladder 3.06e9 -> 2.68e9 atom-steps/s), so it keeps the plain form.
 struct OpPairsBatched : OpPairs {
     static __device__ void run(const BrickBuildArgs& a, int bid, int nbk);
 };
@@ -702,7 +702,15 @@ struct PairsLds {
 // One workgroup builds the rows of one brick (t-th in XCD-aware order). The
 // grid is the brick count, so no kernel-wide value has to survive from one
 // brick to the next through the candidate loop.
-template <bool STAGED, bool OPAQUE_SLOT>
+// BAND: how the rare band test reaches its values. kBandKernarg (k_single):
+// the home slot and the argument block are opaque to the compiler, the block
+// read from the kernel-argument segment; kBandPlain: the values as computed
+// for the brick (the batched stage: the same barriers through the block's
+// own address there cost vector spills on the per-pass path, ladder 3.06e9 ->
+// 2.72e9 atom-steps/s).
+constexpr int kBandPlain = 0, kBandKernarg = 1;
+
+template <bool STAGED, int BAND>
 __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int t, PairsLds& L) {
     constexpr int kWaves = OpPairs::kWaves;
     // wave-uniform values go through readfirstlane so their loops run on
@@ -879,13 +887,13 @@ __device__ __forceinline__ void pairs_run(const BrickBuildArgs& a, int t, PairsL
                     if (band) { // the reference predicate for the band lanes
                         bool e = false;
                         if (__builtin_amdgcn_inverse_ballot_w64(band)) {
-                            // with OPAQUE_SLOT the home atom's slot is made opaque
+                            // with kBandKernarg the home atom's slot is made opaque
                             // here, so the compiler cannot hoist its coordinate
                             // addresses out of the candidate loop (six 64-bit
                             // pointers held in scalar registers through it for this
                             // rare branch: C1 rebuild 689 -> 636 us)
                             int gi = a0 + li;
-                            if constexpr (OPAQUE_SLOT) {
+                            if constexpr (BAND == kBandKernarg) {
                                 // and the argument block too: the band test reads it
                                 // from the kernel-argument segment (the block is the
                                 // only argument of k_single<OpPairs>, at offset 0;
@@ -1016,9 +1024,9 @@ __device__ void OpPairs::run(const BrickBuildArgs& a, int bid, int) {
         return;
     __shared__ PairsLds lds;
     if (a.cap <= kOutCap)
-        pairs_run<true, true>(a, bid, lds);
+        pairs_run<true, kBandKernarg>(a, bid, lds);
     else
-        pairs_run<false, true>(a, bid, lds);
+        pairs_run<false, kBandKernarg>(a, bid, lds);
 }
 
 __device__ void OpPairsBatched::run(const BrickBuildArgs& a, int bid, int) {
@@ -1026,9 +1034,9 @@ __device__ void OpPairsBatched::run(const BrickBuildArgs& a, int bid, int) {
         return;
     __shared__ PairsLds lds;
     if (a.cap <= kOutCap)
-        pairs_run<true, false>(a, bid, lds);
+        pairs_run<true, kBandPlain>(a, bid, lds);
     else
-        pairs_run<false, false>(a, bid, lds);
+        pairs_run<false, kBandPlain>(a, bid, lds);
 }
 
 // Brick-mode export: decode halo rows back to global slots, emit tag pairs a<b.
