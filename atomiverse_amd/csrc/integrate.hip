@@ -74,46 +74,92 @@ __device__ __forceinline__ void block_sum_to(double v, double* out) {
     }
 }
 
+// The kicks: one atom per grid-stride step, its loads issued before its
+// stores. A one-type frame takes dtfm and mass from scalars (no dependent
+// per-atom table loads). (Two atoms per step, both loaded before either is
+// stored, needed 142 registers for the initial kick and lost on the batched
+// ladder what it gained on the 2.048M-atom fluid.)
+struct KickAtom {
+    double vx, vy, vz, fx, fy, fz;
+    int t;
+};
+
+__device__ __forceinline__ KickAtom kick_load(const IntArgs& a, int64_t i, bool typed) {
+    KickAtom k;
+    k.t = typed ? a.type[i] : 0;
+    k.vx = a.vx[i];
+    k.vy = a.vy[i];
+    k.vz = a.vz[i];
+    k.fx = a.fx[i];
+    k.fy = a.fy[i];
+    k.fz = a.fz[i];
+    return k;
+}
+
+// v <- (v s) + dtfm f (the reference's order: the pending NH scale, then the
+// kick); returns m v^2 of the new velocity
+__device__ __forceinline__ double kick_apply(const IntArgs& a, KickAtom& k, double s, bool typed, double dtfm0,
+                                             double m0) {
+    double dtfm = dtfm0, m = m0;
+    if (typed) {
+        dtfm = a.dtfm[k.t];
+        m = a.mass[k.t];
+    }
+    if (a.thermostat) {
+        k.vx *= s;
+        k.vy *= s;
+        k.vz *= s;
+    }
+    k.vx += dtfm * k.fx;
+    k.vy += dtfm * k.fy;
+    k.vz += dtfm * k.fz;
+    return m * (k.vx * k.vx + k.vy * k.vy + k.vz * k.vz);
+}
+
 struct OpInitial {
     using Args = IntArgs;
+    struct Pos {
+        double x, y, z, xs, ys, zs;
+    };
+    static __device__ __forceinline__ Pos load_pos(const IntArgs& a, int64_t i) {
+        return Pos{a.x[i], a.y[i], a.z[i], a.xs[i], a.ys[i], a.zs[i]};
+    }
+    // kick, drift, store; the displacement test of the new position
+    static __device__ __forceinline__ void finish(const IntArgs& a, int64_t i, KickAtom& k, const Pos& p, double s,
+                                                  bool typed, double dtfm0, double m0, double& mv2, int& moved) {
+        mv2 += kick_apply(a, k, s, typed, dtfm0, m0);
+        // drift; the periodic wrap of wrap_orthorhombic is deferred to the next
+        // neighbor rebuild (and to downloads), which keeps the image of every
+        // listed pair fixed between rebuilds
+        const double x = p.x + a.dt * k.vx;
+        const double y = p.y + a.dt * k.vy;
+        const double z = p.z + a.dt * k.vz;
+        a.x[i] = x;
+        a.y[i] = y;
+        a.z[i] = z;
+        a.vx[i] = k.vx;
+        a.vy[i] = k.vy;
+        a.vz[i] = k.vz;
+        double dx = x - p.xs, dy = y - p.ys, dz = z - p.zs;
+        dx -= a.lx * nearbyint(dx / a.lx);
+        dy -= a.ly * nearbyint(dy / a.ly);
+        dz -= a.lz * nearbyint(dz / a.lz);
+        const double dsq = dx * dx + dy * dy + dz * dz;
+        if (dsq * 4.0 > a.skin_sq)
+            moved = 1;
+    }
     static __device__ void run(const IntArgs& a, int bid, int nbk) {
         if (halted(a.ctl))
             return;
         const double s = a.thermostat ? a.ctl->nh_scale : 1.0;
+        const bool typed = a.ntypes > 1;
+        const double dtfm0 = a.dtfm[0], m0 = a.mass[0];
         double mv2 = 0.0;
         int moved = 0;
         for (int64_t i = bid * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)nbk * blockDim.x) {
-            const int t = a.ntypes > 1 ? a.type[i] : 0;
-            const double dtfm = a.dtfm[t];
-            double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
-            if (a.thermostat) {
-                vx *= s;
-                vy *= s;
-                vz *= s;
-            }
-            vx += dtfm * a.fx[i];
-            vy += dtfm * a.fy[i];
-            vz += dtfm * a.fz[i];
-            // drift; the periodic wrap of wrap_orthorhombic is deferred to the next
-            // neighbor rebuild (and to downloads), which keeps the image of every
-            // listed pair fixed between rebuilds
-            const double x = a.x[i] + a.dt * vx;
-            const double y = a.y[i] + a.dt * vy;
-            const double z = a.z[i] + a.dt * vz;
-            a.x[i] = x;
-            a.y[i] = y;
-            a.z[i] = z;
-            a.vx[i] = vx;
-            a.vy[i] = vy;
-            a.vz[i] = vz;
-            mv2 += a.mass[t] * (vx * vx + vy * vy + vz * vz);
-            double dx = x - a.xs[i], dy = y - a.ys[i], dz = z - a.zs[i];
-            dx -= a.lx * nearbyint(dx / a.lx);
-            dy -= a.ly * nearbyint(dy / a.ly);
-            dz -= a.lz * nearbyint(dz / a.lz);
-            const double dsq = dx * dx + dy * dy + dz * dz;
-            if (dsq * 4.0 > a.skin_sq)
-                moved = 1;
+            KickAtom k0 = kick_load(a, i, typed);
+            const Pos p0 = load_pos(a, i);
+            finish(a, i, k0, p0, s, typed, dtfm0, m0, mv2, moved);
         }
         if (__any(moved) && (threadIdx.x & 63) == 0)
             a.ctl->disp_flag = 1;
@@ -123,27 +169,23 @@ struct OpInitial {
 
 struct OpFinal {
     using Args = IntArgs;
+    static __device__ __forceinline__ void finish(const IntArgs& a, int64_t i, KickAtom& k, double s, bool typed,
+                                                  double dtfm0, double m0, double& mv2) {
+        mv2 += kick_apply(a, k, s, typed, dtfm0, m0);
+        a.vx[i] = k.vx;
+        a.vy[i] = k.vy;
+        a.vz[i] = k.vz;
+    }
     static __device__ void run(const IntArgs& a, int bid, int nbk) {
         if (halted(a.ctl))
             return;
         const double s = a.thermostat ? a.ctl->nh_scale : 1.0;
+        const bool typed = a.ntypes > 1;
+        const double dtfm0 = a.dtfm[0], m0 = a.mass[0];
         double mv2 = 0.0;
         for (int64_t i = bid * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)nbk * blockDim.x) {
-            const int t = a.ntypes > 1 ? a.type[i] : 0;
-            const double dtfm = a.dtfm[t];
-            double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
-            if (a.thermostat) {
-                vx *= s;
-                vy *= s;
-                vz *= s;
-            }
-            vx += dtfm * a.fx[i];
-            vy += dtfm * a.fy[i];
-            vz += dtfm * a.fz[i];
-            a.vx[i] = vx;
-            a.vy[i] = vy;
-            a.vz[i] = vz;
-            mv2 += a.mass[t] * (vx * vx + vy * vy + vz * vz);
+            KickAtom k0 = kick_load(a, i, typed);
+            finish(a, i, k0, s, typed, dtfm0, m0, mv2);
         }
         block_sum_to(mv2, &a.ke_partials[bid]);
     }
