@@ -218,6 +218,7 @@ struct atvh_batch {
     int* nl_nb = nullptr;            // [kNlStages][nsys] block counts of the rebuild stages
     int* nl_act = nullptr;           // [kNlStages][nsys+1] this step's prefix over rebuilding members
     unsigned char* nl_flags = nullptr; // [nsys] this step's rebuild flags
+    int* nl_next = nullptr;            // work counter of the pair-build stage (k_batched_queue)
     bool typed = false;
     // members whose force call is not the batched PairLJCut stage (charged
     // LJ + Coulomb with or without PME): queued one by one inside the step

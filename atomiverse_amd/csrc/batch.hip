@@ -167,6 +167,7 @@ void batch_free_stages(atvh_batch* b) {
     dev_free(b->nl_nb);
     dev_free(b->nl_act);
     dev_free(b->nl_flags);
+    dev_free(b->nl_next);
 }
 
 } // namespace atvh

@@ -84,4 +84,34 @@ void k_batched_active(const typename Op::Args* __restrict__ args, const int* __r
     }
 }
 
+// k_batched_active with the active blocks handed out from a counter instead
+// of a fixed stride: block b first takes active block b, then each next one
+// from `next` (zeroed on the device before the launch), offset by the grid.
+// A block that draws a value past the active total leaves, so every block
+// ends. For stages whose blocks differ widely in cost.
+template <class Op>
+__global__ __launch_bounds__(op_threads<Op>::value) __attribute__((amdgpu_waves_per_eu(op_waves<Op>::value)))
+void k_batched_queue(const typename Op::Args* __restrict__ args, const int* __restrict__ act, int nsys,
+                     int* __restrict__ next) {
+    __shared__ int s_next;
+    const int total = act[nsys];
+    int t = blockIdx.x;
+    while (t < total) {
+        int lo = 0, hi = nsys - 1; // largest s with act[s] <= t: an active member
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (act[mid] <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        Op::run(args[lo], t - act[lo], act[lo + 1] - act[lo]);
+        __syncthreads(); // shared memory (and s_next) is reused
+        if (threadIdx.x == 0)
+            s_next = static_cast<int>(gridDim.x) + atomicAdd(next, 1);
+        __syncthreads();
+        t = s_next;
+    }
+}
+
 } // namespace atvh

@@ -1657,9 +1657,11 @@ void nl_batch_build(atvh_batch* b) {
     dev_free(b->nl_nb);
     dev_free(b->nl_act);
     dev_free(b->nl_flags);
+    dev_free(b->nl_next);
     dev_alloc(&b->nl_nb, nb.size());
     dev_alloc(&b->nl_act, size_t(kNlStages) * (nsys + 1));
     dev_alloc(&b->nl_flags, nsys);
+    dev_alloc(&b->nl_next, 1);
     ATVH_HIP(hipMemcpy(b->nl_nb, nb.data(), nb.size() * sizeof(int), hipMemcpyHostToDevice));
 }
 
@@ -1671,7 +1673,10 @@ constexpr int kActiveThreads = 64 * kNlStages;
 __global__ __launch_bounds__(kActiveThreads) void k_nl_active_prefix(DevCtl* const* __restrict__ ctls, int nsys,
                                                                       const int* __restrict__ nb,
                                                                       int* __restrict__ act,
-                                                                      unsigned char* __restrict__ flags) {
+                                                                      unsigned char* __restrict__ flags,
+                                                                      int* __restrict__ next) {
+    if (threadIdx.x == 0)
+        *next = 0; // the pair build's work counter (a kernel write: graph replays need no memset)
     for (int m = threadIdx.x; m < nsys; m += blockDim.x)
         flags[m] = ctls[m]->rebuild ? 1 : 0;
     __syncthreads();
@@ -1704,14 +1709,22 @@ static void nl_batch_one(atvh_batch* b, const char* name, int stage) {
 
 void nl_batch_launch_rebuild(atvh_batch* b) {
     k_nl_active_prefix<<<1, kActiveThreads, 0, b->ctx->stream>>>(b->d_ctls, static_cast<int>(b->sys.size()),
-                                                                 b->nl_nb, b->nl_act, b->nl_flags);
+                                                                 b->nl_nb, b->nl_act, b->nl_flags, b->nl_next);
     nl_batch_one<OpBin>(b, "nl_bin", 0);
     nl_batch_one<OpScanTiles>(b, "nl_scan_tiles", 1);
     nl_batch_one<OpScanSums>(b, "nl_scan_sums", 2);
     nl_batch_one<OpScatter>(b, "nl_scatter", 3);
     nl_batch_one<OpCellOrder>(b, "nl_cell_order", 4);
     nl_batch_one<OpPermuteHalo>(b, "nl_permute_halo", 5);
-    nl_batch_one<OpPairsBatched>(b, "nl_pairs", 6);
+    {
+        // the pair build: bricks differ widely in cost (member size, density),
+        // so its resident blocks take them from a shared counter
+        const BatchStage& st = b->stages.at("nl_pairs");
+        const int nsys = static_cast<int>(b->sys.size());
+        const int grid = std::max(1, std::min(st.nblocks, 2048 * kBlock / OpPairsBatched::kThreads));
+        k_batched_queue<OpPairsBatched><<<grid, OpPairsBatched::kThreads, 0, b->ctx->stream>>>(
+            static_cast<const BrickBuildArgs*>(st.args), b->nl_act + (int64_t)6 * (nsys + 1), nsys, b->nl_next);
+    }
     ATVH_HIP(hipGetLastError());
 }
 
