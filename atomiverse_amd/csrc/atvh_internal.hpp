@@ -208,6 +208,7 @@ struct BatchStage {
     void* args = nullptr;
     int* base = nullptr;
     int nblocks = 0;
+    int nmembers = 0; // entries of args (base holds nmembers + 1): the member search's range
 };
 
 struct atvh_batch {
@@ -388,6 +389,7 @@ void batch_stage(atvh_batch* b, const std::string& name, const std::vector<Args>
     ATVH_HIP(hipMemcpy(d, args.data(), args.size() * sizeof(Args), hipMemcpyHostToDevice));
     ATVH_HIP(hipMemcpy(st.base, base.data(), base.size() * sizeof(int), hipMemcpyHostToDevice));
     st.nblocks = base.back();
+    st.nmembers = static_cast<int>(args.size());
 }
 
 } // namespace atvh

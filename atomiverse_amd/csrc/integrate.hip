@@ -484,7 +484,7 @@ void integ_batch_build(atvh_batch* b) {
 void integ_batch_launch(atvh_batch* b, const char* stage) {
     const BatchStage& st = b->stages.at(stage);
     hipStream_t s = b->ctx->stream;
-    const int nsys = static_cast<int>(b->sys.size());
+    const int nsys = st.nmembers; // every member has an entry in these stages
     const std::string name(stage);
     if (name == "initial")
         k_batched<OpInitial><<<st.nblocks, kBlock, 0, s>>>(static_cast<const IntArgs*>(st.args), st.base, nsys);

@@ -5,8 +5,10 @@
 // HBM budget; admitted segments stay resident until they finish. Every call to
 // atvh_server_advance admits what fits, then advances all resident segments
 // by one chunk (the smallest of the chunk length and every member's remaining
-// steps, so nobody overshoots): PairLJCut members together through one batched
-// step (batch.hip), k-space members through their own fused loop; finished
+// steps, so nobody overshoots): every segment on the brick layout goes
+// through one batched step (batch.hip; PairLJCut members share its pair
+// launch, charged members queue their own pair and k-space calls inside it),
+// the rest (small grids) through their own fused loop; finished
 // segments are downloaded to host memory and their device memory is released,
 // so later arrivals join at the next chunk. A member's trajectory does not
 // depend on what it was batched with (batch invariance), nor on how its run
@@ -131,7 +133,7 @@ void admit(atvh_server* sv, Segment& sg) {
     }
     check_rc(atvh_integrator_configure(sg.sys, g.dt, g.thermostat, g.T, g.tau, 0), c, "segment integrator");
     check_rc(atvh_nlist_build(sg.sys), c, "segment list build");
-    sg.batched = g.pair_style == 0 && sg.sys->brick_mode;
+    sg.batched = sg.sys->brick_mode;
     sg.left = g.nsteps;
 }
 

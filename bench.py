@@ -511,9 +511,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=0,
-                    help="requests of the serving measurement (24 is the recorded request set); "
-                         "0, the default, skips it")
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="requests of the serving measurement (0 skips it)")
     ap.add_argument("--ladder-steps", type=int, default=200,
                     help="timed steps of the batched 256-system ladder; 0 skips it")
     ap.add_argument("--charged-steps", type=int, default=100,

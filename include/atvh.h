@@ -357,9 +357,10 @@ int atvh_batch_ladder(atvh_batch* batch, int* member_of_slot,
  * layer over that boundary. A segment is one system and a step count.
  * Submitted segments are admitted in FIFO order while their estimated device
  * footprint fits the HBM budget, stay resident until done, and advance
- * together one chunk per atvh_server_advance (PairLJCut segments through one
- * batched step, k-space segments through their own fused loop). A segment's
- * result does not depend on what it ran beside.
+ * together one chunk per atvh_server_advance (every brick-layout segment,
+ * LJ or charged with k-space, through one batched step; small-grid segments
+ * through their own fused loop). A segment's result does not depend on what
+ * it ran beside.
  */
 #define ATVH_PAIR_LJ 0
 #define ATVH_PAIR_LJCOUL 1

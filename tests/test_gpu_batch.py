@@ -60,14 +60,22 @@ def test_batch_equals_members_alone(hip_ctx, prec, graph):
         s.close()
 
 
-def charged_members(ctx):
-    """Two LJ fluids and two charged melts with PME (1000 and 1728 ions,
-    different meshes)."""
+def charged_members(ctx, layout="two_lj"):
+    """two_lj: two LJ fluids and two charged melts with PME (1000 and 1728
+    ions, different meshes). one_lj: one LJ fluid and five charged melts, so
+    the batched pair launch's member table (PairLJCut members only) is much
+    shorter than the batch: its member search must stay inside that table."""
+    if layout == "two_lj":
+        ws = [W.lj_trimmed(900, seed=11, thermostat=1),
+              W.charged_melt(10, seed=21, name="charged_melt_1000"),
+              W.lj_trimmed(2000, seed=12, thermostat=1),
+              W.charged_melt(12, seed=22, name="charged_melt_1728")]
+    else:
+        ws = [W.lj_trimmed(900, seed=11, thermostat=1)]
+        ws += [W.charged_melt(side, seed=30 + k, name=f"charged_melt_{side ** 3}_{k}")
+               for k, side in enumerate((10, 12, 10, 12, 10))]
     out = []
-    for k, w in enumerate([W.lj_trimmed(900, seed=11, thermostat=1),
-                           W.charged_melt(10, seed=21, name="charged_melt_1000"),
-                           W.lj_trimmed(2000, seed=12, thermostat=1),
-                           W.charged_melt(12, seed=22, name="charged_melt_1728")]):
+    for k, w in enumerate(ws):
         s = av.System(ctx, w.n, w.box)
         W.upload(s, w)
         W.configure(s, w)
@@ -76,19 +84,19 @@ def charged_members(ctx):
 
 
 @pytest.mark.parametrize("prec", [av.PREC_FP64, av.PREC_MIXED])
-@pytest.mark.parametrize("graph", [False, True])
-def test_batch_with_pme_members_equals_members_alone(hip_ctx, prec, graph):
+@pytest.mark.parametrize("graph,layout", [(False, "two_lj"), (True, "two_lj"), (False, "one_lj")])
+def test_batch_with_pme_members_equals_members_alone(hip_ctx, prec, graph, layout):
     """Charged members (LJ + Coulomb + KSpacePME) advance inside the batched
     step (their force calls queued in the step; a step holding k-space
     members is not captured into a graph even when graphs are on) and equal
     their runs alone bit for bit, energies and k-space included."""
     nsteps = 60
     alone = []
-    for w, s in charged_members(hip_ctx):
+    for w, s in charged_members(hip_ctx, layout):
         s.run(nsteps, prec, energy_every=10)
         alone.append(snapshot(s, nsteps))
         s.close()
-    ms = charged_members(hip_ctx)
+    ms = charged_members(hip_ctx, layout)
     b = av.Batch(hip_ctx, [s for _, s in ms])
     b.set_graph(graph)
     b.run(nsteps // 2, prec, energy_every=10)
