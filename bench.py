@@ -185,12 +185,13 @@ def nve_section(av, W, ctx, lj_steps, melt_steps, every=100):
     return out
 
 
-def ladder_section(av, W, ctx, steps, loop_steps):
+def ladder_section(av, W, ctx, steps, loop_steps, eager=False):
     """Batched REMD-style ensemble: 256 LJ fluids (N log-uniform in [500, 20000],
     T* ladder 0.7..1.5, NVT, mixed), one batched step per launch stage (hipGraph
     replay) against a loop of plain per-system launches over the same members.
     Brick shapes come from the batched autotuner (atomiverse_amd.tuning), per
-    size class; the loop uses the same shapes."""
+    size class; the loop uses the same shapes. eager: the batched steps run
+    without graph replay on the library's default shapes (for kernel traces)."""
     from atomiverse_amd.tuning import autotune_batch
     ws = W.lj_ladder(256)
     total = sum(w.n for w in ws)
@@ -210,14 +211,18 @@ def ladder_section(av, W, ctx, steps, loop_steps):
                 s.nlist_build()
         return out
 
-    log(f"bench: ladder batched ({len(ws)} systems, {total} atoms), {steps} steps")
+    log(f"bench: ladder batched ({len(ws)} systems, {total} atoms), {steps} steps"
+        + (" without graph replay, default brick shapes" if eager else ""))
     ms = make()
-    tuned = autotune_batch(ctx, ms, av.PREC_MIXED)
-    shapes.extend(s.brick_shape() for s in ms)
-    log(f"  autotuned brick shapes: step {tuned['step_ms_before']:.3f} -> {tuned['step_ms_after']:.3f} ms; "
-        + ", ".join(f"{c['class_atoms']}->{c['brick']}" for c in tuned["classes"]))
+    if eager:
+        tuned = None
+    else:
+        tuned = autotune_batch(ctx, ms, av.PREC_MIXED)
+        shapes.extend(s.brick_shape() for s in ms)
+        log(f"  autotuned brick shapes: step {tuned['step_ms_before']:.3f} -> {tuned['step_ms_after']:.3f} ms; "
+            + ", ".join(f"{c['class_atoms']}->{c['brick']}" for c in tuned["classes"]))
     b = av.Batch(ctx, ms)
-    b.set_graph(True)
+    b.set_graph(not eager)
     b.run(20, av.PREC_MIXED)  # warm (graph capture, list growth)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -253,7 +258,7 @@ def ladder_section(av, W, ctx, steps, loop_steps):
         s.close()
     log(f"  loop: {dt_l * 1e3 / loop_steps:.3f} ms/step, {total * loop_steps / dt_l:.4e} atom-steps/s")
     return {"workload": "lj_ladder_256 (N log-uniform 500..20000, T* 0.7..1.5, NVT, mixed)",
-            "atoms": total, "systems": len(ws), "autotuned_brick_shapes": tuned,
+            "atoms": total, "systems": len(ws), "graph_replay": not eager, "autotuned_brick_shapes": tuned,
             "batched_atom_steps_per_s": total * steps / dt_b, "batched_steps": steps,
             "eager_us_per_step_with_events": stages,
             "loop_atom_steps_per_s": total * loop_steps / dt_l, "loop_steps": loop_steps,
@@ -515,6 +520,9 @@ def main():
                     help="requests of the serving measurement (0 skips it)")
     ap.add_argument("--ladder-steps", type=int, default=200,
                     help="timed steps of the batched 256-system ladder; 0 skips it")
+    ap.add_argument("--ladder-eager", action="store_true",
+                    help="run the ladder's batched steps without graph replay and with the default brick "
+                         "shapes (no autotuning, which replays graphs), e.g. under a kernel tracer")
     ap.add_argument("--charged-steps", type=int, default=100,
                     help="timed steps of the charged-melt (PME) side measurement; 0 skips it")
     ap.add_argument("--nve-steps", type=int, default=10000,
@@ -613,7 +621,8 @@ def main():
     if args.nve_steps > 0 or args.nve_charged_steps > 0:
         out["nve_drift"] = nve_section(av, W, ctx, args.nve_steps, args.nve_charged_steps)
     if args.ladder_steps > 0:
-        out["batched_ladder"] = ladder_section(av, W, ctx, args.ladder_steps, max(20, args.ladder_steps // 5))
+        out["batched_ladder"] = ladder_section(av, W, ctx, args.ladder_steps, max(20, args.ladder_steps // 5),
+                                               eager=args.ladder_eager)
     if args.serve_requests > 0:
         out["serving"] = serving_section(av, W, ctx, args.serve_requests, 250)
     if args.config_steps > 0:
