@@ -209,6 +209,7 @@ struct BatchStage {
     int* base = nullptr;
     int nblocks = 0;
     int nmembers = 0; // entries of args (base holds nmembers + 1): the member search's range
+    int* owner = nullptr; // [nblocks] member of every block (one load instead of a search)
 };
 
 struct atvh_batch {
@@ -379,9 +380,16 @@ void batch_stage(atvh_batch* b, const std::string& name, const std::vector<Args>
     BatchStage& st = b->stages[name];
     dev_free(st.args);
     dev_free(st.base);
+    dev_free(st.owner);
     std::vector<int> base(blocks.size() + 1, 0);
     for (size_t k = 0; k < blocks.size(); ++k)
         base[k + 1] = base[k] + blocks[k];
+    std::vector<int> owner(std::max(base.back(), 1), 0);
+    for (size_t k = 0; k < blocks.size(); ++k)
+        for (int b = base[k]; b < base[k + 1]; ++b)
+            owner[b] = static_cast<int>(k);
+    dev_alloc(&st.owner, owner.size());
+    ATVH_HIP(hipMemcpy(st.owner, owner.data(), owner.size() * sizeof(int), hipMemcpyHostToDevice));
     Args* d = nullptr;
     dev_alloc(&d, args.size());
     st.args = d;

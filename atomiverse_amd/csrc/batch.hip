@@ -162,6 +162,7 @@ void batch_free_stages(atvh_batch* b) {
     for (auto& kv : b->stages) {
         dev_free(kv.second.args);
         dev_free(kv.second.base);
+        dev_free(kv.second.owner);
     }
     b->stages.clear();
     dev_free(b->nl_nb);

@@ -4,8 +4,9 @@
 // stand for blockIdx.x/gridDim.x of a launch over one system. The
 // single-system kernel calls it with the hardware indices; the batched
 // kernel gives every system a contiguous range of blocks (prefix table
-// `base`, nsys+1 entries), finds its system by binary search and calls the
-// same body with the system's own argument block. Every system therefore
+// `base`, nsys+1 entries), reads its system from a per-block table (`owner`,
+// built with the stage) and calls the same body with the system's own
+// argument block. Every system therefore
 // runs exactly the code (and the reduction order) it runs alone.
 #pragma once
 
@@ -44,16 +45,10 @@ void k_single(typename Op::Args a) {
 
 template <class Op>
 __global__ __launch_bounds__(op_threads<Op>::value) __attribute__((amdgpu_waves_per_eu(op_waves<Op>::value)))
-void k_batched(const typename Op::Args* __restrict__ args, const int* __restrict__ base, int nsys) {
+void k_batched(const typename Op::Args* __restrict__ args, const int* __restrict__ base,
+               const int* __restrict__ owner) {
     const int blk = blockIdx.x;
-    int lo = 0, hi = nsys - 1; // largest s with base[s] <= blk
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (base[mid] <= blk)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
+    const int lo = owner[blk]; // the member whose range [base[lo], base[lo + 1]) holds blk
     // by reference: the argument block stays in global memory (a private copy
     // would be indexed dynamically for per-type tables and land in scratch)
     Op::run(args[lo], blk - base[lo], base[lo + 1] - base[lo]);

@@ -484,17 +484,17 @@ void integ_batch_build(atvh_batch* b) {
 void integ_batch_launch(atvh_batch* b, const char* stage) {
     const BatchStage& st = b->stages.at(stage);
     hipStream_t s = b->ctx->stream;
-    const int nsys = st.nmembers; // every member has an entry in these stages
+    const int* own = st.owner;
     const std::string name(stage);
     if (name == "initial")
-        k_batched<OpInitial><<<st.nblocks, kBlock, 0, s>>>(static_cast<const IntArgs*>(st.args), st.base, nsys);
+        k_batched<OpInitial><<<st.nblocks, kBlock, 0, s>>>(static_cast<const IntArgs*>(st.args), st.base, own);
     else if (name == "final")
-        k_batched<OpFinal><<<st.nblocks, kBlock, 0, s>>>(static_cast<const IntArgs*>(st.args), st.base, nsys);
+        k_batched<OpFinal><<<st.nblocks, kBlock, 0, s>>>(static_cast<const IntArgs*>(st.args), st.base, own);
     else if (name == "post_initial" || name == "post_final")
-        k_batched<OpPost><<<st.nblocks, kBlock, 0, s>>>(static_cast<const PostArgs*>(st.args), st.base, nsys);
+        k_batched<OpPost><<<st.nblocks, kBlock, 0, s>>>(static_cast<const PostArgs*>(st.args), st.base, own);
     else if (name == "finalize")
         k_batched<OpFinalize><<<st.nblocks, kBlock, 0, s>>>(static_cast<const FinalizeArgs*>(st.args), st.base,
-                                                             nsys);
+                                                             own);
     else
         throw Error(ATVH_E_ARG, "unknown batch stage");
     ATVH_HIP(hipGetLastError());

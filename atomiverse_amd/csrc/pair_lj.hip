@@ -780,10 +780,10 @@ void pair_batch_build(atvh_batch* b) {
 template <class Op>
 static void pair_batch_one(atvh_batch* b) {
     // the stage's table holds only the PairLJCut members (the others queue
-    // their own force calls): the member search runs over those
+    // their own force calls); every block's member comes from its owner table
     const BatchStage& st = b->stages.at("pair");
     k_batched<Op><<<st.nblocks, op_threads<Op>::value, 0, b->ctx->stream>>>(static_cast<const BrickLJArgs*>(st.args), st.base,
-                                                                         st.nmembers);
+                                                                         st.owner);
 }
 
 void pair_batch_launch(atvh_batch* b, int precision, bool ef) {
